@@ -1,0 +1,286 @@
+"""ctypes binding of the C-ABI in include/exahype_amd.h (libexahype_amd.so).
+
+Function names follow the reference kernels they replace
+(kernels::aderdg::generic::c::*, ExaHyPE/kernels/aderdg/generic/Kernels.h:85-213).
+There is no CPU fallback: if the HIP library is missing or a call fails,
+an exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libexahype_amd.so")
+
+EXA_OK = 0
+PDE_EULER = 0
+PDE_ELASTIC = 1
+TERM_FLUX, TERM_SOURCE, TERM_NCP = 1, 2, 4
+STAGE_PREDICTOR, STAGE_RIEMANN, STAGE_UPDATE, STAGE_DT = range(4)
+STAGE_NAMES = ("predictor", "riemann", "update", "dt")
+BC_PERIODIC = 0
+
+# Every symbol declared in include/exahype_amd.h.
+EXPORTED = (
+    "exa_status_string", "exa_init", "exa_solver_supported",
+    "exa_space_time_predictor_nonlinear", "exa_volume_integral_nonlinear",
+    "exa_surface_integral_nonlinear", "exa_riemann_solver_nonlinear", "exa_solution_update",
+    "exa_stable_time_step_size", "exa_batch_predictor_volume",
+    "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells", "exa_grid_upload_solution",
+    "exa_grid_download_solution", "exa_grid_init_euler", "exa_grid_step", "exa_grid_synchronize",
+    "exa_grid_time", "exa_grid_dt_history", "exa_grid_picard_iterations",
+    "exa_grid_set_profiling", "exa_grid_stage_times", "exa_grid_download_cells", "exa_grid_picard_totals",
+)
+
+
+class ExaError(RuntimeError):
+    pass
+
+
+class Solver(ctypes.Structure):
+    """exa_solver: the reference SolverType's constants."""
+    _fields_ = [("pde", ctypes.c_int), ("order", ctypes.c_int), ("num_variables", ctypes.c_int),
+                ("num_parameters", ctypes.c_int), ("terms", ctypes.c_int),
+                ("no_time_averaging", ctypes.c_int), ("cfl", ctypes.c_double)]
+
+    @property
+    def B(self):
+        return self.order + 1
+
+    @property
+    def D(self):
+        return self.num_variables + self.num_parameters
+
+
+class EulerIC(ctypes.Structure):
+    _fields_ = [("rho0", ctypes.c_double), ("amp", ctypes.c_double), ("k", ctypes.c_double * 3),
+                ("vel", ctypes.c_double * 3), ("p", ctypes.c_double), ("nmodes", ctypes.c_int),
+                ("mode_amp", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
+
+
+def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9):
+    return Solver(PDE_EULER, order, 5, npar, terms, 0, cfl)
+
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    """Load libexahype_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ExaError(f"{LIB_PATH} is missing: build it with `python -m exahype_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    S = ctypes.POINTER(Solver)
+    V = ctypes.c_void_p
+    L.exa_status_string.argtypes = [ctypes.c_int]
+    L.exa_status_string.restype = ctypes.c_char_p
+    L.exa_init.argtypes = [ctypes.c_int]
+    L.exa_solver_supported.argtypes = [S]
+    L.exa_space_time_predictor_nonlinear.argtypes = [S, _D, _D, _D, _D, _D, _D, _D, _D, ctypes.c_double, _I]
+    L.exa_volume_integral_nonlinear.argtypes = [S, _D, _D, _D]
+    L.exa_surface_integral_nonlinear.argtypes = [S, _D, _D, _D]
+    L.exa_riemann_solver_nonlinear.argtypes = [S, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
+    L.exa_solution_update.argtypes = [S, _D, _D, ctypes.c_double]
+    L.exa_stable_time_step_size.argtypes = [S, _D, _D, _D]
+    L.exa_batch_predictor_volume.argtypes = [S, ctypes.c_long, _D, _D, _D, _D, _D, ctypes.c_double, _I]
+    L.exa_grid_create.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int,
+                                  ctypes.POINTER(V)]
+    L.exa_grid_destroy.argtypes = [V]
+    L.exa_grid_num_cells.argtypes = [V]
+    L.exa_grid_num_cells.restype = ctypes.c_long
+    L.exa_grid_upload_solution.argtypes = [V, _D]
+    L.exa_grid_download_solution.argtypes = [V, _D]
+    L.exa_grid_init_euler.argtypes = [V, ctypes.POINTER(EulerIC)]
+    L.exa_grid_step.argtypes = [V, ctypes.c_int]
+    L.exa_grid_synchronize.argtypes = [V]
+    L.exa_grid_time.argtypes = [V, _D, _D, ctypes.POINTER(ctypes.c_long)]
+    L.exa_grid_dt_history.argtypes = [V, _D, ctypes.c_long]
+    L.exa_grid_picard_iterations.argtypes = [V, _I]
+    L.exa_grid_set_profiling.argtypes = [V, ctypes.c_int]
+    L.exa_grid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
+    L.exa_grid_download_cells.argtypes = [V, ctypes.c_long, ctypes.c_long, _D]
+    L.exa_grid_picard_totals.argtypes = [V, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
+    _lib = L
+    return L
+
+
+def check(status, what=""):
+    if status != EXA_OK:
+        msg = lib().exa_status_string(status).decode()
+        raise ExaError(f"{what}: {msg} (status {status})")
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dx3(dx):
+    return _f64(np.broadcast_to(np.asarray(dx, dtype=np.float64), (3,)))
+
+
+def supported(s):
+    return bool(lib().exa_solver_supported(ctypes.byref(s)))
+
+
+# ---------------------------------------------------------------------------
+# kernels::aderdg::generic::c single-cell entry points
+# ---------------------------------------------------------------------------
+def space_time_predictor_nonlinear(s, luh, dx, dt, temps=False):
+    """spaceTimePredictorNonlinear: returns dict(lQhbnd, lFhbnd, iters[, lQi, lFi, lQhi, lFhi])."""
+    B, D, V = s.B, s.D, s.num_variables
+    out = dict(lQhbnd=np.zeros(6 * B * B * D), lFhbnd=np.zeros(6 * B * B * V))
+    if temps:
+        out.update(lQi=np.zeros(B ** 4 * D), lFi=np.zeros(4 * B ** 4 * V), lQhi=np.zeros(B ** 3 * D),
+                   lFhi=np.zeros(4 * B ** 3 * V))
+    it = ctypes.c_int(0)
+    check(lib().exa_space_time_predictor_nonlinear(
+        ctypes.byref(s), _dp(out["lQhbnd"]), _dp(out["lFhbnd"]), _dp(out.get("lQi")), _dp(out.get("lFi")),
+        _dp(out.get("lQhi")), _dp(out.get("lFhi")), _dp(_f64(luh)), _dp(_dx3(dx)), float(dt),
+        ctypes.byref(it)), "spaceTimePredictorNonlinear")
+    out["iters"] = it.value
+    return out
+
+
+def volume_integral_nonlinear(s, lFhi, dx):
+    lduh = np.zeros(s.B ** 3 * s.num_variables)
+    check(lib().exa_volume_integral_nonlinear(ctypes.byref(s), _dp(lduh), _dp(_f64(lFhi)), _dp(_dx3(dx))),
+          "volumeIntegralNonlinear")
+    return lduh
+
+
+def surface_integral_nonlinear(s, lduh, lFhbnd, dx):
+    lduh = np.array(lduh, dtype=np.float64)
+    check(lib().exa_surface_integral_nonlinear(ctypes.byref(s), _dp(lduh), _dp(_f64(lFhbnd)), _dp(_dx3(dx))),
+          "surfaceIntegralNonlinear")
+    return lduh
+
+
+def riemann_solver_nonlinear(s, FL, FR, QL, QR, dt, direction):
+    FL = np.array(FL, dtype=np.float64)
+    FR = np.array(FR, dtype=np.float64)
+    check(lib().exa_riemann_solver_nonlinear(ctypes.byref(s), _dp(FL), _dp(FR), _dp(_f64(QL)), _dp(_f64(QR)),
+                                             float(dt), int(direction)), "riemannSolverNonlinear")
+    return FL, FR
+
+
+def solution_update(s, luh, lduh, dt):
+    luh = np.array(luh, dtype=np.float64)
+    check(lib().exa_solution_update(ctypes.byref(s), _dp(luh), _dp(_f64(lduh)), float(dt)), "solutionUpdate")
+    return luh
+
+
+def stable_time_step_size(s, luh, dx):
+    out = np.zeros(1)
+    check(lib().exa_stable_time_step_size(ctypes.byref(s), _dp(_f64(luh)), _dp(_dx3(dx)), _dp(out)),
+          "stableTimeStepSize")
+    return float(out[0])
+
+
+def batch_predictor_volume(s, luh, dx, dt):
+    """performPredictionAndVolumeIntegral over many cells: luh [ncells, B^3*D], dx [ncells,3]."""
+    luh = _f64(luh)
+    B, D, V = s.B, s.D, s.num_variables
+    n = luh.size // (B ** 3 * D)
+    dxa = _f64(np.broadcast_to(np.asarray(dx, dtype=np.float64), (n, 3)))
+    out = dict(lQhbnd=np.zeros(n * 6 * B * B * D), lFhbnd=np.zeros(n * 6 * B * B * V),
+               lduh=np.zeros(n * B ** 3 * V), iters=np.zeros(n, dtype=np.int32))
+    check(lib().exa_batch_predictor_volume(ctypes.byref(s), n, _dp(out["lQhbnd"]), _dp(out["lFhbnd"]),
+                                           _dp(out["lduh"]), _dp(luh), _dp(dxa), float(dt),
+                                           out["iters"].ctypes.data_as(_I)), "batch predictor")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Device-resident grid
+# ---------------------------------------------------------------------------
+class Grid:
+    """A uniform periodic grid level resident in HBM (exa_grid)."""
+
+    def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), boundary=BC_PERIODIC):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.s = s
+        self.n = tuple(n)
+        h = ctypes.c_void_p()
+        check(lib().exa_grid_create(ctypes.byref(s), n[0], n[1], n[2], _dp(_dx3(lo)), _dp(_dx3(hi)), boundary,
+                                    ctypes.byref(h)), "exa_grid_create")
+        self._h = h
+        self.ncells = lib().exa_grid_num_cells(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().exa_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_euler(self, rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=0,
+                   mode_amp=0.0, seed=0):
+        ic = EulerIC(rho0, amp, (ctypes.c_double * 3)(*k), (ctypes.c_double * 3)(*vel), p, nmodes, mode_amp,
+                     seed)
+        check(lib().exa_grid_init_euler(self._h, ctypes.byref(ic)), "exa_grid_init_euler")
+
+    def upload(self, luh):
+        check(lib().exa_grid_upload_solution(self._h, _dp(_f64(luh))), "upload")
+
+    def solution(self):
+        out = np.zeros(self.ncells * self.s.B ** 3 * self.s.D)
+        check(lib().exa_grid_download_solution(self._h, _dp(out)), "download")
+        return out
+
+    def cells(self, first, count):
+        out = np.zeros(count * self.s.B ** 3 * self.s.D)
+        check(lib().exa_grid_download_cells(self._h, int(first), int(count), _dp(out)), "download cells")
+        return out
+
+    def picard_totals(self):
+        it, nc = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(lib().exa_grid_picard_totals(self._h, ctypes.byref(it), ctypes.byref(nc)), "picard totals")
+        return it.value, nc.value
+
+    def step(self, nsteps):
+        check(lib().exa_grid_step(self._h, int(nsteps)), "exa_grid_step")
+
+    def synchronize(self):
+        check(lib().exa_grid_synchronize(self._h), "synchronize")
+
+    def time(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
+        check(lib().exa_grid_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "time")
+        return t.value, dt.value, n.value
+
+    def dt_history(self, n):
+        out = np.zeros(n)
+        check(lib().exa_grid_dt_history(self._h, _dp(out), n), "dt history")
+        return out
+
+    def picard_iterations(self):
+        out = np.zeros(self.ncells, dtype=np.int32)
+        check(lib().exa_grid_picard_iterations(self._h, out.ctypes.data_as(_I)), "picard iterations")
+        return out
+
+    def set_profiling(self, on=True):
+        check(lib().exa_grid_set_profiling(self._h, int(on)), "profiling")
+
+    def stage_times(self):
+        ms = np.zeros(4)
+        n = (ctypes.c_long * 4)()
+        check(lib().exa_grid_stage_times(self._h, _dp(ms), n), "stage times")
+        return {STAGE_NAMES[i]: (float(ms[i]), int(n[i])) for i in range(4)}

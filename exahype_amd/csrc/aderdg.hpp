@@ -1,0 +1,124 @@
+// aderdg.hpp -- internal interface between the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace exa {
+
+enum : int { kPdeEuler = 0, kPdeElastic = 1 };
+// Terms of the PDE (the spec file's `terms`): flux, algebraic source, NCP.
+enum : int { kFlux = 1, kSource = 2, kNCP = 4 };
+
+// Runtime description of a solver instantiation (the reference's SolverType
+// constants + the spec file's `terms`).
+struct SolverDesc {
+  int pde;
+  int order;
+  int nvar;
+  int npar;
+  int terms;  // kFlux | kSource | kNCP
+  int no_time_averaging;
+  double cfl;
+};
+
+// A face between two cells; `left` owns face 2*dir+1, `right` owns face 2*dir.
+struct FaceDesc {
+  int left;
+  int right;
+  int dir;
+  int pad;
+};
+
+constexpr int kDtSlots = 4096;
+constexpr unsigned long long kDtSlotEmpty = 0x7FF0000000000000ull;  // +inf
+
+// Device-resident time-stepping state of one grid.
+struct StepState {
+  double dt;  // current (predictor = corrector) time step size
+  double t;   // time stamp of the current solution
+  long steps;
+  unsigned long long picard_total;  // sum of per-cell Picard counts (profiling only)
+  unsigned long long predictor_cells;
+  double* dt_hist;  // dt of every step (device), may be null
+  long dt_hist_len;
+  unsigned long long dt_slots[kDtSlots];  // per-slot minima of the next dt
+};
+
+struct PredictorArgs {
+  const double* luh;  // [cells][B^3][V+P]
+  double* lduh;       // [cells][B^3][V]  (volume integral; may be null)
+  double* lQhbnd;     // [cells][6][B^2][V+P]
+  double* lFhbnd;     // [cells][6][B^2][V]
+  const double* dx;   // dx[cell * dx_stride + d]
+  long dx_stride;     // 0: one dx for all cells, 3: per cell
+  const double* dt;   // device scalar
+  int* iters;         // [cells] Picard iterations, may be null
+  long cell0;         // first cell of this launch
+  // optional exports of the reference's temporaries (single-cell API only)
+  double* lQi;   // (z,y,x,t,var) B^4 (V+P)
+  double* lFi;   // (t,z,y,x,{x,y,z,src},var) 4 B^4 V
+  double* lQhi;  // (z,y,x,var)
+  double* lFhi;  // 4 blocks of B^3 V
+};
+
+struct RiemannArgs {
+  const FaceDesc* faces;
+  long nfaces;
+  const double* lQhbnd;
+  double* lFhbnd;
+};
+
+struct UpdateArgs {
+  double* luh;
+  double* lduh;
+  const double* lFhbnd;
+  const double* dx;
+  long dx_stride;
+  const double* dt;
+  double cfl_pnpm;                // CFL * PNPM[order]
+  unsigned long long* dt_slots;   // null: no time step computation
+  double* dt_cell;                // [cells] per-cell stable dt, may be null
+  long cell0;
+};
+
+struct VolumeArgs {
+  double* lduh;
+  const double* lFhi;
+  const double* dx;
+  long dx_stride;
+};
+
+constexpr int kMaxModes = 8;
+
+struct ICArgs {
+  double* luh;
+  long ncells;
+  int nx, ny, nz;
+  int order, nvar, npar;
+  double lo[3];
+  double dx[3];
+  // rho = rho0 + amp sin(2 pi k.x) + sum_m mode_amp sin(2 pi k_m.x + phi_m0),
+  // u_i = vel_i + sum_m mode_amp sin(2 pi k_m.x + phi_mi), p constant.
+  double rho0, amp, k[3], vel[3], p;
+  int nmodes;
+  double mode_amp;
+  double mode_k[kMaxModes][3];
+  double mode_phase[kMaxModes][4];
+};
+
+bool supported(const SolverDesc& sd);
+hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a,
+                            long ncells, hipStream_t s);
+hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s);
+// update_kernel modes: surface integral, solution update, write lduh back.
+enum : int { kUpdSurface = 1, kUpdUpdate = 2, kUpdWriteLduh = 4 };
+hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, long ncells,
+                         hipStream_t s);
+hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s);
+hipError_t launch_volume_integral(const SolverDesc& sd, const VolumeArgs& a, long ncells, hipStream_t s);
+hipError_t launch_euler_ic(const ICArgs& a, hipStream_t s);
+// st->picard_total += sum(iters[0..ncells)), st->predictor_cells += ncells
+hipError_t launch_count_iterations(const int* iters, long ncells, StepState* st, hipStream_t s);
+
+}  // namespace exa

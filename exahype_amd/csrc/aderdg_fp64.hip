@@ -1,0 +1,819 @@
+// aderdg_fp64.hip -- FP64 ADER-DG time-step kernels for gfx950 (CDNA4).
+//
+// One workgroup per cell for the cell-local stages, one thread per spatial
+// node (z,y,x) of the B^3 Gauss-Legendre nodes, padded to whole 64-wide
+// wavefronts.  A thread keeps its node's space-time predictor for all B time
+// nodes in registers; the directional contractions read one time slice of
+// fluxes that the workgroup stages in LDS.  All HBM arrays use the
+// reference's per-cell layouts (variable innermost), so a cell is one
+// contiguous, coalesced block.
+//
+// Reference semantics followed (K = ExaHyPE/kernels/aderdg/generic/c/3d):
+//   predictor_kernel   K/spaceTimePredictorNonlinear.cpph:42-528,661-717
+//                      + K/volumeIntegralNonlinear.cpph:136-225 (fused epilogue)
+//   riemann_kernel     K/riemannSolverNonlinear.cpph:37-117
+//   update_kernel      K/surfaceIntegralNonlinear.cpph:21-90, K/solutionUpdate.cpph:23-47,
+//                      K/stableTimeStepSize.cpph:21-63 (per-cell minimum)
+//   dt_finalize_kernel global min of the cell time steps (mappings/SolutionUpdate.cpp:269-327)
+#include "aderdg.hpp"
+#include "exa_device.hpp"
+
+namespace exa {
+
+// S = algebraicSource(Q) - NCP(Q, gradQ) at node (z,y,x) of time slice whose
+// variables are staged in sQ[var][node] (spaceTimePredictorNonlinear.cpph:238-266).
+template <int B, class PDE, bool USE_SOURCE, bool USE_NCP>
+__device__ inline void source_ncp(const double* Q, const double* sQ, const double* sDu,
+                                  const double invDx[3], int z, int y, int x, double* S) {
+  constexpr int V = PDE::V;
+  constexpr int B2 = B * B, B3 = B2 * B;
+  if (USE_SOURCE) {
+    PDE::algebraic_source(Q, S);
+  } else {
+#pragma unroll
+    for (int m = 0; m < V; ++m) S[m] = 0.0;
+  }
+  if (USE_NCP) {
+    // gradQ(dir, var) = sum_n invDx * lQi(..n..) * dudx[l][n] (:170-173, 199-202, 228-231)
+    double gradQ[3][V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+      for (int n = 0; n < B; ++n) {
+        gx += 1.0 * invDx[0] * sQ[m * B3 + (z * B + y) * B + n] * sDu[x * B + n];
+        gy += 1.0 * invDx[1] * sQ[m * B3 + (z * B + n) * B + x] * sDu[y * B + n];
+        gz += 1.0 * invDx[2] * sQ[m * B3 + (n * B + y) * B + x] * sDu[z * B + n];
+      }
+      gradQ[0][m] = gx;
+      gradQ[1][m] = gy;
+      gradQ[2][m] = gz;
+    }
+    double ncp[V];
+    PDE::ncp(Q, gradQ, ncp);
+#pragma unroll
+    for (int m = 0; m < V; ++m) S[m] -= ncp[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Space-time predictor + time averaging + extrapolation + volume integral.
+// ---------------------------------------------------------------------------
+template <int B, class PDE, int TERMS, bool EXPORT>
+__global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArgs a) {
+  constexpr int V = PDE::V, P = PDE::P, D = V + P;
+  constexpr int N = B - 1;
+  constexpr int B2 = B * B, B3 = B2 * B;
+  constexpr int NT = node_threads(B);
+  constexpr bool USE_FLUX = (TERMS & kFlux) != 0;
+  constexpr bool USE_SOURCE = (TERMS & kSource) != 0;
+  constexpr bool USE_NCP = (TERMS & kNCP) != 0;
+  constexpr bool USE_SRC_NCP = USE_SOURCE || USE_NCP;
+  constexpr int MAX_ITER = 2 * B;  // spaceTimePredictorNonlinear.cpph:85
+  // LDS: one slice of fluxes [dir][var][node] (reused for lQhi / lFhi staging)
+  constexpr int SF = (3 * V > D ? 3 * V : D) * B3;
+  __shared__ double sF[SF];
+  __shared__ double sQ[USE_NCP ? V * B3 : 1];
+  __shared__ double sK[B2];   // Kxi[n][l]
+  __shared__ double sDu[B2];  // dudx[l][n]
+  __shared__ double sRed[NT / kWave];
+
+  const size_t cell = (size_t)blockIdx.x + a.cell0;
+  const int tid = threadIdx.x;
+  const bool active = tid < B3;
+  const int node = active ? tid : 0;
+  const int z = node / B2, y = (node / B) % B, x = node % B;
+
+  for (int i = tid; i < B2; i += NT) {
+    sK[i] = exa_kxi[N][i / B][i % B];
+    sDu[i] = exa_dudx[N][i / B][i % B];
+  }
+
+  const double* dxp = a.dx + cell * a.dx_stride;
+  const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
+  const double dt = *a.dt;
+
+  double q0[D];
+  {
+    const double* luh = a.luh + cell * (size_t)(B3 * D) + node * D;
+#pragma unroll
+    for (int m = 0; m < D; ++m) q0[m] = luh[m];
+  }
+  const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
+  const double w3 = wz * wy * wx;  // weight of rhs0 (:130-132) and of iK1 (:286-288)
+
+  double qi[B][V];  // lQi(z,y,x,t,var) of this node, variables only
+#pragma unroll
+  for (int t = 0; t < B; ++t)
+#pragma unroll
+    for (int m = 0; m < V; ++m) qi[t][m] = q0[m];
+
+  // Picard state: qi = current iterate lQi^k, qn = next iterate lQi^{k+1}
+  // accumulated slice by slice (iK1 applied in the reference's n order).
+  double qn[B][V];
+  int iters = MAX_ITER;
+  const double iweight = 1.0 / w3;
+  __syncthreads();
+
+#pragma unroll 1
+  for (int iter = 0; iter < MAX_ITER; ++iter) {
+#pragma unroll
+    for (int l = 0; l < B; ++l)
+#pragma unroll
+      for (int m = 0; m < V; ++m) qn[l][m] = 0.0;
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      double Q[D];
+#pragma unroll
+      for (int m = 0; m < V; ++m) Q[m] = qi[t][m];
+#pragma unroll
+      for (int m = V; m < D; ++m) Q[m] = q0[m];
+      const double wt = exa_gl_weights[N][t];
+      if (USE_FLUX) {
+        double F[3][V];
+        PDE::flux(Q, F);
+        if (active) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int m = 0; m < V; ++m) sF[(d * V + m) * B3 + node] = F[d][m];
+          if (EXPORT) {
+            double* lFi = a.lFi + cell * (size_t)(4 * B3 * B * V) + (((size_t)t * B3 + node) * 4) * V;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+              for (int m = 0; m < V; ++m) lFi[d * V + m] = F[d][m];
+          }
+        }
+      }
+      if (USE_NCP && active) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = qi[t][m];
+      }
+      __syncthreads();
+
+      // rhs0 = w^3 * FLCoeff[t] * luh (:124-141)
+      double r[V];
+      const double fl_t = exa_fl_coeff[N][t];
+#pragma unroll
+      for (int m = 0; m < V; ++m) r[m] = w3 * fl_t * q0[m];
+      if (USE_FLUX) {
+        // x direction (:152-178): rhs(t,z,y,x) -= us * lFi(t,z,y,n,x-flux) * Kxi[n][x]
+        {
+          const double us = wt * wz * wy * dt * invDx[0];
+          const double* row = sF + (0 * V) * B3 + (z * B + y) * B;
+#pragma unroll 2
+          for (int n = 0; n < B; ++n) {
+            const double k = sK[n * B + x];
+#pragma unroll
+            for (int m = 0; m < V; ++m) r[m] -= us * row[m * B3 + n] * k;
+          }
+        }
+        // y direction (:181-207)
+        {
+          const double us = wt * wz * wx * dt * invDx[1];
+          const double* col = sF + (1 * V) * B3 + z * B2 + x;
+#pragma unroll 2
+          for (int n = 0; n < B; ++n) {
+            const double k = sK[n * B + y];
+#pragma unroll
+            for (int m = 0; m < V; ++m) r[m] -= us * col[m * B3 + n * B] * k;
+          }
+        }
+        // z direction (:210-236)
+        {
+          const double us = wt * wy * wx * dt * invDx[2];
+          const double* col = sF + (2 * V) * B3 + y * B + x;
+#pragma unroll 2
+          for (int n = 0; n < B; ++n) {
+            const double k = sK[n * B + z];
+#pragma unroll
+            for (int m = 0; m < V; ++m) r[m] -= us * col[m * B3 + n * B2] * k;
+          }
+        }
+      }
+      if (USE_SRC_NCP) {
+        // source and non-conservative product (:238-276)
+        double S[V];
+        source_ncp<B, PDE, USE_SOURCE, USE_NCP>(Q, sQ, sDu, invDx, z, y, x, S);
+        const double us = wt * wz * wy * wx * dt;
+#pragma unroll
+        for (int m = 0; m < V; ++m) r[m] += us * S[m];
+        if (EXPORT && active) {
+          double* lFi = a.lFi + cell * (size_t)(4 * B3 * B * V) + (((size_t)t * B3 + node) * 4 + 3) * V;
+#pragma unroll
+          for (int m = 0; m < V; ++m) lFi[m] = S[m];
+        }
+      }
+      // lQi_new(l) += iweight * rhs(t) * iK1[l][t]   (:292-299, n = t)
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const double ir = iweight * r[m];
+#pragma unroll
+        for (int l = 0; l < B; ++l) qn[l][m] += ir * exa_ik1[N][l][t];
+      }
+      __syncthreads();  // the next slice overwrites sF / sQ
+    }
+
+    // squared residual (:303) and exit test (:314-318)
+    double sq = 0.0;
+#pragma unroll
+    for (int l = 0; l < B; ++l)
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const double dq = qn[l][m] - qi[l][m];
+        sq += dq * dq;
+      }
+    if (!active) sq = 0.0;
+    const double sq_res = block_sum<NT>(sq, sRed);
+    const bool done = sq_res < 1e-7 * 1e-7;
+    if (done || iter + 1 == MAX_ITER) {
+      iters = iter + 1;
+      break;  // qi still holds lQi^k, the iterate the last fluxes were taken from
+    }
+#pragma unroll
+    for (int l = 0; l < B; ++l)
+#pragma unroll
+      for (int m = 0; m < V; ++m) qi[l][m] = qn[l][m];
+  }
+
+  // Time-averaged fluxes of the last flux evaluation, i.e. of lQi^k (the
+  // reference computes lFi before the final lQi update, :96-122 vs :306):
+  // lFhi = sum_t w_t F(lQi^k(t)), lShi likewise (:367-396).
+  double fh[3][V];
+  double sh[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    fh[0][m] = 0.0;
+    fh[1][m] = 0.0;
+    fh[2][m] = 0.0;
+    sh[m] = 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < B; ++t) {
+    double Q[D];
+#pragma unroll
+    for (int m = 0; m < V; ++m) Q[m] = qi[t][m];
+#pragma unroll
+    for (int m = V; m < D; ++m) Q[m] = q0[m];
+    const double wt = exa_gl_weights[N][t];
+    if (USE_FLUX) {
+      double F[3][V];
+      PDE::flux(Q, F);
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < V; ++m) fh[d][m] += F[d][m] * wt;
+    }
+    if (USE_SRC_NCP) {
+      if (USE_NCP && active) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = qi[t][m];
+      }
+      if (USE_NCP) __syncthreads();
+      double S[V];
+      source_ncp<B, PDE, USE_SOURCE, USE_NCP>(Q, sQ, sDu, invDx, z, y, x, S);
+#pragma unroll
+      for (int m = 0; m < V; ++m) sh[m] += S[m] * wt;
+      if (USE_NCP) __syncthreads();
+    }
+  }
+  // the final iterate lQi^{k+1}
+#pragma unroll
+  for (int l = 0; l < B; ++l)
+#pragma unroll
+    for (int m = 0; m < V; ++m) qi[l][m] = qn[l][m];
+
+  // ---- epilogue: time average, extrapolation, volume integral ----
+  const size_t cQb = cell * (size_t)(6 * B2 * D);
+  const size_t cFb = cell * (size_t)(6 * B2 * V);
+  // lQhi = sum_t lQi(t) w_t (:397-404)
+  double qh[D];
+#pragma unroll
+  for (int m = 0; m < D; ++m) {
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < B; ++t) acc += (m < V ? qi[t][m < V ? m : 0] : q0[m]) * exa_gl_weights[N][t];
+    qh[m] = acc;
+  }
+  if (EXPORT && active) {
+    double* lQi = a.lQi + cell * (size_t)(B3 * B * D) + (size_t)node * B * D;
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int m = 0; m < D; ++m) lQi[t * D + m] = (m < V) ? qi[t][m < V ? m : 0] : q0[m];
+    double* lQhi = a.lQhi + cell * (size_t)(B3 * D) + (size_t)node * D;
+#pragma unroll
+    for (int m = 0; m < D; ++m) lQhi[m] = qh[m];
+    // lFhi blocks: x (z,y,x), y (z,x,y), z (y,x,z), src (z,y,x)  (:375-393)
+    double* lFhi = a.lFhi + cell * (size_t)(4 * B3 * V);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + m] = fh[0][m];
+      lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + m] = fh[1][m];
+      lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + m] = fh[2][m];
+      lFhi[3 * B3 * V + ((z * B + y) * B + x) * V + m] = sh[m];
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < D; ++m) sF[m * B3 + node] = qh[m];
+  }
+  __syncthreads();
+  // lQhbnd (:434-527): face f, face node (a,b), datum k; reduce the normal line.
+  for (int o = tid; o < 6 * B2 * D; o += NT) {
+    const int f = o / (B2 * D);
+    const int rem = o - f * (B2 * D);
+    const int fa = rem / (B * D);
+    const int fb = (rem / D) % B;
+    const int k = rem % D;
+    const int dir = f >> 1;
+    const double* coef = (f & 1) ? exa_fr_coeff[N] : exa_fl_coeff[N];
+    // x-faces: (z=a,y=b,x=l); y-faces: (z=a,y=l,x=b); z-faces: (z=l,y=a,x=b)
+    const int base = dir == 0 ? (fa * B + fb) * B : (dir == 1 ? fa * B2 + fb : fa * B + fb);
+    const int stride = dir == 0 ? 1 : (dir == 1 ? B : B2);
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < B; ++l) acc += sF[k * B3 + base + l * stride] * coef[l];
+    a.lQhbnd[cQb + o] = acc;
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int m = 0; m < V; ++m) sF[(d * V + m) * B3 + node] = fh[d][m];
+  }
+  __syncthreads();
+  // lFhbnd from lFhi_{x,y,z} (:434-527)
+  if (USE_FLUX) {
+    for (int o = tid; o < 6 * B2 * V; o += NT) {
+      const int f = o / (B2 * V);
+      const int rem = o - f * (B2 * V);
+      const int fa = rem / (B * V);
+      const int fb = (rem / V) % B;
+      const int k = rem % V;
+      const int dir = f >> 1;
+      const double* coef = (f & 1) ? exa_fr_coeff[N] : exa_fl_coeff[N];
+      const int base = dir == 0 ? (fa * B + fb) * B : (dir == 1 ? fa * B2 + fb : fa * B + fb);
+      const int stride = dir == 0 ? 1 : (dir == 1 ? B : B2);
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < B; ++l) acc += sF[(dir * V + k) * B3 + base + l * stride] * coef[l];
+      a.lFhbnd[cFb + o] = acc;
+    }
+  } else {
+    for (int o = tid; o < 6 * B2 * V; o += NT) a.lFhbnd[cFb + o] = 0.0;
+  }
+  // volume integral (volumeIntegralNonlinear.cpph:136-225), time-averaged form
+  if (a.lduh && active) {
+    double acc[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) acc[m] = 0.0;
+    if (USE_FLUX) {
+      {
+        const double us = wz * wy * invDx[0];
+        const double* row = sF + (0 * V) * B3 + (z * B + y) * B;
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+#pragma unroll
+          for (int n = 0; n < B; ++n) acc[m] += sK[x * B + n] * row[m * B3 + n] * us;
+      }
+      {
+        const double us = wz * wx * invDx[1];
+        const double* col = sF + (1 * V) * B3 + z * B2 + x;
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+#pragma unroll
+          for (int n = 0; n < B; ++n) acc[m] += sK[y * B + n] * col[m * B3 + n * B] * us;
+      }
+      {
+        const double us = wy * wx * invDx[2];
+        const double* col = sF + (2 * V) * B3 + y * B + x;
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+#pragma unroll
+          for (int n = 0; n < B; ++n) acc[m] += sK[z * B + n] * col[m * B3 + n * B2] * us;
+      }
+    }
+    if (USE_SRC_NCP) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) acc[m] += w3 * sh[m];
+    }
+    double* lduh = a.lduh + cell * (size_t)(B3 * V) + node * V;
+#pragma unroll
+    for (int m = 0; m < V; ++m) lduh[m] = acc[m];
+  }
+  if (a.iters && tid == 0) a.iters[cell] = iters;
+}
+
+// ---------------------------------------------------------------------------
+// Rusanov flux on a list of faces.  One wavefront per face, lane = face node.
+// ---------------------------------------------------------------------------
+template <int B, class PDE, bool USE_NCP>
+__global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
+  constexpr int V = PDE::V, P = PDE::P, D = V + P;
+  constexpr int N = B - 1;
+  constexpr int B2 = B * B;
+  static_assert(B2 <= kWave, "one face per wavefront");
+  const long f = (long)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+  if (f >= a.nfaces) return;  // whole wavefront exits together
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool active = lane < B2;
+  const int node = active ? lane : 0;
+  const FaceDesc fd = a.faces[f];
+  const int d = fd.dir;
+  double* FL = a.lFhbnd + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * V);
+  double* FR = a.lFhbnd + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
+  const double* QL = a.lQhbnd + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * D);
+  const double* QR = a.lQhbnd + ((size_t)fd.right * 6 + 2 * d) * (B2 * D);
+  const int i = node / B, j = node % B;
+  const double weight = exa_gl_weights[N][i] * exa_gl_weights[N][j];
+  double qL[D], qR[D], QavL[D], QavR[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    qL[k] = QL[node * D + k];
+    qR[k] = QR[node * D + k];
+    QavL[k] = wave_sum(active ? weight * qL[k] : 0.0);  // :49-66
+    QavR[k] = wave_sum(active ? weight * qR[k] : 0.0);
+  }
+  const double sL = PDE::max_abs_eigenvalue(QavL, d);  // :68-78
+  const double sR = PDE::max_abs_eigenvalue(QavR, d);
+  const double smax = (sL < sR) ? sR : sL;
+  double ncp[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) ncp[k] = 0.0;
+  if (USE_NCP) {
+    double gradQ[3][V];
+    double Qavg[D];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      gradQ[0][k] = 0.0;
+      gradQ[1][k] = 0.0;
+      gradQ[2][k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) Qavg[k] = 0.5 * (qR[k] + qL[k]);
+#pragma unroll
+    for (int k = 0; k < V; ++k) gradQ[d][k] = qR[k] - qL[k];
+    PDE::ncp(Qavg, gradQ, ncp);
+  }
+  if (!active) return;
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const double fl = FL[node * V + k], fr = FR[node * V + k];
+    const double F = 0.5 * (fr + fl) - 0.5 * smax * (qR[k] - qL[k]);  // :102-112
+    if (USE_NCP) {
+      FR[node * V + k] = F - 0.5 * ncp[k];
+      FL[node * V + k] = F + 0.5 * ncp[k];
+    } else {
+      FR[node * V + k] = F;
+      FL[node * V + k] = F;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Surface integral + solution update + the cell's stable time step.
+// ---------------------------------------------------------------------------
+__device__ inline void atomic_min_positive(unsigned long long* slot, double v) {
+  // For non-negative doubles the IEEE bit patterns order like the values.
+  atomicMin(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+template <int B, class PDE, bool SURFACE, bool UPDATE, bool WRITE_LDUH>
+__global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
+  constexpr int V = PDE::V, P = PDE::P, D = V + P;
+  constexpr int N = B - 1;
+  constexpr int B2 = B * B, B3 = B2 * B;
+  constexpr int NT = node_threads(B);
+  __shared__ double sRed[NT / kWave];
+  const size_t cell = (size_t)blockIdx.x + a.cell0;
+  const int tid = threadIdx.x;
+  const bool active = tid < B3;
+  const int node = active ? tid : 0;
+  const int z = node / B2, y = (node / B) % B, x = node % B;
+  const double* dxp = a.dx + cell * a.dx_stride;
+  const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
+  double* luh = a.luh + cell * (size_t)(B3 * D) + node * D;
+  double q[D];
+#pragma unroll
+  for (int m = 0; m < D; ++m) q[m] = luh[m];
+  if (SURFACE || UPDATE) {
+    const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
+    double* lduhp = a.lduh + cell * (size_t)(B3 * V) + node * V;
+    double du[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) du[m] = lduhp[m];
+    if (SURFACE) {
+      const double* lF = a.lFhbnd + cell * (size_t)(6 * B2 * V);
+      {  // x faces, (z,y)
+        const double us = wz * wy * invDx[0];
+        const double fr = exa_fr_coeff[N][x], fl = exa_fl_coeff[N][x];
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+          du[m] -= (lF[(1 * B2 + z * B + y) * V + m] * fr - lF[(0 * B2 + z * B + y) * V + m] * fl) * us;
+      }
+      {  // y faces, (z,x)
+        const double us = wz * wx * invDx[1];
+        const double fr = exa_fr_coeff[N][y], fl = exa_fl_coeff[N][y];
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+          du[m] -= (lF[(3 * B2 + z * B + x) * V + m] * fr - lF[(2 * B2 + z * B + x) * V + m] * fl) * us;
+      }
+      {  // z faces, (y,x)
+        const double us = wy * wx * invDx[2];
+        const double fr = exa_fr_coeff[N][z], fl = exa_fl_coeff[N][z];
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+          du[m] -= (lF[(5 * B2 + y * B + x) * V + m] * fr - lF[(4 * B2 + y * B + x) * V + m] * fl) * us;
+      }
+    }
+    if (UPDATE) {
+      const double dt = *a.dt;
+      const double updateSize = dt / (wz * wy * wx);
+#pragma unroll
+      for (int m = 0; m < V; ++m) q[m] += du[m] * updateSize;
+      if (active) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) luh[m] = q[m];
+      }
+    }
+    if (WRITE_LDUH && active) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) lduhp[m] = du[m];
+    }
+  }
+  if (a.dt_slots || a.dt_cell) {
+    double dtn = active ? node_stable_dt<PDE>(q, invDx, a.cfl_pnpm) : 1.0e308;
+    dtn = block_min<NT>(dtn, sRed);
+    if (tid == 0) {
+      if (a.dt_cell) a.dt_cell[cell] = dtn;
+      if (a.dt_slots) atomic_min_positive(a.dt_slots + (cell % kDtSlots), dtn);
+    }
+  }
+}
+
+// dt_next = min(slots); t += dt (unless initialising); slots reset.
+__global__ __launch_bounds__(256) void dt_finalize_kernel(StepState* st, int advance_time) {
+  __shared__ double sRed[256 / kWave];
+  double m = 1.0e308;
+  for (int i = threadIdx.x; i < kDtSlots; i += 256) {
+    const double v = __longlong_as_double((long long)st->dt_slots[i]);
+    m = v < m ? v : m;
+  }
+  m = block_min<256>(m, sRed);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kDtSlots; i += 256) st->dt_slots[i] = kDtSlotEmpty;
+  if (threadIdx.x == 0) {
+    if (advance_time) {
+      st->t += st->dt;
+      st->steps += 1;
+    }
+    st->dt = m;
+    if (st->dt_hist && st->steps < st->dt_hist_len) st->dt_hist[st->steps] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stand-alone per-cell kernels for the drop-in single-cell / batch API.
+// ---------------------------------------------------------------------------
+
+// volumeIntegralNonlinear (time-averaged form) from caller-provided lFhi.
+template <int B, int V, bool USE_FLUX, bool USE_SRC_NCP>
+__global__ __launch_bounds__(node_threads(B)) void volume_integral_kernel(VolumeArgs a) {
+  constexpr int N = B - 1;
+  constexpr int B2 = B * B, B3 = B2 * B;
+  const size_t cell = blockIdx.x;
+  const int node = threadIdx.x;
+  if (node >= B3) return;
+  const int z = node / B2, y = (node / B) % B, x = node % B;
+  const double* dxp = a.dx + cell * a.dx_stride;
+  const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
+  const double* lFhi = a.lFhi + cell * (size_t)(4 * B3 * V);
+  const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
+  double acc[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) acc[m] = 0.0;
+  if (USE_FLUX) {
+    const double usx = wz * wy * invDx[0], usy = wz * wx * invDx[1], usz = wy * wx * invDx[2];
+#pragma unroll
+    for (int m = 0; m < V; ++m)
+#pragma unroll
+      for (int n = 0; n < B; ++n) acc[m] += exa_kxi[N][x][n] * lFhi[0 * B3 * V + ((z * B + y) * B + n) * V + m] * usx;
+#pragma unroll
+    for (int m = 0; m < V; ++m)
+#pragma unroll
+      for (int n = 0; n < B; ++n) acc[m] += exa_kxi[N][y][n] * lFhi[1 * B3 * V + ((z * B + x) * B + n) * V + m] * usy;
+#pragma unroll
+    for (int m = 0; m < V; ++m)
+#pragma unroll
+      for (int n = 0; n < B; ++n) acc[m] += exa_kxi[N][z][n] * lFhi[2 * B3 * V + ((y * B + x) * B + n) * V + m] * usz;
+  }
+  if (USE_SRC_NCP) {
+    const double w3 = wz * wy * wx;
+#pragma unroll
+    for (int m = 0; m < V; ++m) acc[m] += w3 * lFhi[3 * B3 * V + node * V + m];
+  }
+  double* lduh = a.lduh + cell * (size_t)(B3 * V) + node * V;
+#pragma unroll
+  for (int m = 0; m < V; ++m) lduh[m] = acc[m];
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions (evaluated at the Gauss-Legendre nodes of each cell,
+// x = centre + dx (xi - 1/2), like solutionAdjustment.cpph:32-43).
+// ---------------------------------------------------------------------------
+__global__ void euler_ic_kernel(ICArgs a) {
+  const int B = a.order + 1;
+  const int B3 = B * B * B;
+  const int D = a.nvar + a.npar;
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)a.ncells * B3;
+  if (gid >= total) return;
+  const size_t cell = gid / B3;
+  const int node = (int)(gid % B3);
+  const int iz = node / (B * B), iy = (node / B) % B, ix = node % B;
+  const int ci = (int)(cell % a.nx), cj = (int)((cell / a.nx) % a.ny), ck = (int)(cell / ((size_t)a.nx * a.ny));
+  const double X[3] = {a.lo[0] + (ci + 0.5) * a.dx[0] + a.dx[0] * (exa_gl_nodes[a.order][ix] - 0.5),
+                       a.lo[1] + (cj + 0.5) * a.dx[1] + a.dx[1] * (exa_gl_nodes[a.order][iy] - 0.5),
+                       a.lo[2] + (ck + 0.5) * a.dx[2] + a.dx[2] * (exa_gl_nodes[a.order][iz] - 0.5)};
+  const double twopi = 6.283185307179586476925286766559;
+  double rho = a.rho0 + a.amp * sin(twopi * (a.k[0] * X[0] + a.k[1] * X[1] + a.k[2] * X[2]));
+  double u[3] = {a.vel[0], a.vel[1], a.vel[2]};
+  for (int mo = 0; mo < a.nmodes; ++mo) {
+    const double arg = twopi * (a.mode_k[mo][0] * X[0] + a.mode_k[mo][1] * X[1] + a.mode_k[mo][2] * X[2]);
+    rho += a.mode_amp * sin(arg + a.mode_phase[mo][0]);
+    u[0] += a.mode_amp * sin(arg + a.mode_phase[mo][1]);
+    u[1] += a.mode_amp * sin(arg + a.mode_phase[mo][2]);
+    u[2] += a.mode_amp * sin(arg + a.mode_phase[mo][3]);
+  }
+  const double GAMMA = 1.4;
+  double* Q = a.luh + gid * D;
+  Q[0] = rho;
+  Q[1] = rho * u[0];
+  Q[2] = rho * u[1];
+  Q[3] = rho * u[2];
+  Q[4] = a.p / (GAMMA - 1.0) + 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (int m = 5; m < D; ++m) Q[m] = 0.0;
+}
+
+// Sum of the per-cell Picard counts of one predictor launch (for the roofline).
+__global__ __launch_bounds__(256) void count_iterations_kernel(const int* iters, long ncells, StepState* st) {
+  __shared__ double sRed[256 / kWave];
+  double acc = 0.0;
+  for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < ncells; c += (long)gridDim.x * 256) acc += iters[c];
+  acc = block_sum<256>(acc, sRed);
+  if (threadIdx.x == 0) {
+    atomicAdd(&st->picard_total, (unsigned long long)acc);
+    if (blockIdx.x == 0) atomicAdd(&st->predictor_cells, (unsigned long long)ncells);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side dispatch: runtime (pde, order, npar, terms) -> instantiation.
+// ---------------------------------------------------------------------------
+#define EXA_ORDERS(X) X(2) X(3) X(4) X(5) X(6) X(7)
+
+template <int B, class PDE>
+static hipError_t launch_predictor_t(int terms, bool exp, const PredictorArgs& a, long ncells,
+                                     hipStream_t s) {
+  const dim3 grid((unsigned)ncells), block(node_threads(B));
+#define EXA_PRED(T)                                                                          \
+  if (terms == (T)) {                                                                        \
+    if (exp)                                                                                 \
+      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), true>), grid, block, 0, s, a);       \
+    else                                                                                     \
+      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), false>), grid, block, 0, s, a);      \
+    return hipGetLastError();                                                                \
+  }
+  EXA_PRED(kFlux)
+  EXA_PRED(kFlux | kSource | kNCP)
+#undef EXA_PRED
+  return hipErrorNotSupported;
+}
+
+hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a,
+                            long ncells, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  if (sd.pde == kPdeEuler && sd.npar == 0) {
+#define EXA_CASE(N) \
+  if (sd.order == N) return launch_predictor_t<N + 1, Euler<0>>(sd.terms, export_temps, a, ncells, s);
+    EXA_ORDERS(EXA_CASE)
+#undef EXA_CASE
+  }
+  if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3)
+    return launch_predictor_t<4, Euler<1>>(sd.terms, export_temps, a, ncells, s);
+  return hipErrorNotSupported;
+}
+
+template <int B, class PDE>
+static hipError_t launch_riemann_t(int terms, const RiemannArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)((a.nfaces + 3) / 4)), block(256);
+  if (terms & kNCP)
+    hipLaunchKernelGGL((riemann_kernel<B, PDE, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((riemann_kernel<B, PDE, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s) {
+  if (a.nfaces <= 0) return hipSuccess;
+  if (sd.pde == kPdeEuler && sd.npar == 0) {
+#define EXA_CASE(N) \
+  if (sd.order == N) return launch_riemann_t<N + 1, Euler<0>>(sd.terms, a, s);
+    EXA_ORDERS(EXA_CASE)
+#undef EXA_CASE
+  }
+  if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3) return launch_riemann_t<4, Euler<1>>(sd.terms, a, s);
+  return hipErrorNotSupported;
+}
+
+template <int B, class PDE>
+static hipError_t launch_update_t(int mode, const UpdateArgs& a, long ncells, hipStream_t s) {
+  const dim3 grid((unsigned)ncells), block(node_threads(B));
+  switch (mode) {
+    case kUpdSurface | kUpdUpdate:
+      hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false>), grid, block, 0, s, a);
+      break;
+    case kUpdSurface | kUpdWriteLduh:
+      hipLaunchKernelGGL((update_kernel<B, PDE, true, false, true>), grid, block, 0, s, a);
+      break;
+    case kUpdUpdate:
+      hipLaunchKernelGGL((update_kernel<B, PDE, false, true, false>), grid, block, 0, s, a);
+      break;
+    case 0:
+      hipLaunchKernelGGL((update_kernel<B, PDE, false, false, false>), grid, block, 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, long ncells,
+                         hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  if (sd.pde == kPdeEuler && sd.npar == 0) {
+#define EXA_CASE(N) \
+  if (sd.order == N) return launch_update_t<N + 1, Euler<0>>(mode, a, ncells, s);
+    EXA_ORDERS(EXA_CASE)
+#undef EXA_CASE
+  }
+  if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3)
+    return launch_update_t<4, Euler<1>>(mode, a, ncells, s);
+  return hipErrorNotSupported;
+}
+
+hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s) {
+  hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, s, st, advance_time ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <int B, int V>
+static hipError_t launch_volume_t(int terms, const VolumeArgs& a, long ncells, hipStream_t s) {
+  const dim3 grid((unsigned)ncells), block(node_threads(B));
+  const bool f = terms & kFlux, sn = terms & (kSource | kNCP);
+  if (f && sn)
+    hipLaunchKernelGGL((volume_integral_kernel<B, V, true, true>), grid, block, 0, s, a);
+  else if (f)
+    hipLaunchKernelGGL((volume_integral_kernel<B, V, true, false>), grid, block, 0, s, a);
+  else if (sn)
+    hipLaunchKernelGGL((volume_integral_kernel<B, V, false, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((volume_integral_kernel<B, V, false, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_volume_integral(const SolverDesc& sd, const VolumeArgs& a, long ncells, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  if (sd.nvar == 5) {
+#define EXA_CASE(N) \
+  if (sd.order == N) return launch_volume_t<N + 1, 5>(sd.terms, a, ncells, s);
+    EXA_ORDERS(EXA_CASE)
+#undef EXA_CASE
+  }
+  return hipErrorNotSupported;
+}
+
+hipError_t launch_euler_ic(const ICArgs& a, hipStream_t s) {
+  const size_t total = (size_t)a.ncells * (a.order + 1) * (a.order + 1) * (a.order + 1);
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  hipLaunchKernelGGL(euler_ic_kernel, dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_iterations(const int* iters, long ncells, StepState* st, hipStream_t s) {
+  hipLaunchKernelGGL(count_iterations_kernel, dim3(512), dim3(256), 0, s, iters, ncells, st);
+  return hipGetLastError();
+}
+
+bool supported(const SolverDesc& sd) {
+  if (sd.pde != kPdeEuler || sd.nvar != 5) return false;
+  if (sd.no_time_averaging) return false;
+  if (sd.npar == 0) return sd.order >= 2 && sd.order <= 7 && sd.terms == kFlux;
+  if (sd.npar == 1) return sd.order == 3 && (sd.terms == kFlux || sd.terms == (kFlux | kSource | kNCP));
+  return false;
+}
+
+}  // namespace exa

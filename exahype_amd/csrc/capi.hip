@@ -1,0 +1,638 @@
+// capi.hip -- the C-ABI (include/exahype_amd.h): single-cell and batched
+// drop-ins for kernels::aderdg::generic::c, and the device-resident grid
+// driver that replaces the reference's per-cell solver loop.
+#include "../../include/exahype_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "aderdg.hpp"
+
+using exa::FaceDesc;
+using exa::SolverDesc;
+
+namespace {
+
+// PNPM[order], stableTimeStepSize.cpph:33-34
+const double kPNPM[10] = {1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015};
+
+int from_hip(hipError_t e) {
+  if (e == hipSuccess) return EXA_OK;
+  if (e == hipErrorNotSupported) return EXA_E_UNSUPPORTED;
+  if (e == hipErrorOutOfMemory) return EXA_E_NOMEM;
+  return EXA_E_HIP;
+}
+
+#define EXA_TRY(expr)                     \
+  do {                                    \
+    const hipError_t e_ = (expr);         \
+    if (e_ != hipSuccess) return from_hip(e_); \
+  } while (0)
+
+bool to_desc(const exa_solver* s, SolverDesc* d) {
+  if (!s) return false;
+  d->pde = s->pde;
+  d->order = s->order;
+  d->nvar = s->num_variables;
+  d->npar = s->num_parameters;
+  d->terms = s->terms;
+  d->no_time_averaging = s->no_time_averaging;
+  d->cfl = s->cfl;
+  return s->order >= 0 && s->order <= 9 && s->num_variables > 0 && s->num_parameters >= 0;
+}
+
+// Device buffer owned for the duration of one synchronous call.
+struct DevBuf {
+  void* p = nullptr;
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  double* d() const { return static_cast<double*>(p); }
+};
+
+struct Sizes {
+  long B, B2, B3, B4, D, V;
+  explicit Sizes(const SolverDesc& sd)
+      : B(sd.order + 1), B2(B * B), B3(B2 * B), B4(B3 * B), D(sd.nvar + sd.npar), V(sd.nvar) {}
+};
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" const char* exa_status_string(int status) {
+  switch (status) {
+    case EXA_OK: return "ok";
+    case EXA_E_HIP: return "HIP runtime error";
+    case EXA_E_UNSUPPORTED: return "no compiled instantiation for this solver";
+    case EXA_E_INVALID: return "invalid argument";
+    case EXA_E_NOMEM: return "device out of memory";
+    case EXA_E_NONFINITE: return "non-finite time step size";
+  }
+  return "unknown status";
+}
+
+extern "C" int exa_init(int device) {
+  EXA_TRY(hipSetDevice(device));
+  return EXA_OK;
+}
+
+extern "C" int exa_solver_supported(const exa_solver* s) {
+  SolverDesc sd;
+  return to_desc(s, &sd) && exa::supported(sd) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Single-cell and batched entry points
+// ---------------------------------------------------------------------------
+static int predictor_impl(const exa_solver* s, long ncells, double* lQhbnd, double* lFhbnd,
+                          double* lduh, double* lQi, double* lFi, double* lQhi, double* lFhi,
+                          const double* luh, const double* dx, double dt, int* iters) {
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || ncells < 1 || !lQhbnd || !lFhbnd || !luh || !dx) return EXA_E_INVALID;
+  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  const bool exp = lQi || lFi || lQhi || lFhi;
+  DevBuf d_luh, d_qb, d_fb, d_du, d_dx, d_dt, d_it, d_qi, d_fi, d_qhi, d_fhi;
+  EXA_TRY(d_luh.alloc(sizeof(double) * ncells * z.B3 * z.D));
+  EXA_TRY(d_qb.alloc(sizeof(double) * ncells * 6 * z.B2 * z.D));
+  EXA_TRY(d_fb.alloc(sizeof(double) * ncells * 6 * z.B2 * z.V));
+  EXA_TRY(d_du.alloc(sizeof(double) * ncells * z.B3 * z.V));
+  EXA_TRY(d_dx.alloc(sizeof(double) * ncells * 3));
+  EXA_TRY(d_dt.alloc(sizeof(double)));
+  EXA_TRY(d_it.alloc(sizeof(int) * ncells));
+  if (exp) {
+    EXA_TRY(d_qi.alloc(sizeof(double) * ncells * z.B4 * z.D));
+    EXA_TRY(d_fi.alloc(sizeof(double) * ncells * 4 * z.B4 * z.V));
+    EXA_TRY(d_qhi.alloc(sizeof(double) * ncells * z.B3 * z.D));
+    EXA_TRY(d_fhi.alloc(sizeof(double) * ncells * 4 * z.B3 * z.V));
+    EXA_TRY(hipMemset(d_fi.p, 0, sizeof(double) * ncells * 4 * z.B4 * z.V));
+    EXA_TRY(hipMemset(d_fhi.p, 0, sizeof(double) * ncells * 4 * z.B3 * z.V));
+  }
+  EXA_TRY(hipMemcpy(d_luh.p, luh, sizeof(double) * ncells * z.B3 * z.D, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dx.p, dx, sizeof(double) * ncells * 3, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dt.p, &dt, sizeof(double), hipMemcpyHostToDevice));
+  exa::PredictorArgs a{};
+  a.luh = d_luh.d();
+  a.lduh = lduh ? d_du.d() : nullptr;
+  a.lQhbnd = d_qb.d();
+  a.lFhbnd = d_fb.d();
+  a.dx = d_dx.d();
+  a.dx_stride = 3;
+  a.dt = d_dt.d();
+  a.iters = static_cast<int*>(d_it.p);
+  a.cell0 = 0;
+  a.lQi = d_qi.d();
+  a.lFi = d_fi.d();
+  a.lQhi = d_qhi.d();
+  a.lFhi = d_fhi.d();
+  EXA_TRY(exa::launch_predictor(sd, exp, a, ncells, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(lQhbnd, d_qb.p, sizeof(double) * ncells * 6 * z.B2 * z.D, hipMemcpyDeviceToHost));
+  EXA_TRY(hipMemcpy(lFhbnd, d_fb.p, sizeof(double) * ncells * 6 * z.B2 * z.V, hipMemcpyDeviceToHost));
+  if (lduh) EXA_TRY(hipMemcpy(lduh, d_du.p, sizeof(double) * ncells * z.B3 * z.V, hipMemcpyDeviceToHost));
+  if (iters) EXA_TRY(hipMemcpy(iters, d_it.p, sizeof(int) * ncells, hipMemcpyDeviceToHost));
+  if (lQi) EXA_TRY(hipMemcpy(lQi, d_qi.p, sizeof(double) * ncells * z.B4 * z.D, hipMemcpyDeviceToHost));
+  if (lFi) EXA_TRY(hipMemcpy(lFi, d_fi.p, sizeof(double) * ncells * 4 * z.B4 * z.V, hipMemcpyDeviceToHost));
+  if (lQhi) EXA_TRY(hipMemcpy(lQhi, d_qhi.p, sizeof(double) * ncells * z.B3 * z.D, hipMemcpyDeviceToHost));
+  if (lFhi) EXA_TRY(hipMemcpy(lFhi, d_fhi.p, sizeof(double) * ncells * 4 * z.B3 * z.V, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+extern "C" int exa_space_time_predictor_nonlinear(const exa_solver* s, double* lQhbnd, double* lFhbnd,
+                                                  double* lQi, double* lFi, double* lQhi, double* lFhi,
+                                                  const double* luh, const double* dx, double dt,
+                                                  int* picard_iterations) {
+  return predictor_impl(s, 1, lQhbnd, lFhbnd, nullptr, lQi, lFi, lQhi, lFhi, luh, dx, dt,
+                        picard_iterations);
+}
+
+extern "C" int exa_batch_predictor_volume(const exa_solver* s, long ncells, double* lQhbnd,
+                                          double* lFhbnd, double* lduh, const double* luh,
+                                          const double* dx, double dt, int* picard_iterations) {
+  return predictor_impl(s, ncells, lQhbnd, lFhbnd, lduh, nullptr, nullptr, nullptr, nullptr, luh, dx,
+                        dt, picard_iterations);
+}
+
+extern "C" int exa_volume_integral_nonlinear(const exa_solver* s, double* lduh, const double* lFhi,
+                                             const double* dx) {
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || !lduh || !lFhi || !dx) return EXA_E_INVALID;
+  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  DevBuf d_du, d_fhi, d_dx;
+  EXA_TRY(d_du.alloc(sizeof(double) * z.B3 * z.V));
+  EXA_TRY(d_fhi.alloc(sizeof(double) * 4 * z.B3 * z.V));
+  EXA_TRY(d_dx.alloc(sizeof(double) * 3));
+  EXA_TRY(hipMemcpy(d_fhi.p, lFhi, sizeof(double) * 4 * z.B3 * z.V, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dx.p, dx, sizeof(double) * 3, hipMemcpyHostToDevice));
+  exa::VolumeArgs a{d_du.d(), d_fhi.d(), d_dx.d(), 0};
+  EXA_TRY(exa::launch_volume_integral(sd, a, 1, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(lduh, d_du.p, sizeof(double) * z.B3 * z.V, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+// Shared body of the single-cell surface / update / time-step entry points.
+static int update_impl(const exa_solver* s, int mode, double* luh, double* lduh, const double* lFhbnd,
+                       const double* dx, double dt, double* dt_out) {
+  SolverDesc sd;
+  if (!to_desc(s, &sd)) return EXA_E_INVALID;
+  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  const double one[3] = {1.0, 1.0, 1.0};
+  DevBuf d_luh, d_du, d_fb, d_dx, d_dt, d_cell;
+  EXA_TRY(d_luh.alloc(sizeof(double) * z.B3 * z.D));
+  EXA_TRY(d_du.alloc(sizeof(double) * z.B3 * z.V));
+  EXA_TRY(d_fb.alloc(sizeof(double) * 6 * z.B2 * z.V));
+  EXA_TRY(d_dx.alloc(sizeof(double) * 3));
+  EXA_TRY(d_dt.alloc(sizeof(double)));
+  EXA_TRY(d_cell.alloc(sizeof(double)));
+  EXA_TRY(hipMemset(d_luh.p, 0, sizeof(double) * z.B3 * z.D));
+  EXA_TRY(hipMemset(d_du.p, 0, sizeof(double) * z.B3 * z.V));
+  EXA_TRY(hipMemset(d_fb.p, 0, sizeof(double) * 6 * z.B2 * z.V));
+  if (luh) EXA_TRY(hipMemcpy(d_luh.p, luh, sizeof(double) * z.B3 * z.D, hipMemcpyHostToDevice));
+  if (lduh) EXA_TRY(hipMemcpy(d_du.p, lduh, sizeof(double) * z.B3 * z.V, hipMemcpyHostToDevice));
+  if (lFhbnd) EXA_TRY(hipMemcpy(d_fb.p, lFhbnd, sizeof(double) * 6 * z.B2 * z.V, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dx.p, dx ? dx : one, sizeof(double) * 3, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dt.p, &dt, sizeof(double), hipMemcpyHostToDevice));
+  exa::UpdateArgs a{};
+  a.luh = d_luh.d();
+  a.lduh = d_du.d();
+  a.lFhbnd = d_fb.d();
+  a.dx = d_dx.d();
+  a.dx_stride = 0;
+  a.dt = d_dt.d();
+  a.cfl_pnpm = sd.cfl * kPNPM[sd.order];
+  a.dt_slots = nullptr;
+  a.dt_cell = dt_out ? d_cell.d() : nullptr;
+  a.cell0 = 0;
+  EXA_TRY(exa::launch_update(sd, mode, a, 1, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  if (mode & exa::kUpdUpdate) EXA_TRY(hipMemcpy(luh, d_luh.p, sizeof(double) * z.B3 * z.D, hipMemcpyDeviceToHost));
+  if (mode & exa::kUpdWriteLduh) EXA_TRY(hipMemcpy(lduh, d_du.p, sizeof(double) * z.B3 * z.V, hipMemcpyDeviceToHost));
+  if (dt_out) EXA_TRY(hipMemcpy(dt_out, d_cell.p, sizeof(double), hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+extern "C" int exa_surface_integral_nonlinear(const exa_solver* s, double* lduh, const double* lFhbnd,
+                                              const double* dx) {
+  if (!lduh || !lFhbnd || !dx) return EXA_E_INVALID;
+  return update_impl(s, exa::kUpdSurface | exa::kUpdWriteLduh, nullptr, lduh, lFhbnd, dx, 0.0, nullptr);
+}
+
+extern "C" int exa_solution_update(const exa_solver* s, double* luh, const double* lduh, double dt) {
+  if (!luh || !lduh) return EXA_E_INVALID;
+  return update_impl(s, exa::kUpdUpdate, luh, const_cast<double*>(lduh), nullptr, nullptr, dt, nullptr);
+}
+
+extern "C" int exa_stable_time_step_size(const exa_solver* s, const double* luh, const double* dx,
+                                         double* dt_out) {
+  if (!luh || !dx || !dt_out) return EXA_E_INVALID;
+  return update_impl(s, 0, const_cast<double*>(luh), nullptr, nullptr, dx, 0.0, dt_out);
+}
+
+extern "C" int exa_riemann_solver_nonlinear(const exa_solver* s, double* FL, double* FR, const double* QL,
+                                            const double* QR, double dt, int direction) {
+  (void)dt;  // unused by the nonlinear Rusanov solver, as in the reference
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || !FL || !FR || !QL || !QR || direction < 0 || direction > 2) return EXA_E_INVALID;
+  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  // Two cells: cell 0 owns face 2d+1 (left state), cell 1 owns face 2d (right state).
+  DevBuf d_qb, d_fb, d_face;
+  const long qf = z.B2 * z.D, ff = z.B2 * z.V;
+  EXA_TRY(d_qb.alloc(sizeof(double) * 2 * 6 * qf));
+  EXA_TRY(d_fb.alloc(sizeof(double) * 2 * 6 * ff));
+  EXA_TRY(d_face.alloc(sizeof(FaceDesc)));
+  const long iqL = (0 * 6 + 2 * direction + 1) * qf, iqR = (1 * 6 + 2 * direction) * qf;
+  const long ifL = (0 * 6 + 2 * direction + 1) * ff, ifR = (1 * 6 + 2 * direction) * ff;
+  EXA_TRY(hipMemcpy(d_qb.d() + iqL, QL, sizeof(double) * qf, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_qb.d() + iqR, QR, sizeof(double) * qf, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_fb.d() + ifL, FL, sizeof(double) * ff, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_fb.d() + ifR, FR, sizeof(double) * ff, hipMemcpyHostToDevice));
+  const FaceDesc f{0, 1, direction, 0};
+  EXA_TRY(hipMemcpy(d_face.p, &f, sizeof(FaceDesc), hipMemcpyHostToDevice));
+  exa::RiemannArgs a{static_cast<const FaceDesc*>(d_face.p), 1, d_qb.d(), d_fb.d()};
+  EXA_TRY(exa::launch_riemann(sd, a, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(FL, d_fb.d() + ifL, sizeof(double) * ff, hipMemcpyDeviceToHost));
+  EXA_TRY(hipMemcpy(FR, d_fb.d() + ifR, sizeof(double) * ff, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grid
+// ---------------------------------------------------------------------------
+struct ProfRec {
+  int stage;
+  hipEvent_t b, e;
+};
+
+struct exa_grid {
+  SolverDesc sd;
+  int nx, ny, nz;
+  long ncells;
+  double lo[3], hi[3], dx[3];
+  long nfaces;
+  hipStream_t stream = nullptr;
+  double* luh = nullptr;
+  double* lduh = nullptr;
+  double* lQhbnd = nullptr;
+  double* lFhbnd = nullptr;
+  double* d_dx = nullptr;
+  int* iters = nullptr;
+  FaceDesc* faces = nullptr;
+  exa::StepState* st = nullptr;
+  double* dt_hist = nullptr;
+  long dt_hist_len = 0;
+  bool predicted = false;
+  bool profiling = false;
+  std::vector<ProfRec> prof;
+  double prof_ms[EXA_NUM_STAGES] = {0, 0, 0, 0};
+  long prof_n[EXA_NUM_STAGES] = {0, 0, 0, 0};
+};
+
+namespace {
+
+template <class F>
+hipError_t staged(exa_grid* g, int stage, F&& launch) {
+  if (!g->profiling) return launch();
+  ProfRec r{stage, nullptr, nullptr};
+  hipError_t e = hipEventCreate(&r.b);
+  if (e == hipSuccess) e = hipEventCreate(&r.e);
+  if (e == hipSuccess) e = hipEventRecord(r.b, g->stream);
+  if (e == hipSuccess) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(r.e, g->stream);
+  g->prof.push_back(r);
+  return e;
+}
+
+void free_grid(exa_grid* g) {
+  if (!g) return;
+  for (auto& r : g->prof) {
+    (void)hipEventDestroy(r.b);
+    (void)hipEventDestroy(r.e);
+  }
+  (void)hipFree(g->luh);
+  (void)hipFree(g->lduh);
+  (void)hipFree(g->lQhbnd);
+  (void)hipFree(g->lFhbnd);
+  (void)hipFree(g->d_dx);
+  (void)hipFree(g->iters);
+  (void)hipFree(g->faces);
+  (void)hipFree(g->st);
+  (void)hipFree(g->dt_hist);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+// initial dt: dt of every cell into the slots, then the global minimum.
+hipError_t launch_initial_dt(exa_grid* g) {
+  exa::UpdateArgs a{};
+  a.luh = g->luh;
+  a.lduh = g->lduh;
+  a.lFhbnd = g->lFhbnd;
+  a.dx = g->d_dx;
+  a.dx_stride = 0;
+  a.dt = &g->st->dt;
+  a.cfl_pnpm = g->sd.cfl * kPNPM[g->sd.order];
+  a.dt_slots = g->st->dt_slots;
+  a.dt_cell = nullptr;
+  a.cell0 = 0;
+  hipError_t e = staged(g, EXA_STAGE_UPDATE, [&] { return exa::launch_update(g->sd, 0, a, g->ncells, g->stream); });
+  if (e != hipSuccess) return e;
+  return staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, false, g->stream); });
+}
+
+hipError_t launch_predict(exa_grid* g) {
+  exa::PredictorArgs a{};
+  a.luh = g->luh;
+  a.lduh = g->lduh;
+  a.lQhbnd = g->lQhbnd;
+  a.lFhbnd = g->lFhbnd;
+  a.dx = g->d_dx;
+  a.dx_stride = 0;
+  a.dt = &g->st->dt;
+  a.iters = g->iters;
+  a.cell0 = 0;
+  hipError_t e = staged(g, EXA_STAGE_PREDICTOR, [&] { return exa::launch_predictor(g->sd, false, a, g->ncells, g->stream); });
+  if (e == hipSuccess && g->profiling) e = exa::launch_count_iterations(g->iters, g->ncells, g->st, g->stream);
+  return e;
+}
+
+hipError_t launch_one_step(exa_grid* g) {
+  exa::RiemannArgs ra{g->faces, g->nfaces, g->lQhbnd, g->lFhbnd};
+  hipError_t e = staged(g, EXA_STAGE_RIEMANN, [&] { return exa::launch_riemann(g->sd, ra, g->stream); });
+  if (e != hipSuccess) return e;
+  exa::UpdateArgs ua{};
+  ua.luh = g->luh;
+  ua.lduh = g->lduh;
+  ua.lFhbnd = g->lFhbnd;
+  ua.dx = g->d_dx;
+  ua.dx_stride = 0;
+  ua.dt = &g->st->dt;
+  ua.cfl_pnpm = g->sd.cfl * kPNPM[g->sd.order];
+  ua.dt_slots = g->st->dt_slots;
+  ua.dt_cell = nullptr;
+  ua.cell0 = 0;
+  e = staged(g, EXA_STAGE_UPDATE, [&] {
+    return exa::launch_update(g->sd, exa::kUpdSurface | exa::kUpdUpdate, ua, g->ncells, g->stream);
+  });
+  if (e != hipSuccess) return e;
+  e = staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, true, g->stream); });
+  if (e != hipSuccess) return e;
+  return launch_predict(g);
+}
+
+}  // namespace
+
+extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo,
+                               const double* hi, int boundary, exa_grid** out) {
+  if (!out) return EXA_E_INVALID;
+  *out = nullptr;
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || nx < 1 || ny < 1 || nz < 1 || !lo || !hi) return EXA_E_INVALID;
+  if (boundary != EXA_BC_PERIODIC) return EXA_E_UNSUPPORTED;
+  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  if ((double)nx * ny * nz > 2.0e9) return EXA_E_INVALID;
+  exa_grid* g = new exa_grid();
+  g->sd = sd;
+  g->nx = nx;
+  g->ny = ny;
+  g->nz = nz;
+  g->ncells = (long)nx * ny * nz;
+  for (int d = 0; d < 3; ++d) {
+    g->lo[d] = lo[d];
+    g->hi[d] = hi[d];
+  }
+  g->dx[0] = (hi[0] - lo[0]) / nx;
+  g->dx[1] = (hi[1] - lo[1]) / ny;
+  g->dx[2] = (hi[2] - lo[2]) / nz;
+  const Sizes z(sd);
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  auto A = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  A((void**)&g->luh, sizeof(double) * g->ncells * z.B3 * z.D);
+  A((void**)&g->lduh, sizeof(double) * g->ncells * z.B3 * z.V);
+  A((void**)&g->lQhbnd, sizeof(double) * g->ncells * 6 * z.B2 * z.D);
+  A((void**)&g->lFhbnd, sizeof(double) * g->ncells * 6 * z.B2 * z.V);
+  A((void**)&g->d_dx, sizeof(double) * 3);
+  A((void**)&g->iters, sizeof(int) * g->ncells);
+  // Periodic: every cell owns its +x, +y, +z face.
+  g->nfaces = 3 * g->ncells;
+  A((void**)&g->faces, sizeof(FaceDesc) * g->nfaces);
+  A((void**)&g->st, sizeof(exa::StepState));
+  g->dt_hist_len = 1 << 16;
+  A((void**)&g->dt_hist, sizeof(double) * g->dt_hist_len);
+  if (e != hipSuccess) {
+    free_grid(g);
+    return from_hip(e);
+  }
+  std::vector<FaceDesc> faces(g->nfaces);
+  auto cid = [&](int i, int j, int k) {
+    i = (i + nx) % nx;
+    j = (j + ny) % ny;
+    k = (k + nz) % nz;
+    return (int)(((long)k * ny + j) * nx + i);
+  };
+  long f = 0;
+  for (int dir = 0; dir < 3; ++dir)
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const int c = cid(i, j, k);
+          const int r = dir == 0 ? cid(i + 1, j, k) : (dir == 1 ? cid(i, j + 1, k) : cid(i, j, k + 1));
+          faces[f++] = FaceDesc{c, r, dir, 0};
+        }
+  exa::StepState st0{};
+  st0.dt = 0.0;
+  st0.t = 0.0;
+  st0.steps = 0;
+  st0.dt_hist = g->dt_hist;
+  st0.dt_hist_len = g->dt_hist_len;
+  for (int i = 0; i < exa::kDtSlots; ++i) st0.dt_slots[i] = exa::kDtSlotEmpty;
+  e = hipMemcpy(g->faces, faces.data(), sizeof(FaceDesc) * g->nfaces, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->st, &st0, sizeof(st0), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->d_dx, g->dx, sizeof(double) * 3, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(g->lduh, 0, sizeof(double) * g->ncells * z.B3 * z.V);
+  if (e == hipSuccess) e = hipMemset(g->iters, 0, sizeof(int) * g->ncells);
+  if (e != hipSuccess) {
+    free_grid(g);
+    return from_hip(e);
+  }
+  *out = g;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_destroy(exa_grid* g) {
+  if (g && g->stream) (void)hipStreamSynchronize(g->stream);
+  free_grid(g);
+  return EXA_OK;
+}
+
+extern "C" long exa_grid_num_cells(const exa_grid* g) { return g ? g->ncells : 0; }
+
+static int reset_time(exa_grid* g) {
+  exa::StepState st0{};
+  st0.dt_hist = g->dt_hist;
+  st0.dt_hist_len = g->dt_hist_len;
+  for (int i = 0; i < exa::kDtSlots; ++i) st0.dt_slots[i] = exa::kDtSlotEmpty;
+  EXA_TRY(hipMemcpyAsync(g->st, &st0, sizeof(st0), hipMemcpyHostToDevice, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  g->predicted = false;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_upload_solution(exa_grid* g, const double* luh) {
+  if (!g || !luh) return EXA_E_INVALID;
+  const Sizes z(g->sd);
+  EXA_TRY(hipMemcpyAsync(g->luh, luh, sizeof(double) * g->ncells * z.B3 * z.D, hipMemcpyHostToDevice, g->stream));
+  return reset_time(g);
+}
+
+extern "C" int exa_grid_download_solution(exa_grid* g, double* luh) {
+  if (!g || !luh) return EXA_E_INVALID;
+  const Sizes z(g->sd);
+  EXA_TRY(hipMemcpyAsync(luh, g->luh, sizeof(double) * g->ncells * z.B3 * z.D, hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_init_euler(exa_grid* g, const exa_euler_ic* ic) {
+  if (!g || !ic || ic->nmodes < 0 || ic->nmodes > exa::kMaxModes) return EXA_E_INVALID;
+  if (g->sd.pde != EXA_PDE_EULER) return EXA_E_UNSUPPORTED;
+  exa::ICArgs a{};
+  a.luh = g->luh;
+  a.ncells = g->ncells;
+  a.nx = g->nx;
+  a.ny = g->ny;
+  a.nz = g->nz;
+  a.order = g->sd.order;
+  a.nvar = g->sd.nvar;
+  a.npar = g->sd.npar;
+  for (int d = 0; d < 3; ++d) {
+    a.lo[d] = g->lo[d];
+    a.dx[d] = g->dx[d];
+    a.k[d] = ic->k[d];
+    a.vel[d] = ic->vel[d];
+  }
+  a.rho0 = ic->rho0;
+  a.amp = ic->amp;
+  a.p = ic->p;
+  a.nmodes = ic->nmodes;
+  a.mode_amp = ic->mode_amp;
+  uint64_t state = ic->seed;
+  for (int m = 0; m < ic->nmodes; ++m) {
+    int kk[3];
+    do {
+      for (int d = 0; d < 3; ++d) kk[d] = (int)(splitmix64(state) % 7) - 3;
+    } while (kk[0] == 0 && kk[1] == 0 && kk[2] == 0);
+    for (int d = 0; d < 3; ++d) a.mode_k[m][d] = kk[d];
+    for (int c = 0; c < 4; ++c)
+      a.mode_phase[m][c] = 6.283185307179586476925286766559 * (double)(splitmix64(state) >> 11) * 0x1.0p-53;
+  }
+  EXA_TRY(exa::launch_euler_ic(a, g->stream));
+  return reset_time(g);
+}
+
+extern "C" int exa_grid_step(exa_grid* g, int nsteps) {
+  if (!g || nsteps < 0) return EXA_E_INVALID;
+  if (!g->predicted) {
+    EXA_TRY(launch_initial_dt(g));
+    EXA_TRY(launch_predict(g));
+    g->predicted = true;
+  }
+  for (int n = 0; n < nsteps; ++n) EXA_TRY(launch_one_step(g));
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_synchronize(exa_grid* g) {
+  if (!g) return EXA_E_INVALID;
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_time(exa_grid* g, double* t, double* dt, long* steps) {
+  if (!g) return EXA_E_INVALID;
+  exa::StepState st;
+  EXA_TRY(hipMemcpyAsync(&st, g->st, sizeof(st), hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  if (t) *t = st.t;
+  if (dt) *dt = st.dt;
+  if (steps) *steps = st.steps;
+  if (!std::isfinite(st.dt)) return EXA_E_NONFINITE;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_dt_history(exa_grid* g, double* out, long n) {
+  if (!g || !out || n < 0 || n > g->dt_hist_len) return EXA_E_INVALID;
+  EXA_TRY(hipMemcpyAsync(out, g->dt_hist, sizeof(double) * n, hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_download_cells(exa_grid* g, long first, long count, double* luh) {
+  if (!g || !luh || first < 0 || count < 0 || first + count > g->ncells) return EXA_E_INVALID;
+  const Sizes z(g->sd);
+  const size_t per = (size_t)z.B3 * z.D;
+  EXA_TRY(hipMemcpyAsync(luh, g->luh + first * per, sizeof(double) * count * per, hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_picard_totals(exa_grid* g, unsigned long long* iterations, unsigned long long* cells) {
+  if (!g) return EXA_E_INVALID;
+  exa::StepState st;
+  EXA_TRY(hipMemcpyAsync(&st, g->st, sizeof(st), hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  if (iterations) *iterations = st.picard_total;
+  if (cells) *cells = st.predictor_cells;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_picard_iterations(exa_grid* g, int* out) {
+  if (!g || !out) return EXA_E_INVALID;
+  EXA_TRY(hipMemcpyAsync(out, g->iters, sizeof(int) * g->ncells, hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_set_profiling(exa_grid* g, int on) {
+  if (!g) return EXA_E_INVALID;
+  g->profiling = on != 0;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_stage_times(exa_grid* g, double* ms, long* launches) {
+  if (!g) return EXA_E_INVALID;
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  for (auto& r : g->prof) {
+    float t = 0.0f;
+    EXA_TRY(hipEventElapsedTime(&t, r.b, r.e));
+    g->prof_ms[r.stage] += t;
+    g->prof_n[r.stage] += 1;
+    (void)hipEventDestroy(r.b);
+    (void)hipEventDestroy(r.e);
+  }
+  g->prof.clear();
+  for (int i = 0; i < EXA_NUM_STAGES; ++i) {
+    if (ms) ms[i] = g->prof_ms[i];
+    if (launches) launches[i] = g->prof_n[i];
+    g->prof_ms[i] = 0.0;
+    g->prof_n[i] = 0;
+  }
+  return EXA_OK;
+}

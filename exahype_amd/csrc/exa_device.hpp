@@ -1,0 +1,154 @@
+// exa_device.hpp -- device-side building blocks shared by the ADER-DG kernels:
+// DG tables in constant memory, PDE functors, wavefront/workgroup reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace exa {
+
+// ---------------------------------------------------------------------------
+// Tables.  The reference's printed literals (DGMatrices.cpp, Gauss*Quadrature.cpp),
+// [order][i] / [order][row][col], zero padded to 10.  Indexed with compile-time
+// indices after unrolling, they become scalar constant loads.
+// ---------------------------------------------------------------------------
+#define EXA_TABLE static __constant__
+#include "dg_tables.inc"
+#undef EXA_TABLE
+
+// Stable time step size factors PNPM[order], stableTimeStepSize.cpph:33-34.
+static __constant__ double exa_pnpm[10] = {1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015};
+
+constexpr int kWave = 64;
+
+__host__ __device__ constexpr int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// Threads per workgroup for a kernel that maps one thread to one of the B^3
+// nodes of a cell: a whole number of 64-wide wavefronts.
+__host__ __device__ constexpr int node_threads(int B) { return round_up(B * B * B, kWave); }
+
+// ---------------------------------------------------------------------------
+// Reductions over a 64-wide wavefront (xor butterfly: every lane ends with the
+// identical value because each level adds the same two operands).
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Workgroup sum; `red` holds NT/64 doubles of LDS.  All threads return the
+// same value (partials are summed in wave order by every thread).
+template <int NT>
+__device__ inline double block_sum(double v, double* red) {
+  constexpr int NW = NT / kWave;
+  v = wave_sum(v);
+  if (NW == 1) return v;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) s += red[w];
+  return s;
+}
+
+template <int NT>
+__device__ inline double block_min(double v, double* red) {
+  constexpr int NW = NT / kWave;
+  v = wave_min(v);
+  if (NW == 1) return v;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) s = red[w] < s ? red[w] : s;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// PDE functors: the user solver interface (UserSolverInterface.h:64-127)
+// compiled for the device.  Q holds V variables followed by P parameters.
+// ---------------------------------------------------------------------------
+
+// Compressible Euler, gamma = 1.4; the expression trees of the reference's
+// kernel test solver (tests/kernels/c/GenericEulerKernelTest3D.cpp:26-82).
+// NPAR trailing parameters are carried and ignored (the test solver has 1).
+template <int NPAR>
+struct Euler {
+  static constexpr int V = 5;
+  static constexpr int P = NPAR;
+
+  __device__ static inline void flux(const double* Q, double (*F)[V]) {
+    const double GAMMA = 1.4;
+    const double irho = 1.0 / Q[0];
+    const double p = (GAMMA - 1) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+    F[0][0] = Q[1];
+    F[0][1] = irho * Q[1] * Q[1] + p;
+    F[0][2] = irho * Q[1] * Q[2];
+    F[0][3] = irho * Q[1] * Q[3];
+    F[0][4] = irho * Q[1] * (Q[4] + p);
+    F[1][0] = Q[2];
+    F[1][1] = irho * Q[2] * Q[1];
+    F[1][2] = irho * Q[2] * Q[2] + p;
+    F[1][3] = irho * Q[2] * Q[3];
+    F[1][4] = irho * Q[2] * (Q[4] + p);
+    F[2][0] = Q[3];
+    F[2][1] = irho * Q[3] * Q[1];
+    F[2][2] = irho * Q[3] * Q[2];
+    F[2][3] = irho * Q[3] * Q[3] + p;
+    F[2][4] = irho * Q[3] * (Q[4] + p);
+  }
+
+  // Largest |eigenvalue| in direction d (the eigenvalues are u_n-c, u_n x3, u_n+c).
+  __device__ static inline double max_abs_eigenvalue(const double* Q, int d) {
+    const double GAMMA = 1.4;
+    const double irho = 1.0 / Q[0];
+    const double p = (GAMMA - 1) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+    const double u_n = Q[d + 1] * irho;
+    const double c = sqrt(GAMMA * p * irho);
+    const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
+    // std::max_element order: first maximum over (l0, l1, l1, l1, l4)
+    double m = l0;
+    m = (m < l1) ? l1 : m;
+    m = (m < l4) ? l4 : m;
+    return m;
+  }
+
+  __device__ static inline void algebraic_source(const double*, double* S) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) S[m] = 0.0;
+  }
+
+  // The test solver's NCP in the state its predictor test uses (zero).
+  __device__ static inline void ncp(const double*, const double (*)[V], double* BgradQ) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) BgradQ[m] = 0.0;
+  }
+};
+
+// Stable time step of one node, stableTimeStepSize.cpph:37-57 (the caller takes
+// the minimum).  maxEigenvalue = max over variables of |lambda|.
+template <class PDE>
+__device__ inline double node_stable_dt(const double* Q, const double invDx[3], double cfl_pnpm) {
+  double denominator = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) denominator += PDE::max_abs_eigenvalue(Q, d) * invDx[d];
+  return cfl_pnpm / denominator;
+}
+
+}  // namespace exa

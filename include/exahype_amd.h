@@ -1,0 +1,163 @@
+/*
+ * exahype_amd.h -- C-ABI of the MI355X-native ADER-DG time-stepping core.
+ *
+ * Plain pointers and sizes only.  Array layouts are those of the reference
+ * (row-major, variable innermost), B = order+1, D = nvar+npar:
+ *   luh    (z,y,x,var)               B^3 D        lduh   (z,y,x,var)  B^3 V
+ *   lQhbnd (face,a,b,var)            6 B^2 D      lFhbnd (face,a,b,var) 6 B^2 V
+ *          faces -x,+x,-y,+y,-z,+z; x-faces (z,y), y-faces (z,x), z-faces (y,x)
+ *   lQi    (z,y,x,t,var)             B^4 D        lFi (t,z,y,x,{x,y,z,src},var) 4 B^4 V
+ *   lQhi   (z,y,x,var)               B^3 D        lFhi 4 blocks of B^3 V:
+ *          x (z,y,x), y (z,x,y), z (y,x,z), src (z,y,x)
+ * Every function returns an exa_status.  Host pointers unless stated.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout,
+ * K = ExaHyPE/kernels/aderdg/generic):
+ *   kernels::aderdg::generic::c::*   K/Kernels.h:85-213 and the K/c/3d/ sources
+ *   the per-cell loop of ADERDGSolver (ExaHyPE/exahype/solvers/ADERDGSolver.cpp:
+ *   1881 performPredictionAndVolumeIntegral, 2185 updateSolution, 2789
+ *   solveRiemannProblemAtInterface, 2004 computeTimeStepSize) driven by the
+ *   Runner's non-fused step (ExaHyPE/exahype/runners/Runner.cpp:1116-1162).
+ */
+#ifndef EXAHYPE_AMD_H
+#define EXAHYPE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum exa_status {
+  EXA_OK = 0,
+  EXA_E_HIP = 1,         /* a HIP runtime call failed */
+  EXA_E_UNSUPPORTED = 2, /* no compiled instantiation for this solver */
+  EXA_E_INVALID = 3,     /* bad argument */
+  EXA_E_NOMEM = 4,       /* device allocation failed */
+  EXA_E_NONFINITE = 5    /* a non-finite time step size was produced */
+} exa_status;
+
+/* PDEs compiled into the library (the user's SolverType becomes a device functor). */
+typedef enum exa_pde { EXA_PDE_EULER = 0, EXA_PDE_ELASTIC = 1 } exa_pde;
+
+/* The spec file's `terms` (useFlux / useSource / useNCP template flags). */
+enum { EXA_TERM_FLUX = 1, EXA_TERM_SOURCE = 2, EXA_TERM_NCP = 4 };
+
+/* SolverType's compile-time constants (AbstractGenericADERDGSolverHeader.template:32-35). */
+typedef struct exa_solver {
+  int pde;               /* exa_pde */
+  int order;             /* SolverType::Order (N) */
+  int num_variables;     /* SolverType::NumberOfVariables (V) */
+  int num_parameters;    /* SolverType::NumberOfParameters (P) */
+  int terms;             /* EXA_TERM_* */
+  int no_time_averaging; /* template flag noTimeAveraging */
+  double cfl;            /* SolverType::CFL */
+} exa_solver;
+
+const char* exa_status_string(int status);
+/* Selects the HIP device for the calling thread; optional (device 0 by default). */
+int exa_init(int device);
+/* 1 if (pde, order, nvar, npar, terms) has a compiled instantiation. */
+int exa_solver_supported(const exa_solver* s);
+
+/* ---------------------------------------------------------------------------
+ * Single-cell entry points: the kernels::aderdg::generic::c call signatures
+ * (host arrays in, host arrays out; synchronous).  The scratch arguments of
+ * the reference become optional outputs of its temporaries (NULL = skip).
+ * ------------------------------------------------------------------------- */
+
+/* spaceTimePredictorNonlinear (K/c/3d/spaceTimePredictorNonlinear.cpph:661).
+ * lQi = tempSpaceTimeUnknowns[0], lFi = tempSpaceTimeFluxUnknowns[0],
+ * lQhi = tempUnknowns, lFhi = tempFluxUnknowns.  picard_iterations may be NULL. */
+int exa_space_time_predictor_nonlinear(const exa_solver* s, double* lQhbnd, double* lFhbnd,
+                                       double* lQi, double* lFi, double* lQhi, double* lFhi,
+                                       const double* luh, const double* dx, double dt,
+                                       int* picard_iterations);
+
+/* volumeIntegralNonlinear (K/c/3d/volumeIntegralNonlinear.cpph:31), time-averaged form. */
+int exa_volume_integral_nonlinear(const exa_solver* s, double* lduh, const double* lFhi,
+                                  const double* dx);
+
+/* surfaceIntegralNonlinear (K/c/3d/surfaceIntegralNonlinear.cpph:21); lduh in/out. */
+int exa_surface_integral_nonlinear(const exa_solver* s, double* lduh, const double* lFhbnd,
+                                   const double* dx);
+
+/* riemannSolverNonlinear (K/c/3d/riemannSolverNonlinear.cpph:37); FL, FR in/out. */
+int exa_riemann_solver_nonlinear(const exa_solver* s, double* FL, double* FR, const double* QL,
+                                 const double* QR, double dt, int direction);
+
+/* solutionUpdate (K/c/3d/solutionUpdate.cpph:23); luh in/out. */
+int exa_solution_update(const exa_solver* s, double* luh, const double* lduh, double dt);
+
+/* stableTimeStepSize (K/c/3d/stableTimeStepSize.cpph:21). */
+int exa_stable_time_step_size(const exa_solver* s, const double* luh, const double* dx,
+                              double* dt_out);
+
+/* ---------------------------------------------------------------------------
+ * Batched entry points: the same kernels over `ncells` cells in one launch
+ * (host arrays, cells contiguous; dx is [ncells][3]).
+ * ------------------------------------------------------------------------- */
+
+/* Predictor + volume integral (performPredictionAndVolumeIntegral,
+ * ADERDGSolver.cpp:1881).  lduh may be NULL; picard_iterations[ncells] may be NULL. */
+int exa_batch_predictor_volume(const exa_solver* s, long ncells, double* lQhbnd, double* lFhbnd,
+                               double* lduh, const double* luh, const double* dx, double dt,
+                               int* picard_iterations);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident grid: one grid level as an array of cells with host-built
+ * face lists; the non-fused global time step of the reference
+ * (Riemann on every face once -> surface integral + update -> dt = min over
+ * cells -> predictor + volume integral) runs entirely on the device.
+ * ------------------------------------------------------------------------- */
+typedef struct exa_grid exa_grid;
+
+enum { EXA_BC_PERIODIC = 0 };
+
+/* Uniform nx*ny*nz Cartesian grid on [lo, hi]; cell c = (k*ny + j)*nx + i. */
+int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo,
+                    const double* hi, int boundary, exa_grid** out);
+int exa_grid_destroy(exa_grid* g);
+long exa_grid_num_cells(const exa_grid* g);
+int exa_grid_upload_solution(exa_grid* g, const double* luh);
+int exa_grid_download_solution(exa_grid* g, double* luh);
+/* Solution of cells [first, first+count). */
+int exa_grid_download_cells(exa_grid* g, long first, long count, double* luh);
+
+/* Euler initial data: rho = rho0 + amp sin(2 pi k.x) + sum of nmodes seeded
+ * Fourier modes (amplitude mode_amp, integer wave vectors in [-3,3]^3,
+ * phases in [0, 2 pi)) in rho and u,v,w; velocity vel; constant pressure p. */
+typedef struct exa_euler_ic {
+  double rho0, amp;
+  double k[3];
+  double vel[3];
+  double p;
+  int nmodes;
+  double mode_amp;
+  unsigned long long seed;
+} exa_euler_ic;
+int exa_grid_init_euler(exa_grid* g, const exa_euler_ic* ic);
+
+/* Advance nsteps time steps (asynchronous on the grid's stream). */
+int exa_grid_step(exa_grid* g, int nsteps);
+int exa_grid_synchronize(exa_grid* g);
+/* Current time stamp, time step size and number of completed steps. */
+int exa_grid_time(exa_grid* g, double* t, double* dt, long* steps);
+/* dt of steps 0..n-1 (dt used by step k = out[k]). */
+int exa_grid_dt_history(exa_grid* g, double* out, long n);
+/* Picard iteration counts of the most recent predictor launch, [ncells]. */
+int exa_grid_picard_iterations(exa_grid* g, int* out);
+
+/* Per-stage timing with HIP events recorded on the grid's stream. */
+enum { EXA_STAGE_PREDICTOR = 0, EXA_STAGE_RIEMANN = 1, EXA_STAGE_UPDATE = 2, EXA_STAGE_DT = 3,
+       EXA_NUM_STAGES = 4 };
+int exa_grid_set_profiling(exa_grid* g, int on);
+/* Sum of per-cell Picard iteration counts and of cells over every predictor
+ * launch made while profiling was on (the work of the predictor launches). */
+int exa_grid_picard_totals(exa_grid* g, unsigned long long* iterations, unsigned long long* cells);
+/* Synchronises, then returns per-stage summed milliseconds and launch counts
+ * since profiling was switched on (and resets them). */
+int exa_grid_stage_times(exa_grid* g, double* ms, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

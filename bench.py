@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "DOF-updates/s, 3D Euler N=5 81^3: FP32-MFMA vs FP64 GPU at L2 err ≤1e-6"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 EULER_FLUX_FLOPS = 40  # flops of one Euler flux evaluation (3 directions)
 
 
@@ -65,6 +66,40 @@ def cpu_baseline(exa_grid, order, n, seconds, threads):
                        f"{elapsed:.2f} s")
 
 
+def run_form(capi, order, n, steps, warmup, seed, precision):
+    """One timed run of the stepping loop; returns timings and the final solution."""
+    s = capi.euler_solver(order, precision=precision)
+    g = capi.Grid(s, n)
+    g.init_euler(rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=8, mode_amp=0.01,
+                 seed=seed)
+    g.step(warmup)
+    g.synchronize()
+    g.set_profiling(True)
+    g.stage_times()  # reset
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    stages = g.stage_times()
+    picard_total, pred_cells = g.picard_totals()
+    g.set_profiling(False)
+    t_now, dt_now, nsteps = g.time()
+    if not np.isfinite(dt_now) or dt_now <= 0:
+        raise RuntimeError(f"non-finite time step after {nsteps} steps")
+    return dict(grid=g, wall=wall, stages=stages, picard_total=picard_total, pred_cells=pred_cells, t=t_now)
+
+
+def roofline_for(r, B, V, D, peak, kernel):
+    pred_ms, pred_n = r["stages"]["predictor"]
+    avg_ms = pred_ms / max(pred_n, 1)
+    flops_per_launch = predictor_flops(B, V, D, r["picard_total"], r["pred_cells"]) / max(pred_n, 1)
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    return dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                frac=round(achieved / peak, 4), traffic=None, kernel=kernel, avg_launch_ms=round(avg_ms, 4),
+                flops_per_launch=flops_per_launch,
+                picard_mean=round(r["picard_total"] / max(r["pred_cells"], 1), 4))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +108,8 @@ def main():
     ap.add_argument("--cells", type=int, default=81, help="cells per direction")
     ap.add_argument("--order", type=int, default=5)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--form", choices=("both", "fp32", "fp64"), default="both",
+                    help="both: time FP64 and FP32-bulk and report the error between them")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -86,42 +123,33 @@ def main():
     capi.lib()
     order, n = args.order, args.cells
     B, V, D = order + 1, 5, 5
-    s = capi.euler_solver(order)
-    g = capi.Grid(s, n)
-    g.init_euler(rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=8, mode_amp=0.01,
-                 seed=args.seed)
-    g.step(args.warmup)
-    g.synchronize()
-
-    g.set_profiling(True)
-    g.stage_times()  # reset
-    t0 = time.perf_counter()
-    g.step(args.steps)
-    g.synchronize()
-    wall = time.perf_counter() - t0
-    stages = g.stage_times()
-    picard_total, pred_cells = g.picard_totals()
-    g.set_profiling(False)
-    t_now, dt_now, nsteps = g.time()
-    if not np.isfinite(dt_now) or dt_now <= 0:
-        raise RuntimeError(f"non-finite time step after {nsteps} steps")
-
     ncells = n ** 3
     dof = ncells * B ** 3
-    value = dof * args.steps / wall
-    pred_ms, pred_n = stages["predictor"]
-    avg_ms = pred_ms / max(pred_n, 1)
-    flops_per_launch = predictor_flops(B, V, D, picard_total, pred_cells) / max(pred_n, 1)
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
-                    kernel="predictor_kernel<6,Euler,flux> (predictor + volume integral)",
-                    avg_launch_ms=round(avg_ms, 4), flops_per_launch=flops_per_launch,
-                    picard_mean=round(picard_total / max(pred_cells, 1), 4))
+    forms = {"both": ("fp64", "fp32"), "fp32": ("fp32",), "fp64": ("fp64",)}[args.form]
+    res = {}
+    sol64 = None
+    for form in forms:
+        prec = capi.PRECISION_FP64 if form == "fp64" else capi.PRECISION_FP32_BULK
+        r = run_form(capi, order, n, args.steps, args.warmup, args.seed, prec)
+        res[form] = r
+        if form == "fp64" and "fp32" in forms:
+            sol64 = r["grid"].solution()
+        if form == "fp32" and sol64 is not None:
+            sol32 = r["grid"].solution()
+            r["l2"] = float(np.linalg.norm(sol32 - sol64) / np.linalg.norm(sol64))
+            r["linf"] = float(np.abs(sol32 - sol64).max() / np.abs(sol64).max())
+            del sol32
+    main_form = "fp32" if "fp32" in res else "fp64"
+    m = res[main_form]
+    value = dof * args.steps / m["wall"]
+    if main_form == "fp32":
+        roof = roofline_for(m, B, V, D, FP32_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,float> (predictor + volume integral)")
+    else:
+        roof = roofline_for(m, B, V, D, FP64_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,double> (predictor + volume integral)")
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-        cpu = cpu_baseline(g, order, n, args.cpu_seconds, threads)
+        cpu = cpu_baseline(m["grid"], order, n, args.cpu_seconds, threads)
     out = {
         "metric": METRIC,
         "value": value,
@@ -129,21 +157,34 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": m["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "fp32+fp64" if main_form == "fp32" else "f64",
         "data": "synthetic (entropy wave + 8 seeded Fourier modes, generated on the device)",
         "config": {"workload": f"euler3d-precision {n}^3 cells N={order} periodic CFL0.9",
-                   "form": "plain FP64", "cells": ncells, "order": order, "dof_per_step": dof,
-                   "scalar_unknowns_per_step": dof * V, "seed": args.seed, "t_end": t_now},
-        "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()},
-        "roofline": roofline,
+                   "form": ("FP32-bulk: predictor/volume/Riemann f32, surface/update/dt f64"
+                            if main_form == "fp32" else "plain FP64"),
+                   "cells": ncells, "order": order, "dof_per_step": dof,
+                   "scalar_unknowns_per_step": dof * V, "seed": args.seed},
+        "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in m["stages"].items()},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if "fp64" in res:
+        r64 = res["fp64"]
+        out["fp64"] = {"value": dof * args.steps / r64["wall"], "ms_per_step": r64["wall"] / args.steps * 1e3,
+                       "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in r64["stages"].items()},
+                       "roofline": roofline_for(r64, B, V, D, FP64_PEAK_TFLOPS,
+                                                "predictor_kernel<6,Euler,flux,double>")}
+    if "fp32" in res and "fp64" in res:
+        out["fp32_vs_fp64"] = {"speedup": out["fp64"]["ms_per_step"] / out["ms_per_step"],
+                               "l2_rel_err": res["fp32"]["l2"], "linf_rel_err": res["fp32"]["linf"],
+                               "steps_compared": args.warmup + args.steps}
     print(json.dumps(out), flush=True)
-    g.close()
+    for r in res.values():
+        r["grid"].close()
 
 
 if __name__ == "__main__":

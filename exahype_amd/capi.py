@@ -20,6 +20,7 @@ TERM_FLUX, TERM_SOURCE, TERM_NCP = 1, 2, 4
 STAGE_PREDICTOR, STAGE_RIEMANN, STAGE_UPDATE, STAGE_DT = range(4)
 STAGE_NAMES = ("predictor", "riemann", "update", "dt")
 BC_PERIODIC = 0
+PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
 
 # Every symbol declared in include/exahype_amd.h.
 EXPORTED = (
@@ -42,7 +43,7 @@ class Solver(ctypes.Structure):
     """exa_solver: the reference SolverType's constants."""
     _fields_ = [("pde", ctypes.c_int), ("order", ctypes.c_int), ("num_variables", ctypes.c_int),
                 ("num_parameters", ctypes.c_int), ("terms", ctypes.c_int),
-                ("no_time_averaging", ctypes.c_int), ("cfl", ctypes.c_double)]
+                ("no_time_averaging", ctypes.c_int), ("cfl", ctypes.c_double), ("precision", ctypes.c_int)]
 
     @property
     def B(self):
@@ -59,8 +60,8 @@ class EulerIC(ctypes.Structure):
                 ("mode_amp", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
 
 
-def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9):
-    return Solver(PDE_EULER, order, 5, npar, terms, 0, cfl)
+def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9, precision=PRECISION_FP64):
+    return Solver(PDE_EULER, order, 5, npar, terms, 0, cfl, precision)
 
 
 _lib = None

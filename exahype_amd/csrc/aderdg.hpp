@@ -20,7 +20,14 @@ struct SolverDesc {
   int terms;  // kFlux | kSource | kNCP
   int no_time_averaging;
   double cfl;
+  int precision;  // kFP64 or kFP32Bulk
 };
+
+// Precision forms.  kFP64: everything in double (the reference's arithmetic).
+// kFP32Bulk: predictor, volume integral, Riemann solver and the face /
+// update arrays in float; surface integral, solution update and the time
+// step size in double on the double solution.
+enum : int { kFP64 = 0, kFP32Bulk = 1 };
 
 // A face between two cells; `left` owns face 2*dir+1, `right` owns face 2*dir.
 struct FaceDesc {
@@ -45,11 +52,12 @@ struct StepState {
   unsigned long long dt_slots[kDtSlots];  // per-slot minima of the next dt
 };
 
+// Arrays marked (bulk) hold double in the kFP64 form and float in kFP32Bulk.
 struct PredictorArgs {
   const double* luh;  // [cells][B^3][V+P]
-  double* lduh;       // [cells][B^3][V]  (volume integral; may be null)
-  double* lQhbnd;     // [cells][6][B^2][V+P]
-  double* lFhbnd;     // [cells][6][B^2][V]
+  void* lduh;         // (bulk) [cells][B^3][V]  (volume integral; may be null)
+  void* lQhbnd;       // (bulk) [cells][6][B^2][V+P]
+  void* lFhbnd;       // (bulk) [cells][6][B^2][V]
   const double* dx;   // dx[cell * dx_stride + d]
   long dx_stride;     // 0: one dx for all cells, 3: per cell
   const double* dt;   // device scalar
@@ -65,14 +73,14 @@ struct PredictorArgs {
 struct RiemannArgs {
   const FaceDesc* faces;
   long nfaces;
-  const double* lQhbnd;
-  double* lFhbnd;
+  const void* lQhbnd;  // (bulk)
+  void* lFhbnd;        // (bulk)
 };
 
 struct UpdateArgs {
   double* luh;
-  double* lduh;
-  const double* lFhbnd;
+  void* lduh;           // (bulk)
+  const void* lFhbnd;   // (bulk)
   const double* dx;
   long dx_stride;
   const double* dt;
@@ -108,6 +116,8 @@ struct ICArgs {
 };
 
 bool supported(const SolverDesc& sd);
+// Kernels that only depend on (order, nvar): volume and surface integrals.
+bool supported_sizes(const SolverDesc& sd);
 hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a,
                             long ncells, hipStream_t s);
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s);

@@ -1,12 +1,11 @@
-// aderdg_fp64.hip -- FP64 ADER-DG time-step kernels for gfx950 (CDNA4).
+// aderdg_kernels.hip -- ADER-DG time-step kernels for gfx950 (CDNA4), templated
+// on the arithmetic type T of the bulk work (double: the plain form; float:
+// the FP32-bulk form).
 //
 // One workgroup per cell for the cell-local stages, one thread per spatial
 // node (z,y,x) of the B^3 Gauss-Legendre nodes, padded to whole 64-wide
-// wavefronts.  A thread keeps its node's space-time predictor for all B time
-// nodes in registers; the directional contractions read one time slice of
-// fluxes that the workgroup stages in LDS.  All HBM arrays use the
-// reference's per-cell layouts (variable innermost), so a cell is one
-// contiguous, coalesced block.
+// wavefronts.  All HBM arrays use the reference's per-cell layouts (variable
+// innermost), so a cell is one contiguous, coalesced block.
 //
 // Reference semantics followed (K = ExaHyPE/kernels/aderdg/generic/c/3d):
 //   predictor_kernel   K/spaceTimePredictorNonlinear.cpph:42-528,661-717
@@ -20,36 +19,50 @@
 
 namespace exa {
 
-// S = algebraicSource(Q) - NCP(Q, gradQ) at node (z,y,x) of time slice whose
+// Picard stopping rule (spaceTimePredictorNonlinear.cpph:314-318): stop when
+// sum (lQi_new - lQi)^2 < 1e-14.  In float that absolute bound lies below the
+// rounding floor of the iterate (about (6e-8 |q|)^2 per entry), so the
+// FP32-bulk form stops when the root-mean-square change falls below
+// kPicardRelTol32 times the root-mean-square of the iterate (or below the
+// reference's bound, whichever is reached first).
+constexpr double kPicardTol2 = 1e-7 * 1e-7;
+constexpr float kPicardRelTol32 = 1e-6f;
+
+template <typename T>
+__device__ constexpr T tc(double v) {
+  return static_cast<T>(v);
+}
+
+// S = algebraicSource(Q) - NCP(Q, gradQ) at node (z,y,x) of a time slice whose
 // variables are staged in sQ[var][node] (spaceTimePredictorNonlinear.cpph:238-266).
-template <int B, class PDE, bool USE_SOURCE, bool USE_NCP>
-__device__ inline void source_ncp(const double* Q, const double* sQ, const double* sDu,
-                                  const double invDx[3], int z, int y, int x, double* S) {
+template <int B, class PDE, bool USE_SOURCE, bool USE_NCP, typename T>
+__device__ inline void source_ncp(const T* Q, const T* sQ, const T* sDu, const T invDx[3], int z, int y,
+                                  int x, T* S) {
   constexpr int V = PDE::V;
   constexpr int B2 = B * B, B3 = B2 * B;
   if (USE_SOURCE) {
     PDE::algebraic_source(Q, S);
   } else {
 #pragma unroll
-    for (int m = 0; m < V; ++m) S[m] = 0.0;
+    for (int m = 0; m < V; ++m) S[m] = T(0);
   }
   if (USE_NCP) {
     // gradQ(dir, var) = sum_n invDx * lQi(..n..) * dudx[l][n] (:170-173, 199-202, 228-231)
-    double gradQ[3][V];
+    T gradQ[3][V];
 #pragma unroll
     for (int m = 0; m < V; ++m) {
-      double gx = 0.0, gy = 0.0, gz = 0.0;
+      T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
       for (int n = 0; n < B; ++n) {
-        gx += 1.0 * invDx[0] * sQ[m * B3 + (z * B + y) * B + n] * sDu[x * B + n];
-        gy += 1.0 * invDx[1] * sQ[m * B3 + (z * B + n) * B + x] * sDu[y * B + n];
-        gz += 1.0 * invDx[2] * sQ[m * B3 + (n * B + y) * B + x] * sDu[z * B + n];
+        gx += T(1) * invDx[0] * sQ[m * B3 + (z * B + y) * B + n] * sDu[x * B + n];
+        gy += T(1) * invDx[1] * sQ[m * B3 + (z * B + n) * B + x] * sDu[y * B + n];
+        gz += T(1) * invDx[2] * sQ[m * B3 + (n * B + y) * B + x] * sDu[z * B + n];
       }
       gradQ[0][m] = gx;
       gradQ[1][m] = gy;
       gradQ[2][m] = gz;
     }
-    double ncp[V];
+    T ncp[V];
     PDE::ncp(Q, gradQ, ncp);
 #pragma unroll
     for (int m = 0; m < V; ++m) S[m] -= ncp[m];
@@ -58,8 +71,17 @@ __device__ inline void source_ncp(const double* Q, const double* sQ, const doubl
 
 // ---------------------------------------------------------------------------
 // Space-time predictor + time averaging + extrapolation + volume integral.
+//
+// Per Picard iteration and time slice t, three phases separated by barriers:
+//  1. node owners evaluate the fluxes of their node and stage them in LDS as
+//     sF[dir][var][node];
+//  2. line owners (one task = one direction, variable and grid line) contract
+//     the line with Kxi -- every staged value is read once, Kxi comes in as
+//     scalar constants -- and stage the directional contributions in sR;
+//  3. node owners form rhs(t) and fold it into the next iterate through iK1
+//     (the reference's n order), so rhs is never stored for all t.
 // ---------------------------------------------------------------------------
-template <int B, class PDE, int TERMS, bool EXPORT>
+template <int B, class PDE, int TERMS, bool EXPORT, typename T>
 __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
@@ -69,14 +91,22 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   constexpr bool USE_SOURCE = (TERMS & kSource) != 0;
   constexpr bool USE_NCP = (TERMS & kNCP) != 0;
   constexpr bool USE_SRC_NCP = USE_SOURCE || USE_NCP;
+  constexpr bool IS_FP64 = sizeof(T) == sizeof(double);
   constexpr int MAX_ITER = 2 * B;  // spaceTimePredictorNonlinear.cpph:85
   // LDS: one slice of fluxes [dir][var][node] (reused for lQhi / lFhi staging)
   constexpr int SF = (3 * V > D ? 3 * V : D) * B3;
-  __shared__ double sF[SF];
-  __shared__ double sQ[USE_NCP ? V * B3 : 1];
-  __shared__ double sK[B2];   // Kxi[n][l]
-  __shared__ double sDu[B2];  // dudx[l][n]
-  __shared__ double sRed[NT / kWave];
+  __shared__ T sF[SF];
+  __shared__ T sR[USE_FLUX ? 3 * V * B3 : 1];  // directional contributions [dir][var][node]
+  __shared__ T sQ[USE_NCP ? V * B3 : 1];
+  __shared__ T sK[B2];   // Kxi[n][l]
+  __shared__ T sDu[B2];  // dudx[l][n]
+  __shared__ T sW[B];    // Gauss-Legendre weights
+  __shared__ T sInvDx[3];
+  __shared__ T sRed[2 * (NT / kWave)];
+
+  T* const out_lduh = static_cast<T*>(a.lduh);
+  T* const out_lQhbnd = static_cast<T*>(a.lQhbnd);
+  T* const out_lFhbnd = static_cast<T*>(a.lFhbnd);
 
   const size_t cell = (size_t)blockIdx.x + a.cell0;
   const int tid = threadIdx.x;
@@ -85,24 +115,32 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   const int z = node / B2, y = (node / B) % B, x = node % B;
 
   for (int i = tid; i < B2; i += NT) {
-    sK[i] = exa_kxi[N][i / B][i % B];
-    sDu[i] = exa_dudx[N][i / B][i % B];
+    sK[i] = tc<T>(exa_kxi[N][i / B][i % B]);
+    sDu[i] = tc<T>(exa_dudx[N][i / B][i % B]);
   }
+  for (int i = tid; i < B; i += NT) sW[i] = tc<T>(exa_gl_weights[N][i]);
 
   const double* dxp = a.dx + cell * a.dx_stride;
-  const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
-  const double dt = *a.dt;
+  const T invDx[3] = {T(1.0) / T(dxp[0]), T(1.0) / T(dxp[1]), T(1.0) / T(dxp[2])};
+  const T dt = T(*a.dt);
+  // static indices only: a lane-indexed private array is placed in scratch memory,
+  // where only the lanes that index it would see it initialised
+  if (tid == 0) {
+    sInvDx[0] = invDx[0];
+    sInvDx[1] = invDx[1];
+    sInvDx[2] = invDx[2];
+  }
 
-  double q0[D];
+  T q0[D];
   {
     const double* luh = a.luh + cell * (size_t)(B3 * D) + node * D;
 #pragma unroll
-    for (int m = 0; m < D; ++m) q0[m] = luh[m];
+    for (int m = 0; m < D; ++m) q0[m] = T(luh[m]);
   }
-  const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
-  const double w3 = wz * wy * wx;  // weight of rhs0 (:130-132) and of iK1 (:286-288)
+  const T wx = tc<T>(exa_gl_weights[N][x]), wy = tc<T>(exa_gl_weights[N][y]), wz = tc<T>(exa_gl_weights[N][z]);
+  const T w3 = wz * wy * wx;  // weight of rhs0 (:130-132) and of iK1 (:286-288)
 
-  double qi[B][V];  // lQi(z,y,x,t,var) of this node, variables only
+  T qi[B][V];  // lQi(z,y,x,t,var) of this node, variables only
 #pragma unroll
   for (int t = 0; t < B; ++t)
 #pragma unroll
@@ -110,27 +148,32 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 
   // Picard state: qi = current iterate lQi^k, qn = next iterate lQi^{k+1}
   // accumulated slice by slice (iK1 applied in the reference's n order).
-  double qn[B][V];
+  T qn[B][V];
   int iters = MAX_ITER;
-  const double iweight = 1.0 / w3;
+  const T iweight = T(1.0) / w3;
   __syncthreads();
 
 #pragma unroll 1
   for (int iter = 0; iter < MAX_ITER; ++iter) {
+    // rhs0 = w^3 FLCoeff[t] luh is loop invariant; keeping B*V copies of it live
+    // across iterations costs more registers than recomputing it per slice.
+    T w3i = w3;
+    asm volatile("" : "+v"(w3i));
 #pragma unroll
     for (int l = 0; l < B; ++l)
 #pragma unroll
-      for (int m = 0; m < V; ++m) qn[l][m] = 0.0;
+      for (int m = 0; m < V; ++m) qn[l][m] = T(0);
 #pragma unroll
     for (int t = 0; t < B; ++t) {
-      double Q[D];
+      const T wt = tc<T>(exa_gl_weights[N][t]);
+      T Q[D];
 #pragma unroll
       for (int m = 0; m < V; ++m) Q[m] = qi[t][m];
 #pragma unroll
       for (int m = V; m < D; ++m) Q[m] = q0[m];
-      const double wt = exa_gl_weights[N][t];
+      // phase 1 (node owners): fluxes of this time slice -> sF[dir][var][node]
       if (USE_FLUX) {
-        double F[3][V];
+        T F[3][V];
         PDE::flux(Q, F);
         if (active) {
 #pragma unroll
@@ -152,51 +195,54 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       }
       __syncthreads();
 
-      // rhs0 = w^3 * FLCoeff[t] * luh (:124-141)
-      double r[V];
-      const double fl_t = exa_fl_coeff[N][t];
-#pragma unroll
-      for (int m = 0; m < V; ++m) r[m] = w3 * fl_t * q0[m];
+      // phase 2 (line owners): one task = one (direction, variable, line);
+      // out(l) = sum_n (us * F(n)) * Kxi[n][l] along the line, with
+      // us = w_t w_a w_b dt / dx_dir (:152-236).  Kxi comes in as constants.
       if (USE_FLUX) {
-        // x direction (:152-178): rhs(t,z,y,x) -= us * lFi(t,z,y,n,x-flux) * Kxi[n][x]
-        {
-          const double us = wt * wz * wy * dt * invDx[0];
-          const double* row = sF + (0 * V) * B3 + (z * B + y) * B;
-#pragma unroll 2
-          for (int n = 0; n < B; ++n) {
-            const double k = sK[n * B + x];
+        for (int task = tid; task < 3 * B2 * V; task += NT) {
+          const int d = task / (B2 * V);
+          const int rem = task - d * (B2 * V);
+          const int m = rem / B2;
+          const int line = rem - m * B2;
+          const int la = line / B, lb = line - (line / B) * B;
+          // x-lines (z,y): base (z B + y) B, stride 1; y-lines (z,x): z B^2 + x, stride B;
+          // z-lines (y,x): y B + x, stride B^2
+          const int base = d == 0 ? line * B : (d == 1 ? la * B2 + lb : line);
+          const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
+          const T us = wt * sW[la] * sW[lb] * dt * sInvDx[d];
+          const T* src = sF + (d * V + m) * B3 + base;
+          T f[B];
 #pragma unroll
-            for (int m = 0; m < V; ++m) r[m] -= us * row[m * B3 + n] * k;
+          for (int n = 0; n < B; ++n) f[n] = us * src[n * stride];
+          T* dst = sR + (d * V + m) * B3 + base;
+#pragma unroll
+          for (int l = 0; l < B; ++l) {
+            T acc = T(0);
+#pragma unroll
+            for (int n = 0; n < B; ++n) acc += f[n] * tc<T>(exa_kxi[N][n][l]);
+            dst[l * stride] = acc;
           }
         }
-        // y direction (:181-207)
-        {
-          const double us = wt * wz * wx * dt * invDx[1];
-          const double* col = sF + (1 * V) * B3 + z * B2 + x;
-#pragma unroll 2
-          for (int n = 0; n < B; ++n) {
-            const double k = sK[n * B + y];
+        __syncthreads();
+      }
+
+      // phase 3 (node owners): rhs = w^3 FLCoeff[t] luh - contributions (+ S dt w^4)
+      // (:124-141, :238-276); lQi_new(l) += iweight * rhs * iK1[l][t] (:292-299, n = t)
+      T r[V];
+      const T fl_t = tc<T>(exa_fl_coeff[N][t]);
 #pragma unroll
-            for (int m = 0; m < V; ++m) r[m] -= us * col[m * B3 + n * B] * k;
-          }
-        }
-        // z direction (:210-236)
-        {
-          const double us = wt * wy * wx * dt * invDx[2];
-          const double* col = sF + (2 * V) * B3 + y * B + x;
-#pragma unroll 2
-          for (int n = 0; n < B; ++n) {
-            const double k = sK[n * B + z];
-#pragma unroll
-            for (int m = 0; m < V; ++m) r[m] -= us * col[m * B3 + n * B2] * k;
-          }
+      for (int m = 0; m < V; ++m) {
+        r[m] = w3i * fl_t * q0[m];
+        if (USE_FLUX) {
+          r[m] -= sR[(0 * V + m) * B3 + node];
+          r[m] -= sR[(1 * V + m) * B3 + node];
+          r[m] -= sR[(2 * V + m) * B3 + node];
         }
       }
       if (USE_SRC_NCP) {
-        // source and non-conservative product (:238-276)
-        double S[V];
+        T S[V];
         source_ncp<B, PDE, USE_SOURCE, USE_NCP>(Q, sQ, sDu, invDx, z, y, x, S);
-        const double us = wt * wz * wy * wx * dt;
+        const T us = wt * wz * wy * wx * dt;
 #pragma unroll
         for (int m = 0; m < V; ++m) r[m] += us * S[m];
         if (EXPORT && active) {
@@ -204,29 +250,39 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 #pragma unroll
           for (int m = 0; m < V; ++m) lFi[m] = S[m];
         }
+        if (USE_NCP && !USE_FLUX) __syncthreads();  // sQ is rewritten by the next slice
       }
-      // lQi_new(l) += iweight * rhs(t) * iK1[l][t]   (:292-299, n = t)
 #pragma unroll
       for (int m = 0; m < V; ++m) {
-        const double ir = iweight * r[m];
+        const T ir = iweight * r[m];
 #pragma unroll
-        for (int l = 0; l < B; ++l) qn[l][m] += ir * exa_ik1[N][l][t];
+        for (int l = 0; l < B; ++l) qn[l][m] += ir * tc<T>(exa_ik1[N][l][t]);
       }
-      __syncthreads();  // the next slice overwrites sF / sQ
     }
 
     // squared residual (:303) and exit test (:314-318)
-    double sq = 0.0;
+    T sq = T(0), qq = T(0);
 #pragma unroll
     for (int l = 0; l < B; ++l)
 #pragma unroll
       for (int m = 0; m < V; ++m) {
-        const double dq = qn[l][m] - qi[l][m];
+        const T dq = qn[l][m] - qi[l][m];
         sq += dq * dq;
+        if (!IS_FP64) qq += qn[l][m] * qn[l][m];
       }
-    if (!active) sq = 0.0;
-    const double sq_res = block_sum<NT>(sq, sRed);
-    const bool done = sq_res < 1e-7 * 1e-7;
+    if (!active) {
+      sq = T(0);
+      qq = T(0);
+    }
+    bool done;
+    if (IS_FP64) {
+      const T sq_res = block_sum<NT>(sq, sRed);
+      done = sq_res < T(kPicardTol2);
+    } else {
+      const T sq_res = block_sum<NT>(sq, sRed);
+      const T qq_sum = block_sum<NT>(qq, sRed + NT / kWave);
+      done = sq_res < T(kPicardTol2) || sq_res < kPicardRelTol32 * kPicardRelTol32 * qq_sum;
+    }
     if (done || iter + 1 == MAX_ITER) {
       iters = iter + 1;
       break;  // qi still holds lQi^k, the iterate the last fluxes were taken from
@@ -237,87 +293,33 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       for (int m = 0; m < V; ++m) qi[l][m] = qn[l][m];
   }
 
-  // Time-averaged fluxes of the last flux evaluation, i.e. of lQi^k (the
-  // reference computes lFi before the final lQi update, :96-122 vs :306):
-  // lFhi = sum_t w_t F(lQi^k(t)), lShi likewise (:367-396).
-  double fh[3][V];
-  double sh[V];
-#pragma unroll
-  for (int m = 0; m < V; ++m) {
-    fh[0][m] = 0.0;
-    fh[1][m] = 0.0;
-    fh[2][m] = 0.0;
-    sh[m] = 0.0;
-  }
-#pragma unroll
-  for (int t = 0; t < B; ++t) {
-    double Q[D];
-#pragma unroll
-    for (int m = 0; m < V; ++m) Q[m] = qi[t][m];
-#pragma unroll
-    for (int m = V; m < D; ++m) Q[m] = q0[m];
-    const double wt = exa_gl_weights[N][t];
-    if (USE_FLUX) {
-      double F[3][V];
-      PDE::flux(Q, F);
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int m = 0; m < V; ++m) fh[d][m] += F[d][m] * wt;
-    }
-    if (USE_SRC_NCP) {
-      if (USE_NCP && active) {
-#pragma unroll
-        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = qi[t][m];
-      }
-      if (USE_NCP) __syncthreads();
-      double S[V];
-      source_ncp<B, PDE, USE_SOURCE, USE_NCP>(Q, sQ, sDu, invDx, z, y, x, S);
-#pragma unroll
-      for (int m = 0; m < V; ++m) sh[m] += S[m] * wt;
-      if (USE_NCP) __syncthreads();
-    }
-  }
-  // the final iterate lQi^{k+1}
-#pragma unroll
-  for (int l = 0; l < B; ++l)
-#pragma unroll
-    for (int m = 0; m < V; ++m) qi[l][m] = qn[l][m];
-
   // ---- epilogue: time average, extrapolation, volume integral ----
   const size_t cQb = cell * (size_t)(6 * B2 * D);
   const size_t cFb = cell * (size_t)(6 * B2 * V);
-  // lQhi = sum_t lQi(t) w_t (:397-404)
-  double qh[D];
+  {
+    // lQhi = sum_t lQi(t) w_t of the final iterate lQi^{k+1} = qn (:397-404)
+    T qh[D];
 #pragma unroll
-  for (int m = 0; m < D; ++m) {
-    double acc = 0.0;
+    for (int m = 0; m < D; ++m) {
+      T acc = T(0);
 #pragma unroll
-    for (int t = 0; t < B; ++t) acc += (m < V ? qi[t][m < V ? m : 0] : q0[m]) * exa_gl_weights[N][t];
-    qh[m] = acc;
-  }
-  if (EXPORT && active) {
-    double* lQi = a.lQi + cell * (size_t)(B3 * B * D) + (size_t)node * B * D;
-#pragma unroll
-    for (int t = 0; t < B; ++t)
-#pragma unroll
-      for (int m = 0; m < D; ++m) lQi[t * D + m] = (m < V) ? qi[t][m < V ? m : 0] : q0[m];
-    double* lQhi = a.lQhi + cell * (size_t)(B3 * D) + (size_t)node * D;
-#pragma unroll
-    for (int m = 0; m < D; ++m) lQhi[m] = qh[m];
-    // lFhi blocks: x (z,y,x), y (z,x,y), z (y,x,z), src (z,y,x)  (:375-393)
-    double* lFhi = a.lFhi + cell * (size_t)(4 * B3 * V);
-#pragma unroll
-    for (int m = 0; m < V; ++m) {
-      lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + m] = fh[0][m];
-      lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + m] = fh[1][m];
-      lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + m] = fh[2][m];
-      lFhi[3 * B3 * V + ((z * B + y) * B + x) * V + m] = sh[m];
+      for (int t = 0; t < B; ++t) acc += (m < V ? qn[t][m < V ? m : 0] : q0[m]) * tc<T>(exa_gl_weights[N][t]);
+      qh[m] = acc;
     }
-  }
-  if (active) {
+    if (EXPORT && active) {
+      double* lQi = a.lQi + cell * (size_t)(B3 * B * D) + (size_t)node * B * D;
 #pragma unroll
-    for (int m = 0; m < D; ++m) sF[m * B3 + node] = qh[m];
+      for (int t = 0; t < B; ++t)
+#pragma unroll
+        for (int m = 0; m < D; ++m) lQi[t * D + m] = (m < V) ? qn[t][m < V ? m : 0] : q0[m];
+      double* lQhi = a.lQhi + cell * (size_t)(B3 * D) + (size_t)node * D;
+#pragma unroll
+      for (int m = 0; m < D; ++m) lQhi[m] = qh[m];
+    }
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < D; ++m) sF[m * B3 + node] = qh[m];
+    }
   }
   __syncthreads();
   // lQhbnd (:434-527): face f, face node (a,b), datum k; reduce the normal line.
@@ -332,12 +334,65 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     // x-faces: (z=a,y=b,x=l); y-faces: (z=a,y=l,x=b); z-faces: (z=l,y=a,x=b)
     const int base = dir == 0 ? (fa * B + fb) * B : (dir == 1 ? fa * B2 + fb : fa * B + fb);
     const int stride = dir == 0 ? 1 : (dir == 1 ? B : B2);
-    double acc = 0.0;
+    T acc = T(0);
 #pragma unroll
-    for (int l = 0; l < B; ++l) acc += sF[k * B3 + base + l * stride] * coef[l];
-    a.lQhbnd[cQb + o] = acc;
+    for (int l = 0; l < B; ++l) acc += sF[k * B3 + base + l * stride] * T(coef[l]);
+    out_lQhbnd[cQb + o] = acc;
   }
   __syncthreads();
+
+  // Time-averaged fluxes of the last flux evaluation, i.e. of lQi^k = qi (the
+  // reference computes lFi before the final lQi update, :96-122 vs :306):
+  // lFhi = sum_t w_t F(lQi^k(t)), lShi likewise (:367-396).
+  T fh[3][V];
+  T sh[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    fh[0][m] = T(0);
+    fh[1][m] = T(0);
+    fh[2][m] = T(0);
+    sh[m] = T(0);
+  }
+#pragma unroll
+  for (int t = 0; t < B; ++t) {
+    T Q[D];
+#pragma unroll
+    for (int m = 0; m < V; ++m) Q[m] = qi[t][m];
+#pragma unroll
+    for (int m = V; m < D; ++m) Q[m] = q0[m];
+    const T wt = tc<T>(exa_gl_weights[N][t]);
+    if (USE_FLUX) {
+      T F[3][V];
+      PDE::flux(Q, F);
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < V; ++m) fh[d][m] += F[d][m] * wt;
+    }
+    if (USE_SRC_NCP) {
+      if (USE_NCP && active) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = qi[t][m];
+      }
+      if (USE_NCP) __syncthreads();
+      T S[V];
+      source_ncp<B, PDE, USE_SOURCE, USE_NCP>(Q, sQ, sDu, invDx, z, y, x, S);
+#pragma unroll
+      for (int m = 0; m < V; ++m) sh[m] += S[m] * wt;
+      if (USE_NCP) __syncthreads();
+    }
+  }
+  if (EXPORT && active) {
+    // lFhi blocks: x (z,y,x), y (z,x,y), z (y,x,z), src (z,y,x)  (:375-393)
+    double* lFhi = a.lFhi + cell * (size_t)(4 * B3 * V);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + m] = fh[0][m];
+      lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + m] = fh[1][m];
+      lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + m] = fh[2][m];
+      lFhi[3 * B3 * V + ((z * B + y) * B + x) * V + m] = sh[m];
+    }
+  }
   if (active) {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -357,39 +412,39 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       const double* coef = (f & 1) ? exa_fr_coeff[N] : exa_fl_coeff[N];
       const int base = dir == 0 ? (fa * B + fb) * B : (dir == 1 ? fa * B2 + fb : fa * B + fb);
       const int stride = dir == 0 ? 1 : (dir == 1 ? B : B2);
-      double acc = 0.0;
+      T acc = T(0);
 #pragma unroll
-      for (int l = 0; l < B; ++l) acc += sF[(dir * V + k) * B3 + base + l * stride] * coef[l];
-      a.lFhbnd[cFb + o] = acc;
+      for (int l = 0; l < B; ++l) acc += sF[(dir * V + k) * B3 + base + l * stride] * T(coef[l]);
+      out_lFhbnd[cFb + o] = acc;
     }
   } else {
-    for (int o = tid; o < 6 * B2 * V; o += NT) a.lFhbnd[cFb + o] = 0.0;
+    for (int o = tid; o < 6 * B2 * V; o += NT) out_lFhbnd[cFb + o] = T(0);
   }
   // volume integral (volumeIntegralNonlinear.cpph:136-225), time-averaged form
-  if (a.lduh && active) {
-    double acc[V];
+  if (out_lduh && active) {
+    T acc[V];
 #pragma unroll
-    for (int m = 0; m < V; ++m) acc[m] = 0.0;
+    for (int m = 0; m < V; ++m) acc[m] = T(0);
     if (USE_FLUX) {
       {
-        const double us = wz * wy * invDx[0];
-        const double* row = sF + (0 * V) * B3 + (z * B + y) * B;
+        const T us = wz * wy * invDx[0];
+        const T* row = sF + (0 * V) * B3 + (z * B + y) * B;
 #pragma unroll
         for (int m = 0; m < V; ++m)
 #pragma unroll
           for (int n = 0; n < B; ++n) acc[m] += sK[x * B + n] * row[m * B3 + n] * us;
       }
       {
-        const double us = wz * wx * invDx[1];
-        const double* col = sF + (1 * V) * B3 + z * B2 + x;
+        const T us = wz * wx * invDx[1];
+        const T* col = sF + (1 * V) * B3 + z * B2 + x;
 #pragma unroll
         for (int m = 0; m < V; ++m)
 #pragma unroll
           for (int n = 0; n < B; ++n) acc[m] += sK[y * B + n] * col[m * B3 + n * B] * us;
       }
       {
-        const double us = wy * wx * invDx[2];
-        const double* col = sF + (2 * V) * B3 + y * B + x;
+        const T us = wy * wx * invDx[2];
+        const T* col = sF + (2 * V) * B3 + y * B + x;
 #pragma unroll
         for (int m = 0; m < V; ++m)
 #pragma unroll
@@ -400,7 +455,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 #pragma unroll
       for (int m = 0; m < V; ++m) acc[m] += w3 * sh[m];
     }
-    double* lduh = a.lduh + cell * (size_t)(B3 * V) + node * V;
+    T* lduh = out_lduh + cell * (size_t)(B3 * V) + node * V;
 #pragma unroll
     for (int m = 0; m < V; ++m) lduh[m] = acc[m];
   }
@@ -410,7 +465,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 // ---------------------------------------------------------------------------
 // Rusanov flux on a list of faces.  One wavefront per face, lane = face node.
 // ---------------------------------------------------------------------------
-template <int B, class PDE, bool USE_NCP>
+template <int B, class PDE, bool USE_NCP, typename T>
 __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
@@ -423,37 +478,37 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
   const int node = active ? lane : 0;
   const FaceDesc fd = a.faces[f];
   const int d = fd.dir;
-  double* FL = a.lFhbnd + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * V);
-  double* FR = a.lFhbnd + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
-  const double* QL = a.lQhbnd + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * D);
-  const double* QR = a.lQhbnd + ((size_t)fd.right * 6 + 2 * d) * (B2 * D);
+  T* FL = static_cast<T*>(a.lFhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * V);
+  T* FR = static_cast<T*>(a.lFhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
+  const T* QL = static_cast<const T*>(a.lQhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * D);
+  const T* QR = static_cast<const T*>(a.lQhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * D);
   const int i = node / B, j = node % B;
-  const double weight = exa_gl_weights[N][i] * exa_gl_weights[N][j];
-  double qL[D], qR[D], QavL[D], QavR[D];
+  const T weight = tc<T>(exa_gl_weights[N][i]) * tc<T>(exa_gl_weights[N][j]);
+  T qL[D], qR[D], QavL[D], QavR[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     qL[k] = QL[node * D + k];
     qR[k] = QR[node * D + k];
-    QavL[k] = wave_sum(active ? weight * qL[k] : 0.0);  // :49-66
-    QavR[k] = wave_sum(active ? weight * qR[k] : 0.0);
+    QavL[k] = wave_sum(active ? weight * qL[k] : T(0));  // :49-66
+    QavR[k] = wave_sum(active ? weight * qR[k] : T(0));
   }
-  const double sL = PDE::max_abs_eigenvalue(QavL, d);  // :68-78
-  const double sR = PDE::max_abs_eigenvalue(QavR, d);
-  const double smax = (sL < sR) ? sR : sL;
-  double ncp[V];
+  const T sL = PDE::max_abs_eigenvalue(QavL, d);  // :68-78
+  const T sR = PDE::max_abs_eigenvalue(QavR, d);
+  const T smax = (sL < sR) ? sR : sL;
+  T ncp[V];
 #pragma unroll
-  for (int k = 0; k < V; ++k) ncp[k] = 0.0;
+  for (int k = 0; k < V; ++k) ncp[k] = T(0);
   if (USE_NCP) {
-    double gradQ[3][V];
-    double Qavg[D];
+    T gradQ[3][V];
+    T Qavg[D];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-      gradQ[0][k] = 0.0;
-      gradQ[1][k] = 0.0;
-      gradQ[2][k] = 0.0;
+      gradQ[0][k] = T(0);
+      gradQ[1][k] = T(0);
+      gradQ[2][k] = T(0);
     }
 #pragma unroll
-    for (int k = 0; k < D; ++k) Qavg[k] = 0.5 * (qR[k] + qL[k]);
+    for (int k = 0; k < D; ++k) Qavg[k] = T(0.5) * (qR[k] + qL[k]);
 #pragma unroll
     for (int k = 0; k < V; ++k) gradQ[d][k] = qR[k] - qL[k];
     PDE::ncp(Qavg, gradQ, ncp);
@@ -461,11 +516,11 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
   if (!active) return;
 #pragma unroll
   for (int k = 0; k < V; ++k) {
-    const double fl = FL[node * V + k], fr = FR[node * V + k];
-    const double F = 0.5 * (fr + fl) - 0.5 * smax * (qR[k] - qL[k]);  // :102-112
+    const T fl = FL[node * V + k], fr = FR[node * V + k];
+    const T F = T(0.5) * (fr + fl) - T(0.5) * smax * (qR[k] - qL[k]);  // :102-112
     if (USE_NCP) {
-      FR[node * V + k] = F - 0.5 * ncp[k];
-      FL[node * V + k] = F + 0.5 * ncp[k];
+      FR[node * V + k] = F - T(0.5) * ncp[k];
+      FL[node * V + k] = F + T(0.5) * ncp[k];
     } else {
       FR[node * V + k] = F;
       FL[node * V + k] = F;
@@ -474,14 +529,15 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Surface integral + solution update + the cell's stable time step.
+// Surface integral + solution update + the cell's stable time step, in double
+// on the double solution; TF is the storage type of lduh and the face fluxes.
 // ---------------------------------------------------------------------------
 __device__ inline void atomic_min_positive(unsigned long long* slot, double v) {
   // For non-negative doubles the IEEE bit patterns order like the values.
   atomicMin(slot, (unsigned long long)__double_as_longlong(v));
 }
 
-template <int B, class PDE, bool SURFACE, bool UPDATE, bool WRITE_LDUH>
+template <int B, class PDE, bool SURFACE, bool UPDATE, bool WRITE_LDUH, typename TF>
 __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
@@ -501,32 +557,32 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
   for (int m = 0; m < D; ++m) q[m] = luh[m];
   if (SURFACE || UPDATE) {
     const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
-    double* lduhp = a.lduh + cell * (size_t)(B3 * V) + node * V;
+    TF* lduhp = static_cast<TF*>(a.lduh) + cell * (size_t)(B3 * V) + node * V;
     double du[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) du[m] = lduhp[m];
     if (SURFACE) {
-      const double* lF = a.lFhbnd + cell * (size_t)(6 * B2 * V);
+      const TF* lF = static_cast<const TF*>(a.lFhbnd) + cell * (size_t)(6 * B2 * V);
       {  // x faces, (z,y)
         const double us = wz * wy * invDx[0];
         const double fr = exa_fr_coeff[N][x], fl = exa_fl_coeff[N][x];
 #pragma unroll
         for (int m = 0; m < V; ++m)
-          du[m] -= (lF[(1 * B2 + z * B + y) * V + m] * fr - lF[(0 * B2 + z * B + y) * V + m] * fl) * us;
+          du[m] -= ((double)lF[(1 * B2 + z * B + y) * V + m] * fr - (double)lF[(0 * B2 + z * B + y) * V + m] * fl) * us;
       }
       {  // y faces, (z,x)
         const double us = wz * wx * invDx[1];
         const double fr = exa_fr_coeff[N][y], fl = exa_fl_coeff[N][y];
 #pragma unroll
         for (int m = 0; m < V; ++m)
-          du[m] -= (lF[(3 * B2 + z * B + x) * V + m] * fr - lF[(2 * B2 + z * B + x) * V + m] * fl) * us;
+          du[m] -= ((double)lF[(3 * B2 + z * B + x) * V + m] * fr - (double)lF[(2 * B2 + z * B + x) * V + m] * fl) * us;
       }
       {  // z faces, (y,x)
         const double us = wy * wx * invDx[2];
         const double fr = exa_fr_coeff[N][z], fl = exa_fl_coeff[N][z];
 #pragma unroll
         for (int m = 0; m < V; ++m)
-          du[m] -= (lF[(5 * B2 + y * B + x) * V + m] * fr - lF[(4 * B2 + y * B + x) * V + m] * fl) * us;
+          du[m] -= ((double)lF[(5 * B2 + y * B + x) * V + m] * fr - (double)lF[(4 * B2 + y * B + x) * V + m] * fl) * us;
       }
     }
     if (UPDATE) {
@@ -541,7 +597,7 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
     }
     if (WRITE_LDUH && active) {
 #pragma unroll
-      for (int m = 0; m < V; ++m) lduhp[m] = du[m];
+      for (int m = 0; m < V; ++m) lduhp[m] = TF(du[m]);
     }
   }
   if (a.dt_slots || a.dt_cell) {
@@ -576,7 +632,7 @@ __global__ __launch_bounds__(256) void dt_finalize_kernel(StepState* st, int adv
 }
 
 // ---------------------------------------------------------------------------
-// Stand-alone per-cell kernels for the drop-in single-cell / batch API.
+// Stand-alone per-cell kernels for the drop-in single-cell / batch API (FP64).
 // ---------------------------------------------------------------------------
 
 // volumeIntegralNonlinear (time-averaged form) from caller-provided lFhi.
@@ -671,98 +727,97 @@ __global__ __launch_bounds__(256) void count_iterations_kernel(const int* iters,
 }
 
 // ---------------------------------------------------------------------------
-// Host-side dispatch: runtime (pde, order, npar, terms) -> instantiation.
+// Host-side dispatch: runtime (pde, order, npar, terms, precision) -> instantiation.
 // ---------------------------------------------------------------------------
 #define EXA_ORDERS(X) X(2) X(3) X(4) X(5) X(6) X(7)
 
-template <int B, class PDE>
-static hipError_t launch_predictor_t(int terms, bool exp, const PredictorArgs& a, long ncells,
-                                     hipStream_t s) {
-  const dim3 grid((unsigned)ncells), block(node_threads(B));
-#define EXA_PRED(T)                                                                          \
-  if (terms == (T)) {                                                                        \
-    if (exp)                                                                                 \
-      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), true>), grid, block, 0, s, a);       \
-    else                                                                                     \
-      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), false>), grid, block, 0, s, a);      \
-    return hipGetLastError();                                                                \
-  }
-  EXA_PRED(kFlux)
-  EXA_PRED(kFlux | kSource | kNCP)
-#undef EXA_PRED
-  return hipErrorNotSupported;
-}
-
-hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a,
-                            long ncells, hipStream_t s) {
-  if (ncells <= 0) return hipSuccess;
+// Calls F.template operator()<B, PDE>() for the compiled (pde, order, npar).
+template <class F>
+static hipError_t dispatch_pde(const SolverDesc& sd, F&& f) {
   if (sd.pde == kPdeEuler && sd.npar == 0) {
 #define EXA_CASE(N) \
-  if (sd.order == N) return launch_predictor_t<N + 1, Euler<0>>(sd.terms, export_temps, a, ncells, s);
+  if (sd.order == N) return f.template operator()<N + 1, Euler<0>>();
     EXA_ORDERS(EXA_CASE)
 #undef EXA_CASE
   }
-  if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3)
-    return launch_predictor_t<4, Euler<1>>(sd.terms, export_temps, a, ncells, s);
+  if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3) return f.template operator()<4, Euler<1>>();
   return hipErrorNotSupported;
 }
 
-template <int B, class PDE>
-static hipError_t launch_riemann_t(int terms, const RiemannArgs& a, hipStream_t s) {
-  const dim3 grid((unsigned)((a.nfaces + 3) / 4)), block(256);
-  if (terms & kNCP)
-    hipLaunchKernelGGL((riemann_kernel<B, PDE, true>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((riemann_kernel<B, PDE, false>), grid, block, 0, s, a);
-  return hipGetLastError();
+hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a, long ncells,
+                            hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  if (export_temps && sd.precision != kFP64) return hipErrorNotSupported;
+  return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
+    const dim3 grid((unsigned)ncells), block(node_threads(B));
+    if (sd.precision == kFP32Bulk) {
+      if (sd.terms != kFlux) return hipErrorNotSupported;
+      hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float>), grid, block, 0, s, a);
+      return hipGetLastError();
+    }
+#define EXA_PRED(T)                                                                               \
+  if (sd.terms == (T)) {                                                                          \
+    if (export_temps)                                                                             \
+      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), true, double>), grid, block, 0, s, a);    \
+    else                                                                                          \
+      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), false, double>), grid, block, 0, s, a);   \
+    return hipGetLastError();                                                                     \
+  }
+    EXA_PRED(kFlux)
+    EXA_PRED(kFlux | kSource | kNCP)
+#undef EXA_PRED
+    return hipErrorNotSupported;
+  });
 }
 
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s) {
   if (a.nfaces <= 0) return hipSuccess;
-  if (sd.pde == kPdeEuler && sd.npar == 0) {
-#define EXA_CASE(N) \
-  if (sd.order == N) return launch_riemann_t<N + 1, Euler<0>>(sd.terms, a, s);
-    EXA_ORDERS(EXA_CASE)
-#undef EXA_CASE
-  }
-  if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3) return launch_riemann_t<4, Euler<1>>(sd.terms, a, s);
-  return hipErrorNotSupported;
+  return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
+    const dim3 grid((unsigned)((a.nfaces + 3) / 4)), block(256);
+    const bool ncp = (sd.terms & kNCP) != 0;
+    if (sd.precision == kFP32Bulk) {
+      if (ncp) return hipErrorNotSupported;
+      hipLaunchKernelGGL((riemann_kernel<B, PDE, false, float>), grid, block, 0, s, a);
+    } else if (ncp) {
+      hipLaunchKernelGGL((riemann_kernel<B, PDE, true, double>), grid, block, 0, s, a);
+    } else {
+      hipLaunchKernelGGL((riemann_kernel<B, PDE, false, double>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+  });
 }
 
-template <int B, class PDE>
-static hipError_t launch_update_t(int mode, const UpdateArgs& a, long ncells, hipStream_t s) {
-  const dim3 grid((unsigned)ncells), block(node_threads(B));
-  switch (mode) {
-    case kUpdSurface | kUpdUpdate:
-      hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false>), grid, block, 0, s, a);
-      break;
-    case kUpdSurface | kUpdWriteLduh:
-      hipLaunchKernelGGL((update_kernel<B, PDE, true, false, true>), grid, block, 0, s, a);
-      break;
-    case kUpdUpdate:
-      hipLaunchKernelGGL((update_kernel<B, PDE, false, true, false>), grid, block, 0, s, a);
-      break;
-    case 0:
-      hipLaunchKernelGGL((update_kernel<B, PDE, false, false, false>), grid, block, 0, s, a);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, long ncells,
-                         hipStream_t s) {
+hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, long ncells, hipStream_t s) {
   if (ncells <= 0) return hipSuccess;
-  if (sd.pde == kPdeEuler && sd.npar == 0) {
-#define EXA_CASE(N) \
-  if (sd.order == N) return launch_update_t<N + 1, Euler<0>>(mode, a, ncells, s);
-    EXA_ORDERS(EXA_CASE)
-#undef EXA_CASE
-  }
-  if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3)
-    return launch_update_t<4, Euler<1>>(mode, a, ncells, s);
-  return hipErrorNotSupported;
+  return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
+    const dim3 grid((unsigned)ncells), block(node_threads(B));
+    if (sd.precision == kFP32Bulk) {
+      if (mode == (kUpdSurface | kUpdUpdate))
+        hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false, float>), grid, block, 0, s, a);
+      else if (mode == 0)
+        hipLaunchKernelGGL((update_kernel<B, PDE, false, false, false, float>), grid, block, 0, s, a);
+      else
+        return hipErrorNotSupported;
+      return hipGetLastError();
+    }
+    switch (mode) {
+      case kUpdSurface | kUpdUpdate:
+        hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false, double>), grid, block, 0, s, a);
+        break;
+      case kUpdSurface | kUpdWriteLduh:
+        hipLaunchKernelGGL((update_kernel<B, PDE, true, false, true, double>), grid, block, 0, s, a);
+        break;
+      case kUpdUpdate:
+        hipLaunchKernelGGL((update_kernel<B, PDE, false, true, false, double>), grid, block, 0, s, a);
+        break;
+      case 0:
+        hipLaunchKernelGGL((update_kernel<B, PDE, false, false, false, double>), grid, block, 0, s, a);
+        break;
+      default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s) {
@@ -808,11 +863,17 @@ hipError_t launch_count_iterations(const int* iters, long ncells, StepState* st,
   return hipGetLastError();
 }
 
+bool supported_sizes(const SolverDesc& sd) {
+  return sd.nvar == 5 && sd.order >= 2 && sd.order <= 7 && !sd.no_time_averaging;
+}
+
 bool supported(const SolverDesc& sd) {
   if (sd.pde != kPdeEuler || sd.nvar != 5) return false;
   if (sd.no_time_averaging) return false;
+  if (sd.precision != kFP64 && sd.precision != kFP32Bulk) return false;
   if (sd.npar == 0) return sd.order >= 2 && sd.order <= 7 && sd.terms == kFlux;
-  if (sd.npar == 1) return sd.order == 3 && (sd.terms == kFlux || sd.terms == (kFlux | kSource | kNCP));
+  if (sd.npar == 1)
+    return sd.precision == kFP64 && sd.order == 3 && (sd.terms == kFlux || sd.terms == (kFlux | kSource | kNCP));
   return false;
 }
 

@@ -42,6 +42,7 @@ bool to_desc(const exa_solver* s, SolverDesc* d) {
   d->terms = s->terms;
   d->no_time_averaging = s->no_time_averaging;
   d->cfl = s->cfl;
+  d->precision = s->precision;
   return s->order >= 0 && s->order <= 9 && s->num_variables > 0 && s->num_parameters >= 0;
 }
 
@@ -101,7 +102,7 @@ static int predictor_impl(const exa_solver* s, long ncells, double* lQhbnd, doub
                           const double* luh, const double* dx, double dt, int* iters) {
   SolverDesc sd;
   if (!to_desc(s, &sd) || ncells < 1 || !lQhbnd || !lFhbnd || !luh || !dx) return EXA_E_INVALID;
-  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  if (!exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   const bool exp = lQi || lFi || lQhi || lFhi;
   DevBuf d_luh, d_qb, d_fb, d_du, d_dx, d_dt, d_it, d_qi, d_fi, d_qhi, d_fhi;
@@ -169,7 +170,7 @@ extern "C" int exa_volume_integral_nonlinear(const exa_solver* s, double* lduh, 
                                              const double* dx) {
   SolverDesc sd;
   if (!to_desc(s, &sd) || !lduh || !lFhi || !dx) return EXA_E_INVALID;
-  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  if (!exa::supported_sizes(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   DevBuf d_du, d_fhi, d_dx;
   EXA_TRY(d_du.alloc(sizeof(double) * z.B3 * z.V));
@@ -189,7 +190,14 @@ static int update_impl(const exa_solver* s, int mode, double* luh, double* lduh,
                        const double* dx, double dt, double* dt_out) {
   SolverDesc sd;
   if (!to_desc(s, &sd)) return EXA_E_INVALID;
-  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  if (sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
+  const bool surface_only = mode == (exa::kUpdSurface | exa::kUpdWriteLduh);
+  if (surface_only ? !exa::supported_sizes(sd) : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  if (surface_only) {  // PDE-independent: run the Euler instantiation of this order
+    sd.pde = exa::kPdeEuler;
+    sd.npar = 0;
+    sd.terms = exa::kFlux;
+  }
   const Sizes z(sd);
   const double one[3] = {1.0, 1.0, 1.0};
   DevBuf d_luh, d_du, d_fb, d_dx, d_dt, d_cell;
@@ -248,7 +256,7 @@ extern "C" int exa_riemann_solver_nonlinear(const exa_solver* s, double* FL, dou
   (void)dt;  // unused by the nonlinear Rusanov solver, as in the reference
   SolverDesc sd;
   if (!to_desc(s, &sd) || !FL || !FR || !QL || !QR || direction < 0 || direction > 2) return EXA_E_INVALID;
-  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  if (!exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   // Two cells: cell 0 owns face 2d+1 (left state), cell 1 owns face 2d (right state).
   DevBuf d_qb, d_fb, d_face;
@@ -288,9 +296,10 @@ struct exa_grid {
   long nfaces;
   hipStream_t stream = nullptr;
   double* luh = nullptr;
-  double* lduh = nullptr;
-  double* lQhbnd = nullptr;
-  double* lFhbnd = nullptr;
+  void* lduh = nullptr;    // bulk arrays: double (FP64 form) or float (FP32-bulk form)
+  void* lQhbnd = nullptr;
+  void* lFhbnd = nullptr;
+  size_t bulk = 8;         // bytes per bulk element
   double* d_dx = nullptr;
   int* iters = nullptr;
   FaceDesc* faces = nullptr;
@@ -425,10 +434,11 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   auto A = [&](void** p, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(p, bytes);
   };
+  g->bulk = sd.precision == exa::kFP32Bulk ? sizeof(float) : sizeof(double);
   A((void**)&g->luh, sizeof(double) * g->ncells * z.B3 * z.D);
-  A((void**)&g->lduh, sizeof(double) * g->ncells * z.B3 * z.V);
-  A((void**)&g->lQhbnd, sizeof(double) * g->ncells * 6 * z.B2 * z.D);
-  A((void**)&g->lFhbnd, sizeof(double) * g->ncells * 6 * z.B2 * z.V);
+  A((void**)&g->lduh, g->bulk * g->ncells * z.B3 * z.V);
+  A((void**)&g->lQhbnd, g->bulk * g->ncells * 6 * z.B2 * z.D);
+  A((void**)&g->lFhbnd, g->bulk * g->ncells * 6 * z.B2 * z.V);
   A((void**)&g->d_dx, sizeof(double) * 3);
   A((void**)&g->iters, sizeof(int) * g->ncells);
   // Periodic: every cell owns its +x, +y, +z face.
@@ -467,7 +477,7 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   e = hipMemcpy(g->faces, faces.data(), sizeof(FaceDesc) * g->nfaces, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->st, &st0, sizeof(st0), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->d_dx, g->dx, sizeof(double) * 3, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(g->lduh, 0, sizeof(double) * g->ncells * z.B3 * z.V);
+  if (e == hipSuccess) e = hipMemset(g->lduh, 0, g->bulk * g->ncells * z.B3 * z.V);
   if (e == hipSuccess) e = hipMemset(g->iters, 0, sizeof(int) * g->ncells);
   if (e != hipSuccess) {
     free_grid(g);

@@ -10,7 +10,8 @@ namespace exa {
 // ---------------------------------------------------------------------------
 // Tables.  The reference's printed literals (DGMatrices.cpp, Gauss*Quadrature.cpp),
 // [order][i] / [order][row][col], zero padded to 10.  Indexed with compile-time
-// indices after unrolling, they become scalar constant loads.
+// indices after unrolling, they become scalar constants (converted to the
+// kernel's arithmetic type at compile time).
 // ---------------------------------------------------------------------------
 #define EXA_TABLE static __constant__
 #include "dg_tables.inc"
@@ -31,25 +32,27 @@ __host__ __device__ constexpr int node_threads(int B) { return round_up(B * B * 
 // Reductions over a 64-wide wavefront (xor butterfly: every lane ends with the
 // identical value because each level adds the same two operands).
 // ---------------------------------------------------------------------------
-__device__ inline double wave_sum(double v) {
+template <typename T>
+__device__ inline T wave_sum(T v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
   return v;
 }
 
-__device__ inline double wave_min(double v) {
+template <typename T>
+__device__ inline T wave_min(T v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_xor(v, off, kWave);
+    const T o = __shfl_xor(v, off, kWave);
     v = o < v ? o : v;
   }
   return v;
 }
 
-// Workgroup sum; `red` holds NT/64 doubles of LDS.  All threads return the
+// Workgroup sum; `red` holds NT/64 values of LDS.  All threads return the
 // same value (partials are summed in wave order by every thread).
-template <int NT>
-__device__ inline double block_sum(double v, double* red) {
+template <int NT, typename T>
+__device__ inline T block_sum(T v, T* red) {
   constexpr int NW = NT / kWave;
   v = wave_sum(v);
   if (NW == 1) return v;
@@ -58,14 +61,14 @@ __device__ inline double block_sum(double v, double* red) {
   __syncthreads();
   if (lane == 0) red[wid] = v;
   __syncthreads();
-  double s = red[0];
+  T s = red[0];
 #pragma unroll
   for (int w = 1; w < NW; ++w) s += red[w];
   return s;
 }
 
-template <int NT>
-__device__ inline double block_min(double v, double* red) {
+template <int NT, typename T>
+__device__ inline T block_min(T v, T* red) {
   constexpr int NW = NT / kWave;
   v = wave_min(v);
   if (NW == 1) return v;
@@ -74,7 +77,7 @@ __device__ inline double block_min(double v, double* red) {
   __syncthreads();
   if (lane == 0) red[wid] = v;
   __syncthreads();
-  double s = red[0];
+  T s = red[0];
 #pragma unroll
   for (int w = 1; w < NW; ++w) s = red[w] < s ? red[w] : s;
   return s;
@@ -82,7 +85,8 @@ __device__ inline double block_min(double v, double* red) {
 
 // ---------------------------------------------------------------------------
 // PDE functors: the user solver interface (UserSolverInterface.h:64-127)
-// compiled for the device.  Q holds V variables followed by P parameters.
+// compiled for the device, for arithmetic type T.  Q holds V variables
+// followed by P parameters.
 // ---------------------------------------------------------------------------
 
 // Compressible Euler, gamma = 1.4; the expression trees of the reference's
@@ -93,10 +97,11 @@ struct Euler {
   static constexpr int V = 5;
   static constexpr int P = NPAR;
 
-  __device__ static inline void flux(const double* Q, double (*F)[V]) {
-    const double GAMMA = 1.4;
-    const double irho = 1.0 / Q[0];
-    const double p = (GAMMA - 1) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+  template <typename T>
+  __device__ static inline void flux(const T* Q, T (*F)[V]) {
+    const T GAMMA = T(1.4);
+    const T irho = T(1.0) / Q[0];
+    const T p = (GAMMA - T(1)) * (Q[4] - T(0.5) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
     F[0][0] = Q[1];
     F[0][1] = irho * Q[1] * Q[1] + p;
     F[0][2] = irho * Q[1] * Q[2];
@@ -115,29 +120,32 @@ struct Euler {
   }
 
   // Largest |eigenvalue| in direction d (the eigenvalues are u_n-c, u_n x3, u_n+c).
-  __device__ static inline double max_abs_eigenvalue(const double* Q, int d) {
-    const double GAMMA = 1.4;
-    const double irho = 1.0 / Q[0];
-    const double p = (GAMMA - 1) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
-    const double u_n = Q[d + 1] * irho;
-    const double c = sqrt(GAMMA * p * irho);
-    const double l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
+  template <typename T>
+  __device__ static inline T max_abs_eigenvalue(const T* Q, int d) {
+    const T GAMMA = T(1.4);
+    const T irho = T(1.0) / Q[0];
+    const T p = (GAMMA - T(1)) * (Q[4] - T(0.5) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+    const T u_n = Q[d + 1] * irho;
+    const T c = sqrt(GAMMA * p * irho);
+    const T l0 = fabs(u_n - c), l1 = fabs(u_n), l4 = fabs(u_n + c);
     // std::max_element order: first maximum over (l0, l1, l1, l1, l4)
-    double m = l0;
+    T m = l0;
     m = (m < l1) ? l1 : m;
     m = (m < l4) ? l4 : m;
     return m;
   }
 
-  __device__ static inline void algebraic_source(const double*, double* S) {
+  template <typename T>
+  __device__ static inline void algebraic_source(const T*, T* S) {
 #pragma unroll
-    for (int m = 0; m < V; ++m) S[m] = 0.0;
+    for (int m = 0; m < V; ++m) S[m] = T(0);
   }
 
   // The test solver's NCP in the state its predictor test uses (zero).
-  __device__ static inline void ncp(const double*, const double (*)[V], double* BgradQ) {
+  template <typename T>
+  __device__ static inline void ncp(const T*, const T (*)[V], T* BgradQ) {
 #pragma unroll
-    for (int m = 0; m < V; ++m) BgradQ[m] = 0.0;
+    for (int m = 0; m < V; ++m) BgradQ[m] = T(0);
   }
 };
 

@@ -41,6 +41,15 @@ typedef enum exa_pde { EXA_PDE_EULER = 0, EXA_PDE_ELASTIC = 1 } exa_pde;
 /* The spec file's `terms` (useFlux / useSource / useNCP template flags). */
 enum { EXA_TERM_FLUX = 1, EXA_TERM_SOURCE = 2, EXA_TERM_NCP = 4 };
 
+/* Arithmetic of the bulk work.  EXA_PRECISION_FP64: the reference's double
+ * everywhere.  EXA_PRECISION_FP32_BULK: space-time predictor, volume integral
+ * and Riemann solver in float on float face / update arrays; surface integral,
+ * solution update and the time step size in double on the double solution.
+ * The float Picard loop stops on a relative residual (RMS change < 1e-6 RMS
+ * of the iterate) because the reference's absolute 1e-14 lies below float
+ * rounding.  Grid entry points only; the single-cell ones are FP64. */
+enum { EXA_PRECISION_FP64 = 0, EXA_PRECISION_FP32_BULK = 1 };
+
 /* SolverType's compile-time constants (AbstractGenericADERDGSolverHeader.template:32-35). */
 typedef struct exa_solver {
   int pde;               /* exa_pde */
@@ -50,6 +59,7 @@ typedef struct exa_solver {
   int terms;             /* EXA_TERM_* */
   int no_time_averaging; /* template flag noTimeAveraging */
   double cfl;            /* SolverType::CFL */
+  int precision;         /* EXA_PRECISION_* (0 = FP64) */
 } exa_solver;
 
 const char* exa_status_string(int status);

@@ -1,0 +1,142 @@
+// exahype_amd/kernels.hpp -- the kernels::aderdg::generic::c templates of the
+// reference (ExaHyPE/kernels/aderdg/generic/Kernels.h:85-213) re-declared on
+// top of the C-ABI, so that generated solver glue
+// (Toolkit/src/eu/exahype/solvers/templates/AbstractGenericADERDGSolverImplementation.template:46-161)
+// compiles unchanged against the HIP implementation.
+//
+// Differences a caller must know:
+//  * PDE callbacks cannot cross into device code, so SolverType names one of
+//    the device PDEs compiled into the library:
+//        static constexpr int ExaDevicePde = EXA_PDE_EULER;
+//    (NumberOfVariables, NumberOfParameters, Order, CFL as in the reference).
+//  * dx may be any type with operator[] (tarch::la::Vector<3,double> works).
+//  * Calls are synchronous and take host arrays; errors throw exahype_amd::Error.
+//  * For whole grid levels use exa_grid_* instead: these single-cell calls pay
+//    a host<->device copy each.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "../exahype_amd.h"
+
+namespace exahype_amd {
+
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+inline void check(int status, const char* what) {
+  if (status != EXA_OK) throw Error(std::string(what) + ": " + exa_status_string(status));
+}
+
+template <typename SolverType>
+inline exa_solver descriptor(int terms, bool noTimeAveraging) {
+  exa_solver s;
+  s.pde = SolverType::ExaDevicePde;
+  s.order = SolverType::Order;
+  s.num_variables = SolverType::NumberOfVariables;
+  s.num_parameters = SolverType::NumberOfParameters;
+  s.terms = terms;
+  s.no_time_averaging = noTimeAveraging ? 1 : 0;
+  s.cfl = SolverType::CFL;
+  s.precision = EXA_PRECISION_FP64;
+  return s;
+}
+
+// Kernels that do not depend on the PDE (integrals) only need sizes.
+inline exa_solver size_descriptor(int numberOfVariables, int basisSize, int terms, bool noTimeAveraging) {
+  exa_solver s;
+  s.pde = EXA_PDE_EULER;
+  s.order = basisSize - 1;
+  s.num_variables = numberOfVariables;
+  s.num_parameters = 0;
+  s.terms = terms;
+  s.no_time_averaging = noTimeAveraging ? 1 : 0;
+  s.cfl = 0.9;
+  s.precision = EXA_PRECISION_FP64;
+  return s;
+}
+
+inline int terms(bool useSource, bool useFlux, bool useNCP) {
+  return (useFlux ? EXA_TERM_FLUX : 0) | (useSource ? EXA_TERM_SOURCE : 0) | (useNCP ? EXA_TERM_NCP : 0);
+}
+
+template <typename DxVector>
+inline void dx3(const DxVector& dx, double out[3]) {
+  out[0] = dx[0];
+  out[1] = dx[1];
+  out[2] = dx[2];
+}
+
+}  // namespace exahype_amd
+
+namespace kernels {
+namespace aderdg {
+namespace generic {
+namespace c {
+
+template <bool useSource, bool useFlux, bool useNCP, bool noTimeAveraging, typename SolverType, typename DxVector>
+void spaceTimePredictorNonlinear(SolverType& /*solver*/, double* lQhbnd, double* lFhbnd,
+                                 double** tempSpaceTimeUnknowns, double** tempSpaceTimeFluxUnknowns,
+                                 double* tempUnknowns, double* tempFluxUnknowns, const double* const luh,
+                                 const DxVector& dx, const double dt) {
+  const exa_solver s =
+      exahype_amd::descriptor<SolverType>(exahype_amd::terms(useSource, useFlux, useNCP), noTimeAveraging);
+  double d[3];
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(
+      exa_space_time_predictor_nonlinear(&s, lQhbnd, lFhbnd,
+                                         tempSpaceTimeUnknowns ? tempSpaceTimeUnknowns[0] : nullptr,
+                                         tempSpaceTimeFluxUnknowns ? tempSpaceTimeFluxUnknowns[0] : nullptr,
+                                         tempUnknowns, tempFluxUnknowns, luh, d, dt, nullptr),
+      "spaceTimePredictorNonlinear");
+}
+
+template <bool useSourceOrNCP, bool useFlux, bool noTimeAveraging, int numberOfVariables, int basisSize,
+          typename DxVector>
+void volumeIntegralNonlinear(double* lduh, const double* const /*lFi*/, const double* const lFhi,
+                             const DxVector& dx) {
+  const exa_solver s = exahype_amd::size_descriptor(
+      numberOfVariables, basisSize, (useFlux ? EXA_TERM_FLUX : 0) | (useSourceOrNCP ? EXA_TERM_SOURCE : 0),
+      noTimeAveraging);
+  double d[3];
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(exa_volume_integral_nonlinear(&s, lduh, lFhi, d), "volumeIntegralNonlinear");
+}
+
+template <int numberOfVariables, int basisSize, typename DxVector>
+void surfaceIntegralNonlinear(double* lduh, const double* const lFbnd, const DxVector& dx) {
+  const exa_solver s = exahype_amd::size_descriptor(numberOfVariables, basisSize, EXA_TERM_FLUX, false);
+  double d[3];
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(exa_surface_integral_nonlinear(&s, lduh, lFbnd, d), "surfaceIntegralNonlinear");
+}
+
+template <bool useNCP, typename SolverType>
+void riemannSolverNonlinear(SolverType& /*solver*/, double* FL, double* FR, const double* const QL,
+                            const double* const QR, const double dt, const int normalNonZero) {
+  const exa_solver s = exahype_amd::descriptor<SolverType>(EXA_TERM_FLUX | (useNCP ? EXA_TERM_NCP : 0), false);
+  exahype_amd::check(exa_riemann_solver_nonlinear(&s, FL, FR, QL, QR, dt, normalNonZero),
+                     "riemannSolverNonlinear");
+}
+
+template <typename SolverType>
+void solutionUpdate(SolverType& /*solver*/, double* luh, const double* const lduh, const double dt) {
+  const exa_solver s = exahype_amd::descriptor<SolverType>(EXA_TERM_FLUX, false);
+  exahype_amd::check(exa_solution_update(&s, luh, lduh, dt), "solutionUpdate");
+}
+
+template <typename SolverType, typename DxVector>
+double stableTimeStepSize(SolverType& /*solver*/, const double* const luh, const DxVector& dx) {
+  const exa_solver s = exahype_amd::descriptor<SolverType>(EXA_TERM_FLUX, false);
+  double d[3], dt = 0.0;
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(exa_stable_time_step_size(&s, luh, d, &dt), "stableTimeStepSize");
+  return dt;
+}
+
+}  // namespace c
+}  // namespace generic
+}  // namespace aderdg
+}  // namespace kernels

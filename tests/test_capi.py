@@ -58,3 +58,52 @@ def test_invalid_arguments_rejected_before_any_device_work(exa):
                                                 None, 0.1, None) == 3
     with pytest.raises(exa.ExaError):
         exa.check(3, "probe")
+
+
+def test_fp32_bulk_is_grid_only(exa):
+    """The single-cell drop-ins are FP64 (the reference's arithmetic); the FP32-bulk
+    form is rejected before any device work."""
+    import numpy as np
+    s = exa.euler_solver(3, precision=exa.PRECISION_FP32_BULK)
+    assert exa.supported(s)
+    with pytest.raises(exa.ExaError):
+        exa.space_time_predictor_nonlinear(s, np.ones(64 * 5), 0.1, 1e-3)
+
+
+GLUE = r'''
+#include <exahype_amd/kernels.hpp>
+#include <vector>
+struct Vec3 { double v[3]; double operator[](int i) const { return v[i]; } };
+struct MyEulerSolver {  // the constants a generated Abstract<Solver> carries
+  static constexpr int NumberOfVariables = 5;
+  static constexpr int NumberOfParameters = 0;
+  static constexpr int Order = 3;
+  static constexpr double CFL = 0.9;
+  static constexpr int ExaDevicePde = EXA_PDE_EULER;
+};
+// The calls AbstractGenericADERDGSolverImplementation.template makes.
+void glue(MyEulerSolver& solver, double* luh, double* lduh, double* lQhbnd, double* lFhbnd,
+          double** stu, double** stfu, double* tu, double* tfu, double* FL, double* FR) {
+  const Vec3 dx{{0.1, 0.1, 0.1}};
+  kernels::aderdg::generic::c::spaceTimePredictorNonlinear<false, true, false, false, MyEulerSolver>(
+      solver, lQhbnd, lFhbnd, stu, stfu, tu, tfu, luh, dx, 1e-3);
+  kernels::aderdg::generic::c::volumeIntegralNonlinear<false, true, false, 5, 4>(lduh, stfu[0], tfu, dx);
+  kernels::aderdg::generic::c::riemannSolverNonlinear<false, MyEulerSolver>(solver, FL, FR, lQhbnd, lQhbnd, 1e-3, 0);
+  kernels::aderdg::generic::c::surfaceIntegralNonlinear<5, 4>(lduh, lFhbnd, dx);
+  kernels::aderdg::generic::c::solutionUpdate<MyEulerSolver>(solver, luh, lduh, 1e-3);
+  (void)kernels::aderdg::generic::c::stableTimeStepSize<MyEulerSolver>(solver, luh, dx);
+}
+int main(int argc, char**) { if (argc > 99) glue(*(MyEulerSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0); return 0; }
+'''
+
+
+def test_cpp_dropin_header_compiles_and_links(exa, tmp_path):
+    """Generated-glue-style calls compile against include/exahype_amd/kernels.hpp and link
+    against the library (not executed: no GPU needed)."""
+    src = tmp_path / "glue.cpp"
+    src.write_text(GLUE)
+    libdir = os.path.dirname(exa.LIB_PATH)
+    res = subprocess.run(["g++", "-std=c++14", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                          str(tmp_path / "glue"), "-L", libdir, "-lexahype_amd", "-Wl,-rpath," + libdir],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr

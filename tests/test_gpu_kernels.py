@@ -80,12 +80,14 @@ def test_batch_predictor_volume_vs_oracle(exa, oracle, order):
     n = 24
     luh = np.concatenate([smooth_euler_cells(rng, n, order), random_euler_cells(rng, n, order)])
     dx = np.tile([0.05, 0.04, 0.06], (2 * n, 1))
-    dt = 2e-3
-    print(f"case: batch predictor N={order}", flush=True)
-    r = exa.batch_predictor_volume(exa.euler_solver(order), luh, dx, dt)
     s = oracle.make_solver(order, 5, 0)
     B = order + 1
     nb3, nbf = B ** 3 * 5, 6 * B * B * 5
+    # the time step the reference would take for these cells (CFL 0.9): beyond it the
+    # Picard iteration of rough data diverges and only rounding noise is compared
+    dt = min(oracle.stable_dt(s, luh[c * nb3:(c + 1) * nb3], dx[c]) for c in range(2 * n))
+    print(f"case: batch predictor N={order} dt={dt:.3e}", flush=True)
+    r = exa.batch_predictor_volume(exa.euler_solver(order), luh, dx, dt)
     for c in range(2 * n):
         o = oracle.space_time_predictor(s, luh[c * nb3:(c + 1) * nb3], dx[c], dt)
         lduh = oracle.volume_integral(s, o["lFhi"], dx[c])

@@ -1,5 +1,6 @@
 """Build the HIP library in-tree: exahype_amd/libexahype_amd.so (gfx950)."""
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -23,16 +24,49 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+RESOURCES = os.path.join(HERE, "kernel_resources.txt")
+
+
+def parse_resource_remarks(text):
+    """Per-kernel resource usage from hipcc's -Rpass-analysis=kernel-resource-usage remarks."""
+    kernels = {}
+    name = None
+    for line in text.splitlines():
+        m = re.search(r"remark: Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            kernels[name] = {}
+            continue
+        m = re.search(r"remark:\s+([A-Za-z ]+?)(?: \[[^\]]*\])?: (\d+)", line)
+        if m and name:
+            kernels[name][m.group(1).strip()] = int(m.group(2))
+    return kernels
+
+
 def build(verbose=False):
+    """Compile for gfx950.  Fails if any kernel spills to scratch memory: a spill
+    stored inside a divergent region saves only the active lanes, so kernels
+    must stay within their register budget."""
     srcs = [os.path.join(HERE, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libexahype_amd.so")
+    kernels = parse_resource_remarks(res.stderr)
+    with open(RESOURCES, "w") as f:
+        for k in sorted(kernels):
+            r = kernels[k]
+            f.write(f"{k} vgpr={r.get('VGPRs')} agpr={r.get('AGPRs')} scratch={r.get('ScratchSize')} "
+                    f"occupancy={r.get('Occupancy')} lds={r.get('LDS Size')}\n")
+    spilling = sorted(k for k, r in kernels.items() if r.get("ScratchSize", 0) > 0)
+    if not kernels:
+        raise RuntimeError("no kernel resource remarks parsed from hipcc output")
+    if spilling:
+        raise RuntimeError("kernels spill to scratch memory:\n  " + "\n  ".join(spilling))
     os.replace(tmp, LIB)
     return LIB
 

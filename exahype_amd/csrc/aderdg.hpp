@@ -68,6 +68,12 @@ struct PredictorArgs {
   double* lFi;   // (t,z,y,x,{x,y,z,src},var) 4 B^4 V
   double* lQhi;  // (z,y,x,var)
   double* lFhi;  // 4 blocks of B^3 V
+  // Device workspace for the time slices of the Picard iterate that fit neither
+  // LDS nor registers (large orders): ws_cells cells of predictor_ws_bytes_per_cell
+  // each; launches are split into chunks of ws_cells cells.  Unused (null, 0)
+  // where predictor_ws_bytes_per_cell is 0.
+  void* ws;
+  long ws_cells;
 };
 
 struct RiemannArgs {
@@ -120,6 +126,10 @@ bool supported(const SolverDesc& sd);
 bool supported_sizes(const SolverDesc& sd);
 hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a,
                             long ncells, hipStream_t s);
+// Bytes of PredictorArgs::ws one cell needs (0: no workspace).
+size_t predictor_ws_bytes_per_cell(const SolverDesc& sd);
+// Cells per workspace chunk the host allocates for (bounds the workspace size).
+constexpr long kPredictorWsCells = 4096;
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s);
 // update_kernel modes: surface integral, solution update, write lduh back.
 enum : int { kUpdSurface = 1, kUpdUpdate = 2, kUpdWriteLduh = 4 };

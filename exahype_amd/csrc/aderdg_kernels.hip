@@ -69,6 +69,36 @@ __device__ inline void source_ncp(const T* Q, const T* sQ, const T* sDu, const T
   }
 }
 
+// Where the predictor keeps the Picard iterate lQi^k of a cell (B^4 V values):
+// its first NQL time slices live in LDS (as many as keep the workgroup's LDS
+// within 144 KB next to one slice of fluxes sF), the other NQR slices in
+// registers or, when registers run out (WSQ), in a per-cell device workspace
+// that each thread reads and writes only at its own nodes.  Large sizes run
+// the loop over time slices rolled (ROLLED).
+template <int B, int V, int D, bool USE_NCP, typename T>
+struct PredictorLayout {
+  static constexpr int B2 = B * B, B3 = B2 * B;
+  static constexpr int NT = node_threads(B);
+  static constexpr bool IS_FP64 = sizeof(T) == sizeof(double);
+  // LDS: one slice of fluxes [dir][var][node] (reused for lQhi / lFhi staging)
+  static constexpr int SF = (3 * V > D ? 3 * V : D) * B3;
+  static constexpr long kLdsBudget = 144 * 1024;
+  static constexpr long kSliceBytes = (long)V * B3 * sizeof(T);
+  static constexpr long kOther =
+      (long)(SF + (USE_NCP ? V * B3 : 0) + 2 * B2 + B + 3 + 2 * (NT / kWave)) * sizeof(T);
+  static constexpr long kFree = kLdsBudget - kOther;
+  static constexpr int NQL = kFree <= 0 ? 0 : (kFree / kSliceBytes >= B ? B : (int)(kFree / kSliceBytes));
+  static constexpr int NQR = B - NQL;
+  // Rolled slices: the large sizes, whose unrolled slices exhaust the registers.
+  static constexpr bool ROLLED = B >= 7 && (IS_FP64 || B >= 8);
+  // 8+ waves per workgroup leave 256 registers per lane, no room for slices;
+  // a rolled loop cannot index registers.
+  static constexpr bool WSQ = NQR > 0 && (NT >= 8 * kWave || ROLLED);
+  // unroll factor of the loop over time slices
+  static constexpr int UNROLL_T = ROLLED ? 1 : B;
+  static constexpr size_t ws_bytes = WSQ ? (size_t)NQR * V * B3 * sizeof(T) : 0;
+};
+
 // ---------------------------------------------------------------------------
 // Space-time predictor + time averaging + extrapolation + volume integral.
 //
@@ -76,10 +106,11 @@ __device__ inline void source_ncp(const T* Q, const T* sQ, const T* sDu, const T
 //  1. node owners evaluate the fluxes of their node and stage them in LDS as
 //     sF[dir][var][node];
 //  2. line owners (one task = one direction, variable and grid line) contract
-//     the line with Kxi -- every staged value is read once, Kxi comes in as
-//     scalar constants -- and stage the directional contributions in sR;
-//  3. node owners form rhs(t) and fold it into the next iterate through iK1
-//     (the reference's n order), so rhs is never stored for all t.
+//     the line with Kxi in place -- every staged value is read once, Kxi comes
+//     in as scalar constants;
+//  3. node owners form rhs(t) from the directional contributions and fold it
+//     into the next iterate through iK1 (the reference's n order), so rhs is
+//     never stored for all t.
 // ---------------------------------------------------------------------------
 template <int B, class PDE, int TERMS, bool EXPORT, typename T>
 __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArgs a) {
@@ -93,10 +124,11 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   constexpr bool USE_SRC_NCP = USE_SOURCE || USE_NCP;
   constexpr bool IS_FP64 = sizeof(T) == sizeof(double);
   constexpr int MAX_ITER = 2 * B;  // spaceTimePredictorNonlinear.cpph:85
-  // LDS: one slice of fluxes [dir][var][node] (reused for lQhi / lFhi staging)
-  constexpr int SF = (3 * V > D ? 3 * V : D) * B3;
+  using L = PredictorLayout<B, V, D, USE_NCP, T>;
+  constexpr int SF = L::SF, NQL = L::NQL, NQR = L::NQR, UNROLL_T = L::UNROLL_T;
+  constexpr bool WSQ = L::WSQ;
   __shared__ T sF[SF];
-  __shared__ T sR[USE_FLUX ? 3 * V * B3 : 1];  // directional contributions [dir][var][node]
+  __shared__ T sQi[NQL > 0 ? NQL * V * B3 : 1];  // lQi^k slices t < NQL, [t][var][node]
   __shared__ T sQ[USE_NCP ? V * B3 : 1];
   __shared__ T sK[B2];   // Kxi[n][l]
   __shared__ T sDu[B2];  // dudx[l][n]
@@ -108,6 +140,10 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   T* const out_lQhbnd = static_cast<T*>(a.lQhbnd);
   T* const out_lFhbnd = static_cast<T*>(a.lFhbnd);
 
+  // Control flow is uniform across the workgroup: lanes beyond the B^3 nodes
+  // compute node 0 again and write the same values, and loops over tasks or
+  // outputs run whole rounds with the index clamped (surplus lanes redo the last
+  // item).  Divergent branches next to register spills are what this avoids.
   const size_t cell = (size_t)blockIdx.x + a.cell0;
   const int tid = threadIdx.x;
   const bool active = tid < B3;
@@ -123,13 +159,11 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   const double* dxp = a.dx + cell * a.dx_stride;
   const T invDx[3] = {T(1.0) / T(dxp[0]), T(1.0) / T(dxp[1]), T(1.0) / T(dxp[2])};
   const T dt = T(*a.dt);
-  // static indices only: a lane-indexed private array is placed in scratch memory,
-  // where only the lanes that index it would see it initialised
-  if (tid == 0) {
-    sInvDx[0] = invDx[0];
-    sInvDx[1] = invDx[1];
-    sInvDx[2] = invDx[2];
-  }
+  // Every lane writes the same values: no divergent branch here (and no
+  // lane-indexed private array, which would live in scratch memory).
+  sInvDx[0] = invDx[0];
+  sInvDx[1] = invDx[1];
+  sInvDx[2] = invDx[2];
 
   T q0[D];
   {
@@ -140,14 +174,26 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   const T wx = tc<T>(exa_gl_weights[N][x]), wy = tc<T>(exa_gl_weights[N][y]), wz = tc<T>(exa_gl_weights[N][z]);
   const T w3 = wz * wy * wx;  // weight of rhs0 (:130-132) and of iK1 (:286-288)
 
-  T qi[B][V];  // lQi(z,y,x,t,var) of this node, variables only
+  // Picard state: the current iterate lQi^k of this node (in LDS, each thread
+  // touching only its own node's entries so no barriers are needed, or in
+  // registers); the next iterate qn = lQi^{k+1} is accumulated in registers
+  // slice by slice (iK1 applied in the reference's n order).
+  T qi_reg[(NQR > 0 && !WSQ) ? NQR : 1][V];
+  T* const ws = WSQ ? static_cast<T*>(a.ws) + (size_t)blockIdx.x * (NQR * V * B3) : nullptr;
+  // register slices are only used with compile-time t (unrolled loops)
+  auto qi = [&](int t, int m) -> T& {
+    if (t < NQL)
+      return sQi[(t * V + m) * B3 + node];
+    else if (WSQ)
+      return ws[((t - NQL) * V + m) * B3 + node];
+    else
+      return qi_reg[t - NQL][m];
+  };
 #pragma unroll
   for (int t = 0; t < B; ++t)
 #pragma unroll
-    for (int m = 0; m < V; ++m) qi[t][m] = q0[m];
-
-  // Picard state: qi = current iterate lQi^k, qn = next iterate lQi^{k+1}
-  // accumulated slice by slice (iK1 applied in the reference's n order).
+    for (int m = 0; m < V; ++m)
+      qi(t, m) = q0[m];
   T qn[B][V];
   int iters = MAX_ITER;
   const T iweight = T(1.0) / w3;
@@ -163,19 +209,21 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     for (int l = 0; l < B; ++l)
 #pragma unroll
       for (int m = 0; m < V; ++m) qn[l][m] = T(0);
-#pragma unroll
+    // Rolled (large sizes): table entries come from constant memory with a
+    // uniform index; unrolled: they are immediates.
+#pragma unroll UNROLL_T
     for (int t = 0; t < B; ++t) {
       const T wt = tc<T>(exa_gl_weights[N][t]);
       T Q[D];
 #pragma unroll
-      for (int m = 0; m < V; ++m) Q[m] = qi[t][m];
+      for (int m = 0; m < V; ++m) Q[m] = qi(t, m);
 #pragma unroll
       for (int m = V; m < D; ++m) Q[m] = q0[m];
       // phase 1 (node owners): fluxes of this time slice -> sF[dir][var][node]
       if (USE_FLUX) {
         T F[3][V];
         PDE::flux(Q, F);
-        if (active) {
+        {
 #pragma unroll
           for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -189,17 +237,23 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
           }
         }
       }
-      if (USE_NCP && active) {
+      if (USE_NCP) {
 #pragma unroll
-        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = qi[t][m];
+        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = Q[m];
       }
       __syncthreads();
 
       // phase 2 (line owners): one task = one (direction, variable, line);
       // out(l) = sum_n (us * F(n)) * Kxi[n][l] along the line, with
       // us = w_t w_a w_b dt / dx_dir (:152-236).  Kxi comes in as constants.
+      // A task reads its whole line before writing the results back to the same
+      // positions, and no other task touches them, so the contraction is in place.
       if (USE_FLUX) {
-        for (int task = tid; task < 3 * B2 * V; task += NT) {
+        constexpr int NTASK = 3 * B2 * V;
+#pragma unroll 1
+        for (int round = 0; round < (NTASK + NT - 1) / NT; ++round) {
+          // surplus lanes of the last round redo the last task (identical writes)
+          const int task = min(tid + round * NT, NTASK - 1);
           const int d = task / (B2 * V);
           const int rem = task - d * (B2 * V);
           const int m = rem / B2;
@@ -214,7 +268,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
           T f[B];
 #pragma unroll
           for (int n = 0; n < B; ++n) f[n] = us * src[n * stride];
-          T* dst = sR + (d * V + m) * B3 + base;
+          T* dst = sF + (d * V + m) * B3 + base;
 #pragma unroll
           for (int l = 0; l < B; ++l) {
             T acc = T(0);
@@ -234,9 +288,9 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       for (int m = 0; m < V; ++m) {
         r[m] = w3i * fl_t * q0[m];
         if (USE_FLUX) {
-          r[m] -= sR[(0 * V + m) * B3 + node];
-          r[m] -= sR[(1 * V + m) * B3 + node];
-          r[m] -= sR[(2 * V + m) * B3 + node];
+          r[m] -= sF[(0 * V + m) * B3 + node];
+          r[m] -= sF[(1 * V + m) * B3 + node];
+          r[m] -= sF[(2 * V + m) * B3 + node];
         }
       }
       if (USE_SRC_NCP) {
@@ -245,12 +299,11 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
         const T us = wt * wz * wy * wx * dt;
 #pragma unroll
         for (int m = 0; m < V; ++m) r[m] += us * S[m];
-        if (EXPORT && active) {
+        if (EXPORT) {
           double* lFi = a.lFi + cell * (size_t)(4 * B3 * B * V) + (((size_t)t * B3 + node) * 4 + 3) * V;
 #pragma unroll
           for (int m = 0; m < V; ++m) lFi[m] = S[m];
         }
-        if (USE_NCP && !USE_FLUX) __syncthreads();  // sQ is rewritten by the next slice
       }
 #pragma unroll
       for (int m = 0; m < V; ++m) {
@@ -258,6 +311,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 #pragma unroll
         for (int l = 0; l < B; ++l) qn[l][m] += ir * tc<T>(exa_ik1[N][l][t]);
       }
+      if (USE_FLUX || USE_NCP) __syncthreads();  // the next slice rewrites sF / sQ
     }
 
     // squared residual (:303) and exit test (:314-318)
@@ -266,7 +320,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     for (int l = 0; l < B; ++l)
 #pragma unroll
       for (int m = 0; m < V; ++m) {
-        const T dq = qn[l][m] - qi[l][m];
+        const T dq = qn[l][m] - qi(l, m);
         sq += dq * dq;
         if (!IS_FP64) qq += qn[l][m] * qn[l][m];
       }
@@ -290,7 +344,8 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 #pragma unroll
     for (int l = 0; l < B; ++l)
 #pragma unroll
-      for (int m = 0; m < V; ++m) qi[l][m] = qn[l][m];
+      for (int m = 0; m < V; ++m)
+        qi(l, m) = qn[l][m];
   }
 
   // ---- epilogue: time average, extrapolation, volume integral ----
@@ -306,7 +361,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       for (int t = 0; t < B; ++t) acc += (m < V ? qn[t][m < V ? m : 0] : q0[m]) * tc<T>(exa_gl_weights[N][t]);
       qh[m] = acc;
     }
-    if (EXPORT && active) {
+    if (EXPORT) {
       double* lQi = a.lQi + cell * (size_t)(B3 * B * D) + (size_t)node * B * D;
 #pragma unroll
       for (int t = 0; t < B; ++t)
@@ -316,14 +371,14 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 #pragma unroll
       for (int m = 0; m < D; ++m) lQhi[m] = qh[m];
     }
-    if (active) {
 #pragma unroll
-      for (int m = 0; m < D; ++m) sF[m * B3 + node] = qh[m];
-    }
+    for (int m = 0; m < D; ++m) sF[m * B3 + node] = qh[m];
   }
   __syncthreads();
   // lQhbnd (:434-527): face f, face node (a,b), datum k; reduce the normal line.
-  for (int o = tid; o < 6 * B2 * D; o += NT) {
+#pragma unroll 1
+  for (int round = 0; round < (6 * B2 * D + NT - 1) / NT; ++round) {
+    const int o = min(tid + round * NT, 6 * B2 * D - 1);
     const int f = o / (B2 * D);
     const int rem = o - f * (B2 * D);
     const int fa = rem / (B * D);
@@ -357,7 +412,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   for (int t = 0; t < B; ++t) {
     T Q[D];
 #pragma unroll
-    for (int m = 0; m < V; ++m) Q[m] = qi[t][m];
+    for (int m = 0; m < V; ++m) Q[m] = qi(t, m);
 #pragma unroll
     for (int m = V; m < D; ++m) Q[m] = q0[m];
     const T wt = tc<T>(exa_gl_weights[N][t]);
@@ -370,9 +425,9 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
         for (int m = 0; m < V; ++m) fh[d][m] += F[d][m] * wt;
     }
     if (USE_SRC_NCP) {
-      if (USE_NCP && active) {
+      if (USE_NCP) {
 #pragma unroll
-        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = qi[t][m];
+        for (int m = 0; m < V; ++m) sQ[m * B3 + node] = Q[m];
       }
       if (USE_NCP) __syncthreads();
       T S[V];
@@ -382,7 +437,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       if (USE_NCP) __syncthreads();
     }
   }
-  if (EXPORT && active) {
+  if (EXPORT) {
     // lFhi blocks: x (z,y,x), y (z,x,y), z (y,x,z), src (z,y,x)  (:375-393)
     double* lFhi = a.lFhi + cell * (size_t)(4 * B3 * V);
 #pragma unroll
@@ -393,16 +448,16 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       lFhi[3 * B3 * V + ((z * B + y) * B + x) * V + m] = sh[m];
     }
   }
-  if (active) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+  for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int m = 0; m < V; ++m) sF[(d * V + m) * B3 + node] = fh[d][m];
-  }
+    for (int m = 0; m < V; ++m) sF[(d * V + m) * B3 + node] = fh[d][m];
   __syncthreads();
   // lFhbnd from lFhi_{x,y,z} (:434-527)
   if (USE_FLUX) {
-    for (int o = tid; o < 6 * B2 * V; o += NT) {
+#pragma unroll 1
+    for (int round = 0; round < (6 * B2 * V + NT - 1) / NT; ++round) {
+      const int o = min(tid + round * NT, 6 * B2 * V - 1);
       const int f = o / (B2 * V);
       const int rem = o - f * (B2 * V);
       const int fa = rem / (B * V);
@@ -418,10 +473,12 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       out_lFhbnd[cFb + o] = acc;
     }
   } else {
-    for (int o = tid; o < 6 * B2 * V; o += NT) out_lFhbnd[cFb + o] = T(0);
+#pragma unroll 1
+    for (int round = 0; round < (6 * B2 * V + NT - 1) / NT; ++round)
+      out_lFhbnd[cFb + min(tid + round * NT, 6 * B2 * V - 1)] = T(0);
   }
   // volume integral (volumeIntegralNonlinear.cpph:136-225), time-averaged form
-  if (out_lduh && active) {
+  if (out_lduh) {
     T acc[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) acc[m] = T(0);
@@ -748,26 +805,52 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
                             hipStream_t s) {
   if (ncells <= 0) return hipSuccess;
   if (export_temps && sd.precision != kFP64) return hipErrorNotSupported;
+  const size_t ws = predictor_ws_bytes_per_cell(sd);
+  if (ws > 0 && (a.ws == nullptr || a.ws_cells <= 0)) return hipErrorInvalidValue;
+  const long chunk = ws > 0 ? a.ws_cells : ncells;
   return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
-    const dim3 grid((unsigned)ncells), block(node_threads(B));
-    if (sd.precision == kFP32Bulk) {
-      if (sd.terms != kFlux) return hipErrorNotSupported;
-      hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float>), grid, block, 0, s, a);
-      return hipGetLastError();
+    // chunks of at most ws_cells cells share the workspace (stream order)
+    for (long c0 = 0; c0 < ncells; c0 += chunk) {
+      PredictorArgs ac = a;
+      ac.cell0 = a.cell0 + c0;
+      const dim3 grid((unsigned)(ncells - c0 < chunk ? ncells - c0 : chunk)), block(node_threads(B));
+      if (sd.precision == kFP32Bulk) {
+        if (sd.terms != kFlux) return hipErrorNotSupported;
+        hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float>), grid, block, 0, s, ac);
+      } else if (sd.terms == kFlux) {
+        if (export_temps)
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, true, double>), grid, block, 0, s, ac);
+        else
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, double>), grid, block, 0, s, ac);
+      } else if (sd.terms == (kFlux | kSource | kNCP)) {
+        if (export_temps)
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux | kSource | kNCP, true, double>), grid, block, 0, s, ac);
+        else
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux | kSource | kNCP, false, double>), grid, block, 0, s,
+                             ac);
+      } else {
+        return hipErrorNotSupported;
+      }
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
     }
-#define EXA_PRED(T)                                                                               \
-  if (sd.terms == (T)) {                                                                          \
-    if (export_temps)                                                                             \
-      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), true, double>), grid, block, 0, s, a);    \
-    else                                                                                          \
-      hipLaunchKernelGGL((predictor_kernel<B, PDE, (T), false, double>), grid, block, 0, s, a);   \
-    return hipGetLastError();                                                                     \
-  }
-    EXA_PRED(kFlux)
-    EXA_PRED(kFlux | kSource | kNCP)
-#undef EXA_PRED
-    return hipErrorNotSupported;
+    return hipSuccess;
   });
+}
+
+size_t predictor_ws_bytes_per_cell(const SolverDesc& sd) {
+  size_t bytes = 0;
+  (void)dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
+    constexpr int V = PDE::V, D = PDE::V + PDE::P;
+    if (sd.precision == kFP32Bulk)
+      bytes = PredictorLayout<B, V, D, false, float>::ws_bytes;
+    else if (sd.terms & kNCP)
+      bytes = PredictorLayout<B, V, D, true, double>::ws_bytes;
+    else
+      bytes = PredictorLayout<B, V, D, false, double>::ws_bytes;
+    return hipSuccess;
+  });
+  return bytes;
 }
 
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s) {

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -138,6 +139,12 @@ static int predictor_impl(const exa_solver* s, long ncells, double* lQhbnd, doub
   a.lFi = d_fi.d();
   a.lQhi = d_qhi.d();
   a.lFhi = d_fhi.d();
+  DevBuf d_ws;
+  if (const size_t wsb = exa::predictor_ws_bytes_per_cell(sd)) {
+    a.ws_cells = std::min(ncells, exa::kPredictorWsCells);
+    EXA_TRY(d_ws.alloc(wsb * a.ws_cells));
+    a.ws = d_ws.p;
+  }
   EXA_TRY(exa::launch_predictor(sd, exp, a, ncells, nullptr));
   EXA_TRY(hipDeviceSynchronize());
   EXA_TRY(hipMemcpy(lQhbnd, d_qb.p, sizeof(double) * ncells * 6 * z.B2 * z.D, hipMemcpyDeviceToHost));
@@ -302,6 +309,8 @@ struct exa_grid {
   size_t bulk = 8;         // bytes per bulk element
   double* d_dx = nullptr;
   int* iters = nullptr;
+  void* ws = nullptr;      // predictor workspace (large orders only)
+  long ws_cells = 0;
   FaceDesc* faces = nullptr;
   exa::StepState* st = nullptr;
   double* dt_hist = nullptr;
@@ -340,6 +349,7 @@ void free_grid(exa_grid* g) {
   (void)hipFree(g->lFhbnd);
   (void)hipFree(g->d_dx);
   (void)hipFree(g->iters);
+  (void)hipFree(g->ws);
   (void)hipFree(g->faces);
   (void)hipFree(g->st);
   (void)hipFree(g->dt_hist);
@@ -376,6 +386,8 @@ hipError_t launch_predict(exa_grid* g) {
   a.dt = &g->st->dt;
   a.iters = g->iters;
   a.cell0 = 0;
+  a.ws = g->ws;
+  a.ws_cells = g->ws_cells;
   hipError_t e = staged(g, EXA_STAGE_PREDICTOR, [&] { return exa::launch_predictor(g->sd, false, a, g->ncells, g->stream); });
   if (e == hipSuccess && g->profiling) e = exa::launch_count_iterations(g->iters, g->ncells, g->st, g->stream);
   return e;
@@ -441,6 +453,10 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   A((void**)&g->lFhbnd, g->bulk * g->ncells * 6 * z.B2 * z.V);
   A((void**)&g->d_dx, sizeof(double) * 3);
   A((void**)&g->iters, sizeof(int) * g->ncells);
+  if (const size_t wsb = exa::predictor_ws_bytes_per_cell(sd)) {
+    g->ws_cells = std::min(g->ncells, exa::kPredictorWsCells);
+    A(&g->ws, wsb * g->ws_cells);
+  }
   // Periodic: every cell owns its +x, +y, +z face.
   g->nfaces = 3 * g->ncells;
   A((void**)&g->faces, sizeof(FaceDesc) * g->nfaces);

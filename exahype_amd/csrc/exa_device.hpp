@@ -56,10 +56,9 @@ __device__ inline T block_sum(T v, T* red) {
   constexpr int NW = NT / kWave;
   v = wave_sum(v);
   if (NW == 1) return v;
-  const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   __syncthreads();
-  if (lane == 0) red[wid] = v;
+  red[wid] = v;  // every lane of the wave writes the same value: no divergent branch
   __syncthreads();
   T s = red[0];
 #pragma unroll
@@ -72,10 +71,9 @@ __device__ inline T block_min(T v, T* red) {
   constexpr int NW = NT / kWave;
   v = wave_min(v);
   if (NW == 1) return v;
-  const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   __syncthreads();
-  if (lane == 0) red[wid] = v;
+  red[wid] = v;  // every lane of the wave writes the same value: no divergent branch
   __syncthreads();
   T s = red[0];
 #pragma unroll

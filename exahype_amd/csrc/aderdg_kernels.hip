@@ -140,10 +140,11 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   T* const out_lQhbnd = static_cast<T*>(a.lQhbnd);
   T* const out_lFhbnd = static_cast<T*>(a.lFhbnd);
 
-  // Control flow is uniform across the workgroup: lanes beyond the B^3 nodes
-  // compute node 0 again and write the same values, and loops over tasks or
-  // outputs run whole rounds with the index clamped (surplus lanes redo the last
-  // item).  Divergent branches next to register spills are what this avoids.
+  // Lanes beyond the B^3 nodes compute node 0 again and write the same values,
+  // and loops over outputs run whole rounds with the index clamped (surplus
+  // lanes redo the last item), so control flow stays uniform; only the in-place
+  // line contraction guards its stores.  (No instantiation may spill: the build
+  // rejects any kernel with scratch memory.)
   const size_t cell = (size_t)blockIdx.x + a.cell0;
   const int tid = threadIdx.x;
   const bool active = tid < B3;
@@ -252,8 +253,11 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
         constexpr int NTASK = 3 * B2 * V;
 #pragma unroll 1
         for (int round = 0; round < (NTASK + NT - 1) / NT; ++round) {
-          // surplus lanes of the last round redo the last task (identical writes)
-          const int task = min(tid + round * NT, NTASK - 1);
+          // Surplus lanes of the last round compute the last task again but
+          // must not store: the contraction is in place, and a duplicate in
+          // another wavefront could read the line after the owner rewrote it.
+          const int raw = tid + round * NT;
+          const int task = min(raw, NTASK - 1);
           const int d = task / (B2 * V);
           const int rem = task - d * (B2 * V);
           const int m = rem / B2;
@@ -274,7 +278,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
             T acc = T(0);
 #pragma unroll
             for (int n = 0; n < B; ++n) acc += f[n] * tc<T>(exa_kxi[N][n][l]);
-            dst[l * stride] = acc;
+            if (raw < NTASK) dst[l * stride] = acc;
           }
         }
         __syncthreads();

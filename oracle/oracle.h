@@ -90,6 +90,44 @@ void ora_grid_picard_iterations(const ora_grid* g, int* out);
  * dt) for the first m cells; solution of those cells is advanced. */
 void ora_grid_sample_step(ora_grid* g, long m, int nthreads);
 
+/* ---- limiter path (limiter.c) ---- */
+/* Projection tables of one order: uh2lim (B x L), uh2lob (B x B), lim2uh
+ * (L x B), L = 2N+1.  Returns 0 if the least-squares matrix is singular. */
+int ora_limiter_tables(int order, double* uh2lim, double* uh2lob, double* lim2uh);
+/* DG (z,y,x,var) <-> FV patch (S,S,S,var), S = L + 2g, interior only. */
+void ora_project_on_fv(int order, int V, int g, const double* uh2lim, const double* luh, double* lim);
+void ora_project_on_dg(int order, int V, int g, const double* lim2uh, const double* lim, double* luh);
+void ora_cell_local_min_max(int order, int V, int nobs, const double* uh2lim, const double* uh2lob,
+                            const double* luh, double* mn, double* mx);
+void ora_limiter_local_min_max(int order, int D, int nobs, int g, const double* lim, double* mn, double* mx);
+int ora_dmp(int order, int V, int nobs, const double* uh2lim, const double* uh2lob, const double* luh,
+            double relax, double scaling, double* bmin, double* bmax);
+int ora_euler_admissible(const double* mn);
+double ora_fv_rusanov(const ora_solver* s, double* fnL, double* fnR, const double* qL, const double* qR, int d);
+double ora_muscl_update(const ora_solver* s, int L, int g, double* luh_new, const double* luh,
+                        const double* dx, double dt);
+void ora_fv_ghost_fill(int L, int g, int D, double* luh, const double* nb, int dir, int side);
+void ora_fv_boundary_transmissive(int L, int g, int D, double* luh, int face);
+double ora_fv_stable_dt(const ora_solver* s, int L, int g, const double* luh, const double* dx, double cfl);
+
+/* Limiting grid (uniform, periodic or transmissive), restated driver. */
+typedef struct ora_lgrid ora_lgrid;
+ora_lgrid* ora_lgrid_create(const ora_solver* s, int nx, int ny, int nz, const double* lo, const double* hi,
+                            int periodic, int helper_layers, int steps_till_cured, double relax,
+                            double scaling);
+void ora_lgrid_destroy(ora_lgrid* g);
+long ora_lgrid_num_cells(const ora_lgrid* g);
+/* lim_ic: FV initial condition at subcell centres [ncells][L^3][V+P] or NULL */
+void ora_lgrid_set_solution(ora_lgrid* g, const double* luh, const double* lim_ic);
+void ora_lgrid_step(ora_lgrid* g, int nsteps, int* status_out, double* dt_out, int* irregular_out);
+void ora_lgrid_get_solution(const ora_lgrid* g, double* luh);
+void ora_lgrid_get_status(const ora_lgrid* g, int* status);
+void ora_lgrid_get_limiter_solution(const ora_lgrid* g, long c, double* lim);
+void ora_lgrid_get_min_max(const ora_lgrid* g, double* mn, double* mx);
+double ora_lgrid_time(const ora_lgrid* g);
+double ora_lgrid_dt(const ora_lgrid* g);
+int ora_lgrid_last_irregular(const ora_lgrid* g);
+
 #ifdef __cplusplus
 }
 #endif

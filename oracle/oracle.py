@@ -68,6 +68,42 @@ def lib():
         L.ora_grid_time.restype = ctypes.c_double
         L.ora_grid_picard_iterations.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.ora_grid_sample_step.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int]
+        I = ctypes.c_int
+        IP = ctypes.POINTER(ctypes.c_int)
+        D_ = ctypes.c_double
+        L.ora_limiter_tables.argtypes = [I, P, P, P]
+        L.ora_limiter_tables.restype = I
+        L.ora_project_on_fv.argtypes = [I, I, I, P, P, P]
+        L.ora_project_on_dg.argtypes = [I, I, I, P, P, P]
+        L.ora_cell_local_min_max.argtypes = [I, I, I, P, P, P, P, P]
+        L.ora_limiter_local_min_max.argtypes = [I, I, I, I, P, P, P]
+        L.ora_dmp.argtypes = [I, I, I, P, P, P, D_, D_, P, P]
+        L.ora_dmp.restype = I
+        L.ora_euler_admissible.argtypes = [P]
+        L.ora_euler_admissible.restype = I
+        L.ora_fv_rusanov.argtypes = [S, P, P, P, P, I]
+        L.ora_fv_rusanov.restype = D_
+        L.ora_muscl_update.argtypes = [S, I, I, P, P, P, D_]
+        L.ora_muscl_update.restype = D_
+        L.ora_fv_ghost_fill.argtypes = [I, I, I, P, P, I, I]
+        L.ora_fv_boundary_transmissive.argtypes = [I, I, I, P, I]
+        L.ora_fv_stable_dt.argtypes = [S, I, I, P, P, D_]
+        L.ora_fv_stable_dt.restype = D_
+        L.ora_lgrid_create.argtypes = [S, I, I, I, P, P, I, I, I, D_, D_]
+        L.ora_lgrid_create.restype = ctypes.c_void_p
+        L.ora_lgrid_destroy.argtypes = [ctypes.c_void_p]
+        L.ora_lgrid_num_cells.argtypes = [ctypes.c_void_p]
+        L.ora_lgrid_num_cells.restype = ctypes.c_long
+        L.ora_lgrid_set_solution.argtypes = [ctypes.c_void_p, P, P]
+        L.ora_lgrid_step.argtypes = [ctypes.c_void_p, I, IP, P, IP]
+        L.ora_lgrid_get_solution.argtypes = [ctypes.c_void_p, P]
+        L.ora_lgrid_get_status.argtypes = [ctypes.c_void_p, IP]
+        L.ora_lgrid_get_limiter_solution.argtypes = [ctypes.c_void_p, ctypes.c_long, P]
+        L.ora_lgrid_get_min_max.argtypes = [ctypes.c_void_p, P, P]
+        L.ora_lgrid_time.argtypes = [ctypes.c_void_p]
+        L.ora_lgrid_time.restype = D_
+        L.ora_lgrid_dt.argtypes = [ctypes.c_void_p]
+        L.ora_lgrid_dt.restype = D_
         _lib = L
     return _lib
 
@@ -196,3 +232,147 @@ class Grid:
 
     def sample_step(self, m, nthreads=0):
         lib().ora_grid_sample_step(self._h, m, nthreads)
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+# ---------------------------------------------------------------------------
+# limiter path
+# ---------------------------------------------------------------------------
+def limiter_tables(order):
+    """(uh2lim [B,L], uh2lob [B,B], lim2uh [L,B])."""
+    B, L = order + 1, 2 * order + 1
+    a, b, c = np.zeros(B * L), np.zeros(B * B), np.zeros(L * B)
+    if not lib().ora_limiter_tables(order, _p(a), _p(b), _p(c)):
+        raise RuntimeError("singular least-squares matrix")
+    return a.reshape(B, L), b.reshape(B, B), c.reshape(L, B)
+
+
+def project_on_fv(order, V, g, luh, lim=None):
+    B, L = order + 1, 2 * order + 1
+    S = L + 2 * g
+    uh2lim = np.ascontiguousarray(limiter_tables(order)[0])
+    out = np.zeros(S ** 3 * V) if lim is None else np.array(lim, dtype=np.float64).ravel()
+    lib().ora_project_on_fv(order, V, g, _p(uh2lim.ravel()), _p(np.ascontiguousarray(luh, dtype=np.float64).ravel()),
+                            _p(out))
+    return out
+
+
+def project_on_dg(order, V, g, lim):
+    B = order + 1
+    lim2uh = np.ascontiguousarray(limiter_tables(order)[2])
+    out = np.zeros(B ** 3 * V)
+    lib().ora_project_on_dg(order, V, g, _p(lim2uh.ravel()), _p(np.ascontiguousarray(lim, dtype=np.float64).ravel()),
+                            _p(out))
+    return out
+
+
+def cell_local_min_max(order, V, luh, nobs=None):
+    nobs = V if nobs is None else nobs
+    a, b, _ = limiter_tables(order)
+    mn, mx = np.zeros(nobs), np.zeros(nobs)
+    lib().ora_cell_local_min_max(order, V, nobs, _p(np.ascontiguousarray(a).ravel()), _p(np.ascontiguousarray(b).ravel()),
+                                 _p(np.ascontiguousarray(luh, dtype=np.float64).ravel()), _p(mn), _p(mx))
+    return mn, mx
+
+
+def dmp(order, V, luh, bmin, bmax, relax=1e-4, scaling=1e-3):
+    """Returns (ok, bmin', bmax') with faces 0/1 overwritten as the reference does."""
+    a, b, _ = limiter_tables(order)
+    bmin = np.array(bmin, dtype=np.float64).ravel()
+    bmax = np.array(bmax, dtype=np.float64).ravel()
+    ok = lib().ora_dmp(order, V, V, _p(np.ascontiguousarray(a).ravel()), _p(np.ascontiguousarray(b).ravel()),
+                       _p(np.ascontiguousarray(luh, dtype=np.float64).ravel()), relax, scaling, _p(bmin), _p(bmax))
+    return bool(ok), bmin, bmax
+
+
+def fv_rusanov(s, qL, qR, d):
+    fL, fR = np.zeros(s.nvar), np.zeros(s.nvar)
+    smax = lib().ora_fv_rusanov(ctypes.byref(s), _p(fL), _p(fR), _p(np.ascontiguousarray(qL, dtype=np.float64)),
+                                _p(np.ascontiguousarray(qR, dtype=np.float64)), d)
+    return fL, fR, smax
+
+
+def muscl_update(s, L, g, patch, dx, dt):
+    """One MUSCL-Hancock update of a patch with filled ghost layers; returns (new patch, dt_max)."""
+    old = np.ascontiguousarray(patch, dtype=np.float64).ravel()
+    new = old.copy()
+    dtm = lib().ora_muscl_update(ctypes.byref(s), L, g, _p(new), _p(old), _p(dx3(dx)), dt)
+    return new, dtm
+
+
+def fv_ghost_fill(L, g, D, patch, neighbour, direction, side):
+    out = np.array(patch, dtype=np.float64).ravel()
+    lib().ora_fv_ghost_fill(L, g, D, _p(out), _p(np.ascontiguousarray(neighbour, dtype=np.float64).ravel()),
+                            direction, side)
+    return out
+
+
+def fv_boundary_transmissive(L, g, D, patch, face):
+    out = np.array(patch, dtype=np.float64).ravel()
+    lib().ora_fv_boundary_transmissive(L, g, D, _p(out), face)
+    return out
+
+
+class LimitingGrid:
+    """Uniform grid with the restated limiting driver (non-fused time stepping)."""
+
+    def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), periodic=False, helper_layers=1,
+                 steps_till_cured=0, relax=1e-4, scaling=1e-3):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.s = s
+        self.n = tuple(n)
+        self._h = lib().ora_lgrid_create(ctypes.byref(s), n[0], n[1], n[2], _p(dx3(lo)), _p(dx3(hi)), int(periodic),
+                                         helper_layers, steps_till_cured, relax, scaling)
+        self.ncells = lib().ora_lgrid_num_cells(self._h)
+        self.L = 2 * s.order + 1
+        self.S = self.L + 4
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ora_lgrid_destroy(self._h)
+            self._h = None
+
+    def set_solution(self, luh, lim_ic=None):
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        lim_ic = None if lim_ic is None else np.ascontiguousarray(lim_ic, dtype=np.float64).ravel()
+        lib().ora_lgrid_set_solution(self._h, _p(luh), _p(lim_ic))
+
+    def step(self, nsteps):
+        st = np.zeros(nsteps * self.ncells, dtype=np.int32)
+        dts = np.zeros(nsteps)
+        irr = np.zeros(nsteps, dtype=np.int32)
+        lib().ora_lgrid_step(self._h, nsteps, _ip(st), _p(dts), _ip(irr))
+        return st.reshape(nsteps, self.ncells), dts, irr
+
+    def solution(self):
+        B = self.s.order + 1
+        out = np.zeros(self.ncells * B ** 3 * (self.s.nvar + self.s.npar))
+        lib().ora_lgrid_get_solution(self._h, _p(out))
+        return out
+
+    def status(self):
+        out = np.zeros(self.ncells, dtype=np.int32)
+        lib().ora_lgrid_get_status(self._h, _ip(out))
+        return out
+
+    def limiter_solution(self, c):
+        out = np.zeros(self.S ** 3 * (self.s.nvar + self.s.npar))
+        lib().ora_lgrid_get_limiter_solution(self._h, c, _p(out))
+        return out
+
+    def min_max(self):
+        mn = np.zeros(self.ncells * 6 * self.s.nvar)
+        mx = np.zeros(self.ncells * 6 * self.s.nvar)
+        lib().ora_lgrid_get_min_max(self._h, _p(mn), _p(mx))
+        return mn, mx
+
+    def time(self):
+        return lib().ora_lgrid_time(self._h)
+
+    def dt(self):
+        return lib().ora_lgrid_dt(self._h)

@@ -131,3 +131,44 @@ def smooth_euler_cells(rng, ncells, order, scale=0.05):
         E = p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
         out.append(np.stack([rho, rho * u, rho * v, rho * w, E], -1))
     return np.ascontiguousarray(np.array(out).reshape(-1))
+
+
+def splitmix64_uniform(counter, seed):
+    """Uniform [0,1) doubles from a counter-based splitmix64: state = seed +
+    (counter+1) * 0x9E3779B97F4A7C15, one mixing round, top 53 bits."""
+    with np.errstate(over="ignore"):
+        c = np.asarray(counter, dtype=np.uint64)
+        z = np.uint64(seed) + (c + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def blast_cells(order, n, seed=7, nodes=None, radius=0.2, p_in=10.0, p_out=0.1, noise=0.01, subcells=False):
+    """Spherical blast on the unit cube: rho = 1 + noise*(2u-1) with u from
+    splitmix64 keyed by the global node index (layout independent), u = 0,
+    p = p_in inside |x - 1/2| < radius, p_out outside.  [ncells*B^3*5].
+    subcells=True evaluates it at the 2N+1 FV subcell centres per direction
+    instead (noise keyed by the global subcell index, stream seed + 1)."""
+    if subcells:
+        L = 2 * order + 1
+        nodes = (2.0 * np.arange(L) + 1.0) / (2.0 * L)
+        order, seed = L - 1, seed + 1
+    B = order + 1
+    n3 = (n, n, n) if isinstance(n, int) else n
+    X = node_coords(order, n3, nodes=nodes)  # [c, z, y, x, 3]
+    nc = X.shape[0]
+    c = np.arange(nc)
+    ci, cj, ck = c % n3[0], (c // n3[0]) % n3[1], c // (n3[0] * n3[1])
+    z, y, x = np.meshgrid(np.arange(B), np.arange(B), np.arange(B), indexing="ij")
+    gx = ci[:, None, None, None] * B + x[None]
+    gy = cj[:, None, None, None] * B + y[None]
+    gz = ck[:, None, None, None] * B + z[None]
+    gid = (gz * (n3[1] * B) + gy) * (n3[0] * B) + gx
+    u = splitmix64_uniform(gid.astype(np.uint64), seed)
+    rho = 1.0 + noise * (2.0 * u - 1.0)
+    r = np.sqrt(((X - 0.5) ** 2).sum(-1))
+    p = np.where(r < radius, p_in, p_out)
+    Q = np.stack([rho, 0 * rho, 0 * rho, 0 * rho, p / (GAMMA - 1.0)], -1)
+    return np.ascontiguousarray(Q.reshape(-1))

@@ -112,6 +112,68 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}), flush=True)
 
 
+def cpu_baseline_limiter(n, seconds):
+    """The oracle's limiting driver, single thread, on a small blast grid
+    (rate per DOF-update, an intensive figure)."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refutil import blast_cells
+    order, m = 3, 6
+    s = O.make_solver(order)
+    nodes = np.array([O.table(O.T_NODES, order, i) for i in range(order + 1)])
+    g = O.LimitingGrid(s, m)
+    g.set_solution(blast_cells(order, m, nodes=nodes, radius=0.3),
+                   blast_cells(order, m, subcells=True, radius=0.3))
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        g.step(1)
+        steps += 1
+    el = time.perf_counter() - t0
+    st = g.status()
+    return dict(value=m ** 3 * (order + 1) ** 3 * steps / el, unit="DOF-updates/s", cores=1, kind="port",
+                sample=f"{steps} steps of the limiting driver (oracle) on a {m}^3 N=3 blast (r=0.3, periodic), "
+                       f"{(st >= 2).mean():.2f} of the cells FV-active at the end, 1 thread, {el:.1f} s")
+
+
+def bench_limiter(capi, args):
+    """Limiter workload: Euler blast, N=3, 81^3, transmissive, FP64."""
+    order, n = 3, args.cells
+    B = order + 1
+    g = capi.LimitingGrid(capi.euler_solver(order), n)
+    g.init_blast(seed=args.seed, radius=0.2)
+    g.step(args.warmup)
+    g.synchronize()
+    g.set_profiling(True)
+    g.stage_times()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    stages = g.stage_times()
+    st, irr = g.status()
+    t_now, dt_now, nsteps = g.time()
+    if not np.isfinite(dt_now) or dt_now <= 0:
+        raise RuntimeError("non-finite time step")
+    dof = n ** 3 * B ** 3
+    out = {
+        "metric": "DOF-updates/s, 3D Euler blast N=3 81^3 with a-posteriori subcell limiter (FP64)",
+        "value": dof * args.steps / wall, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (spherical blast + 1% seeded density noise, generated on the device)",
+        "config": {"workload": f"euler3d-limiter-blast {n}^3 cells N={order} transmissive, MUSCL-Hancock 7^3 "
+                               f"subcells, DMP 1e-4/1e-3, helper layers 1", "cells": n ** 3,
+                   "dof_per_step": dof, "seed": args.seed},
+        "stage_ms_per_step": {k: round(v[0] / max(args.steps, 1), 4) for k, v in stages.items()},
+        "limited": {"fv_active_fraction": float((st >= 2).mean()), "troubled_fraction": float((st == 3).mean()),
+                    "irregular_steps_total": int(irr)},
+        "roofline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_limiter(n, args.cpu_seconds),
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +186,9 @@ def main():
                     help="both: time FP64 and FP32-bulk and report the error between them")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("precision", "limiter"), default="precision",
+                    help="precision: Euler N=5 81^3, FP32-bulk vs FP64 (the headline line); "
+                         "limiter: Euler blast N=3 81^3 with the subcell limiter, FP64")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times): the known-size "
                          "reference for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -139,6 +204,9 @@ def main():
 
     from exahype_amd import capi
     capi.lib()
+    if args.workload == "limiter":
+        bench_limiter(capi, args)
+        return
     order, n = args.order, args.cells
     B, V, D = order + 1, 5, 5
     ncells = n ** 3

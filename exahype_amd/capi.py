@@ -20,6 +20,7 @@ TERM_FLUX, TERM_SOURCE, TERM_NCP = 1, 2, 4
 STAGE_PREDICTOR, STAGE_RIEMANN, STAGE_UPDATE, STAGE_DT = range(4)
 STAGE_NAMES = ("predictor", "riemann", "update", "dt")
 BC_PERIODIC = 0
+BC_TRANSMISSIVE = 1
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
 
 # Every symbol declared in include/exahype_amd.h.
@@ -32,6 +33,11 @@ EXPORTED = (
     "exa_grid_download_solution", "exa_grid_init_euler", "exa_grid_step", "exa_grid_synchronize",
     "exa_grid_time", "exa_grid_dt_history", "exa_grid_picard_iterations",
     "exa_grid_set_profiling", "exa_grid_stage_times", "exa_grid_download_cells", "exa_grid_picard_totals",
+    "exa_project_on_fv_limiter_space", "exa_project_on_dg_space", "exa_find_cell_local_min_and_max",
+    "exa_discrete_maximum_principle_and_min_and_max_search", "exa_musclhancock_solution_update",
+    "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
+    "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
+    "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
 )
 
 
@@ -107,6 +113,27 @@ def lib():
     L.exa_grid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     L.exa_grid_download_cells.argtypes = [V, ctypes.c_long, ctypes.c_long, _D]
     L.exa_grid_picard_totals.argtypes = [V, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
+    i, d_ = ctypes.c_int, ctypes.c_double
+    L.exa_project_on_fv_limiter_space.argtypes = [_D, i, i, i, _D]
+    L.exa_project_on_dg_space.argtypes = [_D, i, i, i, _D]
+    L.exa_find_cell_local_min_and_max.argtypes = [_D, i, i, _D, _D]
+    L.exa_discrete_maximum_principle_and_min_and_max_search.argtypes = [_D, i, i, d_, d_, _D, _D, _I]
+    L.exa_musclhancock_solution_update.argtypes = [S, _D, _D, _D, d_, i, i]
+    L.exa_lgrid_create.argtypes = [S, i, i, i, _D, _D, i, i, i, d_, d_, ctypes.POINTER(V)]
+    L.exa_lgrid_destroy.argtypes = [V]
+    L.exa_lgrid_num_cells.argtypes = [V]
+    L.exa_lgrid_num_cells.restype = ctypes.c_long
+    L.exa_lgrid_upload.argtypes = [V, _D, _D]
+    L.exa_lgrid_init_blast.argtypes = [V, ctypes.c_ulonglong, d_, d_, d_, d_]
+    L.exa_lgrid_step.argtypes = [V, i]
+    L.exa_lgrid_synchronize.argtypes = [V]
+    L.exa_lgrid_time.argtypes = [V, _D, _D, ctypes.POINTER(ctypes.c_long)]
+    L.exa_lgrid_download.argtypes = [V, _D]
+    L.exa_lgrid_status.argtypes = [V, _I, ctypes.POINTER(ctypes.c_long)]
+    L.exa_lgrid_limiter_solution.argtypes = [V, ctypes.c_long, _D]
+    L.exa_lgrid_min_max.argtypes = [V, _D, _D]
+    L.exa_lgrid_set_profiling.argtypes = [V, i]
+    L.exa_lgrid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     _lib = L
     return L
 
@@ -285,3 +312,129 @@ class Grid:
         n = (ctypes.c_long * 4)()
         check(lib().exa_grid_stage_times(self._h, _dp(ms), n), "stage times")
         return {STAGE_NAMES[i]: (float(ms[i]), int(n[i])) for i in range(4)}
+
+
+# ---------------------------------------------------------------------------
+# kernels::limiter::generic::c and MUSCL-Hancock single-cell entry points
+# ---------------------------------------------------------------------------
+def project_on_fv(luh, basis_size, ghost_layer_width, lim=None, num_variables=5):
+    L = 2 * basis_size - 1
+    S = L + 2 * ghost_layer_width
+    out = np.zeros(S ** 3 * num_variables) if lim is None else _f64(lim).copy()
+    check(lib().exa_project_on_fv_limiter_space(_dp(_f64(luh).ravel()), num_variables, basis_size,
+                                                ghost_layer_width, _dp(out)), "project_on_fv")
+    return out
+
+
+def project_on_dg(lim, basis_size, ghost_layer_width, num_variables=5):
+    out = np.zeros(basis_size ** 3 * num_variables)
+    check(lib().exa_project_on_dg_space(_dp(_f64(lim).ravel()), num_variables, basis_size, ghost_layer_width,
+                                        _dp(out)), "project_on_dg")
+    return out
+
+
+def cell_local_min_max(luh, basis_size, num_variables=5):
+    mn, mx = np.zeros(num_variables), np.zeros(num_variables)
+    check(lib().exa_find_cell_local_min_and_max(_dp(_f64(luh).ravel()), num_variables, basis_size, _dp(mn),
+                                                _dp(mx)), "local min/max")
+    return mn, mx
+
+
+def dmp(luh, basis_size, bmin, bmax, relax=1e-4, scaling=1e-3, num_variables=5):
+    bmin = _f64(bmin).ravel().copy()
+    bmax = _f64(bmax).ravel().copy()
+    ok = np.zeros(1, dtype=np.int32)
+    check(lib().exa_discrete_maximum_principle_and_min_and_max_search(
+        _dp(_f64(luh).ravel()), num_variables, basis_size, relax, scaling, _dp(bmin), _dp(bmax),
+        ok.ctypes.data_as(_I)), "dmp")
+    return bool(ok[0]), bmin, bmax
+
+
+def muscl_update(s, patch, dx, dt):
+    """One MUSCL-Hancock update of a patch with filled ghost layers (interior written)."""
+    old = _f64(patch).ravel()
+    new = old.copy()
+    L = 2 * s.order + 1
+    check(lib().exa_musclhancock_solution_update(ctypes.byref(s), _dp(new), _dp(old), _dp(_dx3(dx)), dt, L, 2),
+          "musclhancock")
+    return new
+
+
+class LimitingGrid:
+    """A uniform grid with the a-posteriori subcell limiter (exa_lgrid)."""
+
+    def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), boundary=BC_TRANSMISSIVE, helper_layers=1,
+                 steps_till_cured=0, relax=1e-4, scaling=1e-3):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.s = s
+        self.n = tuple(n)
+        h = ctypes.c_void_p()
+        check(lib().exa_lgrid_create(ctypes.byref(s), n[0], n[1], n[2], _dp(_dx3(lo)), _dp(_dx3(hi)), boundary,
+                                     helper_layers, steps_till_cured, relax, scaling, ctypes.byref(h)),
+              "exa_lgrid_create")
+        self._h = h
+        self.ncells = lib().exa_lgrid_num_cells(h)
+        self.L = 2 * s.order + 1
+        self.S = self.L + 4
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().exa_lgrid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, luh, lim_ic=None):
+        check(lib().exa_lgrid_upload(self._h, _dp(_f64(luh).ravel()),
+                                     None if lim_ic is None else _dp(_f64(lim_ic).ravel())), "upload")
+
+    def init_blast(self, seed=7, radius=0.2, p_in=10.0, p_out=0.1, noise=0.01):
+        check(lib().exa_lgrid_init_blast(self._h, seed, radius, p_in, p_out, noise), "init_blast")
+
+    def step(self, n=1):
+        check(lib().exa_lgrid_step(self._h, n), "step")
+
+    def synchronize(self):
+        check(lib().exa_lgrid_synchronize(self._h), "synchronize")
+
+    def time(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
+        check(lib().exa_lgrid_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "time")
+        return t.value, dt.value, n.value
+
+    def solution(self):
+        out = np.zeros(self.ncells * self.s.B ** 3 * 5)
+        check(lib().exa_lgrid_download(self._h, _dp(out)), "download")
+        return out
+
+    def status(self):
+        out = np.zeros(self.ncells, dtype=np.int32)
+        irr = ctypes.c_long()
+        check(lib().exa_lgrid_status(self._h, out.ctypes.data_as(_I), ctypes.byref(irr)), "status")
+        return out, irr.value
+
+    def limiter_solution(self, c):
+        out = np.zeros(self.S ** 3 * 5)
+        check(lib().exa_lgrid_limiter_solution(self._h, c, _dp(out)), "limiter_solution")
+        return out
+
+    def min_max(self):
+        mn = np.zeros(self.ncells * 30)
+        mx = np.zeros(self.ncells * 30)
+        check(lib().exa_lgrid_min_max(self._h, _dp(mn), _dp(mx)), "min_max")
+        return mn, mx
+
+    def set_profiling(self, on):
+        check(lib().exa_lgrid_set_profiling(self._h, int(on)), "set_profiling")
+
+    def stage_times(self):
+        ms = np.zeros(5)
+        n = (ctypes.c_long * 5)()
+        check(lib().exa_lgrid_stage_times(self._h, _dp(ms), n), "stage_times")
+        names = ("predictor", "riemann", "update", "dt", "limiter")
+        return {names[i]: (float(ms[i]), int(n[i])) for i in range(5)}

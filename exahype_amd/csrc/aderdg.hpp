@@ -74,6 +74,10 @@ struct PredictorArgs {
   // where predictor_ws_bytes_per_cell is 0.
   void* ws;
   long ws_cells;
+  // optional gate: cells with status[cell] >= status_limit are skipped (the
+  // limiter's troubled cells); null status = every cell
+  const int* status;
+  int status_limit;
 };
 
 struct RiemannArgs {
@@ -81,6 +85,10 @@ struct RiemannArgs {
   long nfaces;
   const void* lQhbnd;  // (bulk)
   void* lFhbnd;        // (bulk)
+  // optional gate: faces whose two cells both have status >= status_limit are
+  // skipped (faces between FV-active limiter cells); null status = every face
+  const int* status;
+  int status_limit;
 };
 
 struct UpdateArgs {

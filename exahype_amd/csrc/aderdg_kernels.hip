@@ -17,6 +17,8 @@
 #include "aderdg.hpp"
 #include "exa_device.hpp"
 
+#include <cfloat>
+
 namespace exa {
 
 // Picard stopping rule (spaceTimePredictorNonlinear.cpph:314-318): stop when
@@ -150,6 +152,10 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   const bool active = tid < B3;
   const int node = active ? tid : 0;
   const int z = node / B2, y = (node / B) % B, x = node % B;
+  if (a.status && a.status[cell] >= a.status_limit) {  // whole workgroup
+    if (a.iters && tid == 0) a.iters[cell] = 0;
+    return;
+  }
 
   for (int i = tid; i < B2; i += NT) {
     sK[i] = tc<T>(exa_kxi[N][i / B][i % B]);
@@ -538,6 +544,7 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
   const bool active = lane < B2;
   const int node = active ? lane : 0;
   const FaceDesc fd = a.faces[f];
+  if (a.status && a.status[fd.left] >= a.status_limit && a.status[fd.right] >= a.status_limit) return;
   const int d = fd.dir;
   T* FL = static_cast<T*>(a.lFhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * V);
   T* FR = static_cast<T*>(a.lFhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
@@ -662,7 +669,8 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
     }
   }
   if (a.dt_slots || a.dt_cell) {
-    double dtn = active ? node_stable_dt<PDE>(q, invDx, a.cfl_pnpm) : 1.0e308;
+    double dtn = active ? node_stable_dt<PDE>(q, invDx, a.cfl_pnpm) : DBL_MAX;
+    dtn = dtn < DBL_MAX ? dtn : DBL_MAX;  // a NaN or inf candidate never wins (std::min order)
     dtn = block_min<NT>(dtn, sRed);
     if (tid == 0) {
       if (a.dt_cell) a.dt_cell[cell] = dtn;

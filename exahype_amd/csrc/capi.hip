@@ -16,61 +16,7 @@
 using exa::FaceDesc;
 using exa::SolverDesc;
 
-namespace {
-
-// PNPM[order], stableTimeStepSize.cpph:33-34
-const double kPNPM[10] = {1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015};
-
-int from_hip(hipError_t e) {
-  if (e == hipSuccess) return EXA_OK;
-  if (e == hipErrorNotSupported) return EXA_E_UNSUPPORTED;
-  if (e == hipErrorOutOfMemory) return EXA_E_NOMEM;
-  return EXA_E_HIP;
-}
-
-#define EXA_TRY(expr)                     \
-  do {                                    \
-    const hipError_t e_ = (expr);         \
-    if (e_ != hipSuccess) return from_hip(e_); \
-  } while (0)
-
-bool to_desc(const exa_solver* s, SolverDesc* d) {
-  if (!s) return false;
-  d->pde = s->pde;
-  d->order = s->order;
-  d->nvar = s->num_variables;
-  d->npar = s->num_parameters;
-  d->terms = s->terms;
-  d->no_time_averaging = s->no_time_averaging;
-  d->cfl = s->cfl;
-  d->precision = s->precision;
-  return s->order >= 0 && s->order <= 9 && s->num_variables > 0 && s->num_parameters >= 0;
-}
-
-// Device buffer owned for the duration of one synchronous call.
-struct DevBuf {
-  void* p = nullptr;
-  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  double* d() const { return static_cast<double*>(p); }
-};
-
-struct Sizes {
-  long B, B2, B3, B4, D, V;
-  explicit Sizes(const SolverDesc& sd)
-      : B(sd.order + 1), B2(B * B), B3(B2 * B), B4(B3 * B), D(sd.nvar + sd.npar), V(sd.nvar) {}
-};
-
-uint64_t splitmix64(uint64_t& x) {
-  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-}  // namespace
+#include "capi_util.hpp"
 
 // ---------------------------------------------------------------------------
 extern "C" const char* exa_status_string(int status) {

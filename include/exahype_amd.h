@@ -167,6 +167,78 @@ int exa_grid_picard_totals(exa_grid* g, unsigned long long* iterations, unsigned
  * since profiling was switched on (and resets them). */
 int exa_grid_stage_times(exa_grid* g, double* ms, long* launches);
 
+/* ---------------------------------------------------------------------------
+ * A-posteriori subcell limiter (kernels::limiter::generic::c and the
+ * MUSCL-Hancock finite-volume kernels).  Euler, FP64, orders 2..3; FV patches
+ * are (L+2g)^3 cells with L = 2N+1, variable fastest, host arrays.
+ * ------------------------------------------------------------------------- */
+
+/* projectOnFVLimiterSpace (ExaHyPE/kernels/limiter/generic/c/Limiter.cpp:23):
+ * writes the interior of lim[(L+2g)^3 * V] from luh[B^3 * V]; ghosts untouched. */
+int exa_project_on_fv_limiter_space(const double* luh, int num_variables, int basis_size,
+                                    int ghost_layer_width, double* lim);
+/* projectOnDGSpace (Limiter.cpp:69): luh[B^3 * V] from the interior of lim. */
+int exa_project_on_dg_space(const double* lim, int num_variables, int basis_size,
+                            int ghost_layer_width, double* luh);
+/* findCellLocalMinAndMax (Limiter.cpp:210), observables = the variables:
+ * extrema over Gauss-Legendre nodes, Gauss-Lobatto nodes and subcell means. */
+int exa_find_cell_local_min_and_max(const double* luh, int num_variables, int basis_size,
+                                    double* local_min, double* local_max);
+/* discreteMaximumPrincipleAndMinAndMaxSearch (Limiter.cpp:114): bmin/bmax hold
+ * 6 face blocks of num_variables; faces 0/1 are overwritten with the local /
+ * boundary extrema; *dmp_ok = 1 when the DMP holds. */
+int exa_discrete_maximum_principle_and_min_and_max_search(const double* luh, int num_variables,
+                                                          int basis_size, double relaxation,
+                                                          double difference_scaling, double* bmin,
+                                                          double* bmax, int* dmp_ok);
+/* musclhancock::c::solutionUpdate (ExaHyPE/kernels/finitevolumes/musclhancock/
+ * c/3d/musclhancock.cpph:49) for Euler: luh_new = update(luh) on one patch
+ * with filled ghost layers (patch_size L, ghost_layer_width 2); dx is the
+ * DG cell size.  Only the interior of luh_new is written. */
+int exa_musclhancock_solution_update(const exa_solver* s, double* luh_new, const double* luh,
+                                     const double* dx, double dt, int patch_size,
+                                     int ghost_layer_width);
+
+/* Device-resident limiting grid: LimitingADERDGSolver's non-fused time step
+ * (ExaHyPE/exahype/solvers/LimitingADERDGSolver.cpp:806-1742, Runner.cpp:866-946,
+ * 1116-1162) -- DG / FV neighbour merging by limiter status, status-dependent
+ * update, DMP + admissibility, status spreading, rollback and local
+ * recomputation -- on one uniform level. */
+typedef struct exa_lgrid exa_lgrid;
+enum { EXA_BC_TRANSMISSIVE = 1 };
+/* boundary: EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE; helper_layers h gives
+ * troubled status 2h+1 and FV-active status >= h+1 (ADERDGSolver.cpp:461-462). */
+int exa_lgrid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo, const double* hi,
+                     int boundary, int helper_layers, int steps_till_cured, double dmp_relaxation,
+                     double dmp_difference_scaling, exa_lgrid** out);
+int exa_lgrid_destroy(exa_lgrid* g);
+long exa_lgrid_num_cells(const exa_lgrid* g);
+/* Initial state: DG solution luh [cells][B^3][V] and the FV initial condition
+ * at subcell centres lim_ic [cells][L^3][V] (may be NULL: projection of luh),
+ * used for the patches of initially troubled cells and their helpers. */
+int exa_lgrid_upload(exa_lgrid* g, const double* luh, const double* lim_ic);
+/* Spherical blast initial condition generated on the device: rho = 1 +
+ * noise (2u - 1), u from splitmix64 keyed by the global node (subcell) index,
+ * zero velocity, p = p_in inside |x - centre| < radius else p_out. */
+int exa_lgrid_init_blast(exa_lgrid* g, unsigned long long seed, double radius, double p_in,
+                         double p_out, double noise);
+int exa_lgrid_step(exa_lgrid* g, int nsteps);
+int exa_lgrid_synchronize(exa_lgrid* g);
+int exa_lgrid_time(exa_lgrid* g, double* t, double* dt, long* steps);
+int exa_lgrid_download(exa_lgrid* g, double* luh);
+/* Limiter status per cell, and the number of steps with an irregular change. */
+int exa_lgrid_status(exa_lgrid* g, int* status, long* irregular_steps);
+/* Current FV patch of one cell, (L+4)^3 * V. */
+int exa_lgrid_limiter_solution(exa_lgrid* g, long cell, double* lim);
+/* Solution min/max face blocks [cells][6][V]. */
+int exa_lgrid_min_max(exa_lgrid* g, double* mn, double* mx);
+/* Stage timing: EXA_STAGE_PREDICTOR, _RIEMANN, _UPDATE (DG update, FV update,
+ * DMP and status), _DT, and EXA_STAGE_LIMITER (merging of min/max and ghost
+ * layers, spreading, rollback, recomputation). */
+enum { EXA_STAGE_LIMITER = 4, EXA_NUM_LSTAGES = 5 };
+int exa_lgrid_set_profiling(exa_lgrid* g, int on);
+int exa_lgrid_stage_times(exa_lgrid* g, double* ms, long* launches);
+
 #ifdef __cplusplus
 }
 #endif

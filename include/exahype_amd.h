@@ -120,9 +120,12 @@ int exa_batch_predictor_volume(const exa_solver* s, long ncells, double* lQhbnd,
  * ------------------------------------------------------------------------- */
 typedef struct exa_grid exa_grid;
 
-enum { EXA_BC_PERIODIC = 0 };
+enum { EXA_BC_PERIODIC = 0, EXA_BC_TRANSMISSIVE = 1 };
 
-/* Uniform nx*ny*nz Cartesian grid on [lo, hi]; cell c = (k*ny + j)*nx + i. */
+/* Uniform nx*ny*nz Cartesian grid on [lo, hi]; cell c = (k*ny + j)*nx + i;
+ * boundary EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE (boundaryConditions,
+ * ExaHyPE/kernels/aderdg/generic/c/3d/boundaryConditions.cpph:24, with
+ * boundaryValues returning the inside state). */
 int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo,
                     const double* hi, int boundary, exa_grid** out);
 int exa_grid_destroy(exa_grid* g);
@@ -205,7 +208,6 @@ int exa_musclhancock_solution_update(const exa_solver* s, double* luh_new, const
  * update, DMP + admissibility, status spreading, rollback and local
  * recomputation -- on one uniform level. */
 typedef struct exa_lgrid exa_lgrid;
-enum { EXA_BC_TRANSMISSIVE = 1 };
 /* boundary: EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE; helper_layers h gives
  * troubled status 2h+1 and FV-active status >= h+1 (ADERDGSolver.cpp:461-462). */
 int exa_lgrid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo, const double* hi,

@@ -731,6 +731,7 @@ double ora_stable_time_step_size(const ora_solver* s, const double* luh, const d
 struct ora_grid {
   ora_solver s;
   int nx, ny, nz;
+  int transmissive;
   long ncells;
   double dx[3];
   double t, dt;
@@ -748,7 +749,13 @@ static long cell_index(const ora_grid* g, int i, int j, int k) {
 
 ora_grid* ora_grid_create(const ora_solver* s, int nx, int ny, int nz, const double* lo,
                           const double* hi) {
+  return ora_grid_create_bc(s, nx, ny, nz, lo, hi, 0);
+}
+
+ora_grid* ora_grid_create_bc(const ora_solver* s, int nx, int ny, int nz, const double* lo,
+                             const double* hi, int transmissive) {
   ora_grid* g = (ora_grid*)calloc(1, sizeof(ora_grid));
+  g->transmissive = transmissive;
   g->s = *s;
   g->nx = nx;
   g->ny = ny;
@@ -872,6 +879,21 @@ static void riemann_face(ora_grid* g, long cl, long cr, int d) {
   ora_riemann_solver_nonlinear(s, FL, FR, QL, QR, g->dt, d);
 }
 
+static void boundary_face(ora_grid* g, long c, int face) {
+  const ora_solver* s = &g->s;
+  const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
+  const long B2 = B * B;
+  double* FIn = g->lFhbnd + (c * 6 + face) * B2 * V;
+  const double* QIn = g->lQhbnd + (c * 6 + face) * B2 * D;
+  double FOut[10 * 10 * 16], QOut[10 * 10 * 16];
+  memcpy(FOut, FIn, sizeof(double) * B2 * V);
+  memcpy(QOut, QIn, sizeof(double) * B2 * D);
+  if (face % 2 == 0)
+    ora_riemann_solver_nonlinear(s, FOut, FIn, QOut, QIn, g->dt, face / 2);
+  else
+    ora_riemann_solver_nonlinear(s, FIn, FOut, QIn, QOut, g->dt, face / 2);
+}
+
 static void update_cells(ora_grid* g, long c0, long c1) {
   const ora_solver* s = &g->s;
   const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
@@ -897,17 +919,29 @@ void ora_grid_step(ora_grid* g, int nsteps, int nthreads, double* dt_out, int* i
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < g->ncells; c++) {
       const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
-      riemann_face(g, c, cell_index(g, i + 1, j, k), 0);
+      if (!g->transmissive || i + 1 < g->nx) riemann_face(g, c, cell_index(g, i + 1, j, k), 0);
     }
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < g->ncells; c++) {
       const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
-      riemann_face(g, c, cell_index(g, i, j + 1, k), 1);
+      if (!g->transmissive || j + 1 < g->ny) riemann_face(g, c, cell_index(g, i, j + 1, k), 1);
     }
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < g->ncells; c++) {
       const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
-      riemann_face(g, c, cell_index(g, i, j, k + 1), 2);
+      if (!g->transmissive || k + 1 < g->nz) riemann_face(g, c, cell_index(g, i, j, k + 1), 2);
+    }
+    /* boundary faces (ADERDGSolver::applyBoundaryConditions, ADERDGSolver.cpp:2893-2966):
+     * boundaryValues returns the inside state and flux (transmissive), then
+     * the Riemann solve with the outside state left of even faces */
+    if (g->transmissive) {
+#pragma omp parallel for schedule(static)
+      for (long c = 0; c < g->ncells; c++) {
+        const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
+        const int at[6] = {i == 0, i == g->nx - 1, j == 0, j == g->ny - 1, k == 0, k == g->nz - 1};
+        for (int f = 0; f < 6; f++)
+          if (at[f]) boundary_face(g, c, f);
+      }
     }
     /* SolutionUpdate, then the new global time step size. */
     update_cells(g, 0, g->ncells);

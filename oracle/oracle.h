@@ -73,6 +73,10 @@ typedef struct ora_grid ora_grid;
 /* nx*ny*nz cells on [lo,hi], cell c = (k*ny + j)*nx + i. */
 ora_grid* ora_grid_create(const ora_solver* s, int nx, int ny, int nz, const double* lo,
                           const double* hi);
+/* transmissive = 1: boundary faces get boundaryValues = inside state and the
+ * reference's boundary Riemann solve instead of the periodic wrap. */
+ora_grid* ora_grid_create_bc(const ora_solver* s, int nx, int ny, int nz, const double* lo,
+                             const double* hi, int transmissive);
 void ora_grid_destroy(ora_grid* g);
 long ora_grid_num_cells(const ora_grid* g);
 void ora_grid_set_solution(ora_grid* g, const double* luh);

@@ -55,6 +55,8 @@ def lib():
         L.ora_table.restype = ctypes.c_double
         L.ora_grid_create.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]
         L.ora_grid_create.restype = ctypes.c_void_p
+        L.ora_grid_create_bc.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int]
+        L.ora_grid_create_bc.restype = ctypes.c_void_p
         L.ora_grid_destroy.argtypes = [ctypes.c_void_p]
         L.ora_grid_num_cells.argtypes = [ctypes.c_void_p]
         L.ora_grid_num_cells.restype = ctypes.c_long
@@ -189,12 +191,13 @@ def flux(s, Q):
 class Grid:
     """Uniform periodic grid; the non-fused reference time-stepping driver."""
 
-    def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), transmissive=False):
         if isinstance(n, int):
             n = (n, n, n)
         self.s = s
         self.n = tuple(n)
-        self._h = lib().ora_grid_create(ctypes.byref(s), n[0], n[1], n[2], _p(dx3(lo)), _p(dx3(hi)))
+        self._h = lib().ora_grid_create_bc(ctypes.byref(s), n[0], n[1], n[2], _p(dx3(lo)), _p(dx3(hi)),
+                                           int(transmissive))
         self.ncells = lib().ora_grid_num_cells(self._h)
 
     def __del__(self):

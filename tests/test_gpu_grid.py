@@ -160,3 +160,26 @@ def test_precision_workload_full_size_81cube_n5(exa, oracle):
     tot2 = (w3 * g.solution().reshape(-1, B, B, B, 5)).sum(axis=(0, 1, 2, 3))
     assert np.abs(tot2 - tot0).max() < 1e-11 * np.abs(tot0).max()
     g.close()
+
+
+@pytest.mark.parametrize("order,n,steps", [(3, 6, 8), (5, 4, 5)])
+def test_transmissive_grid_vs_oracle(exa, oracle, order, n, steps):
+    """Transmissive boundaries (boundaryConditions with outside = inside state):
+    device grid vs the oracle driver, which solves the boundary Riemann problems."""
+    print(f"case: transmissive grid vs oracle, N={order} {n}^3 {steps} steps", flush=True)
+    g = exa.Grid(exa.euler_solver(order), n, boundary=exa.BC_TRANSMISSIVE)
+    g.init_euler(amp=0.2, nmodes=8, mode_amp=0.01, seed=5)
+    luh0 = g.solution()
+    g.step(steps)
+    g.synchronize()
+    gpu = g.solution()
+    o = oracle.Grid(oracle.make_solver(order, 5, 0), n, transmissive=True)
+    o.set_solution(luh0)
+    odts, oit = o.step(steps, nthreads=0, record_iters=True)
+    dts = g.dt_history(steps)
+    assert np.abs(dts - odts).max() <= 1e-13 * odts.max()
+    assert np.array_equal(g.picard_iterations(), o.picard_iterations())
+    cpu = o.solution()
+    assert rel_per_dof(gpu.reshape(-1, 5)[:, [0, 4]], cpu.reshape(-1, 5)[:, [0, 4]]) <= TOL
+    assert rel_max(gpu, cpu) <= TOL
+    g.close()

@@ -13,7 +13,7 @@
 import numpy as np
 import pytest
 
-from refutil import (GAMMA, cell_weights, euler_flux, euler_state, gauss_legendre, golden, lagrange,
+from refutil import (GAMMA, random_euler_cells, cell_weights, euler_flux, euler_state, gauss_legendre, golden, lagrange,
                      node_coords)
 
 G = golden()
@@ -174,6 +174,33 @@ def test_riemann_equal_states(oracle):
         F = euler_flux(Q, d).reshape(-1)
         FL, FR = oracle.riemann(s, F, F, Q.reshape(-1), Q.reshape(-1), 0.0, d)
         assert np.abs(FL - F).max() < 1e-14 and np.abs(FR - F).max() < 1e-14
+
+
+def test_transmissive_boundary_riemann_is_exact_identity(oracle):
+    """boundaryConditions with transmissive boundaryValues (outside = inside,
+    boundaryConditions.cpph:24-85) followed by the Riemann solve of
+    ADERDGSolver.cpp:2952-2960 returns the inner flux bit for bit: Rusanov of
+    equal states is 0.5 (F + F) - 0.5 s (Q - Q) = F exactly."""
+    s = oracle.make_solver(3, 5, 0)
+    B = 4
+    rng = np.random.default_rng(2)
+    for d in range(3):
+        Q = random_euler_cells(rng, 1, 3)[: B * B * 5]
+        F = rng.normal(size=B * B * 5)
+        FL, FR = oracle.riemann(s, F, F, Q, Q, 0.0, d)
+        assert np.array_equal(FL, F) and np.array_equal(FR, F)
+
+
+def test_transmissive_grid_keeps_a_uniform_state(oracle):
+    order, n = 3, 3
+    s = oracle.make_solver(order, 5, 0)
+    g = oracle.Grid(s, n, transmissive=True)
+    luh = np.tile(euler_state(np.array([1.3]), (0.3, -0.2, 0.1), 2.0).ravel(), n ** 3 * (order + 1) ** 3)
+    g.set_solution(luh)
+    g.step(3)
+    # table rounding (sum_n Kxi[n][l] ~ 1e-13, 13-digit literals) moves it by
+    # ~1e-12; a wrong boundary flux would move it by O(dt/dx)
+    assert np.abs(g.solution() - luh).max() < 1e-10 * np.abs(luh).max()
 
 
 def entropy_wave_cell(order, u, centre, h, dt):

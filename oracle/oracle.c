@@ -93,25 +93,76 @@ ORA_INLINE void euler_eigenvalues(const double* Q, int d, double* lambda) {
   lambda[4] = u_n + c;
 }
 
+/* Linear elasticity, velocity-stress form, 9 variables (sxx, syy, szz, sxy,
+ * syz, sxz, u, v, w) and 3 parameters (lambda, mu, rho) in the layout of the
+ * reference's elasticity test solver (ExaHyPE/exahype/tests/kernels/c/
+ * ElasticityKernelTest2D.cpp:101-182): eigenvalues -cp,-cs,-cs,0,0,0,cs,cs,cp
+ * (:101-129) and the non-conservative product B(Q) grad Q, whose x- and
+ * y-blocks are :150-170 and whose z-block is the one written out there as a
+ * comment (:172-180).  The nonlinear kernels take the sum over directions. */
+ORA_INLINE void elastic_eigenvalues(const double* Q, double* lambda) {
+  const double lam = Q[9], mu = Q[10], rho0 = Q[11];
+  const double cp = sqrt((lam + 2 * mu) / rho0);
+  const double cs = sqrt(mu / rho0);
+  lambda[0] = -cp;
+  lambda[1] = -cs;
+  lambda[2] = -cs;
+  lambda[3] = 0.0;
+  lambda[4] = 0.0;
+  lambda[5] = 0.0;
+  lambda[6] = +cs;
+  lambda[7] = +cs;
+  lambda[8] = +cp;
+}
+
+ORA_INLINE void elastic_ncp(const double* Q, const double* gradQ, double* BgradQ) {
+  const int nv = 9;
+  const double lam = Q[nv], mu = Q[nv + 1], irho = 1.0 / Q[nv + 2];
+  const double* gx = gradQ;
+  const double* gy = gradQ + nv;
+  const double* gz = gradQ + 2 * nv;
+  BgradQ[0] = (-(lam + 2 * mu) * gx[6] + -lam * gy[7]) + -lam * gz[8];
+  BgradQ[1] = (-lam * gx[6] + -(lam + 2 * mu) * gy[7]) + -lam * gz[8];
+  BgradQ[2] = (-lam * gx[6] + -lam * gy[7]) + -(lam + 2 * mu) * gz[8];
+  BgradQ[3] = (-mu * gx[7] + -mu * gy[6]) + 0.0;
+  BgradQ[4] = (0.0 + -mu * gy[8]) + -mu * gz[7];
+  BgradQ[5] = (-mu * gx[8] + 0.0) + -mu * gz[6];
+  BgradQ[6] = (-irho * gx[0] + -irho * gy[3]) + -irho * gz[5];
+  BgradQ[7] = (-irho * gx[3] + -irho * gy[1]) + -irho * gz[4];
+  BgradQ[8] = (-irho * gx[5] + -irho * gy[4]) + -irho * gz[2];
+}
+
 void ora_flux(const ora_solver* s, const double* Q, double** F) {
-  (void)s;
+  if (s->pde == ORA_PDE_ELASTIC) {
+    for (int d = 0; d < 3; d++)
+      for (int m = 0; m < s->nvar; m++) F[d][m] = 0.0;  /* NCP form: no flux */
+    return;
+  }
   euler_flux(Q, F);
 }
 
 void ora_eigenvalues(const ora_solver* s, const double* Q, int d, double* lambda) {
-  (void)s;
+  if (s->pde == ORA_PDE_ELASTIC) {
+    elastic_eigenvalues(Q, lambda);
+    return;
+  }
   euler_eigenvalues(Q, d, lambda);
 }
 
-/* GenericEulerKernelTest3D.cpp:59-66: the Euler test solver's source is zero. */
+/* GenericEulerKernelTest3D.cpp:59-66 and the elasticity test solver
+ * (ElasticityKernelTest2D.cpp:95-99): zero sources. */
 void ora_algebraic_source(const ora_solver* s, const double* Q, double* S) {
   (void)Q;
   for (int m = 0; m < s->nvar; m++) S[m] = 0.0;
 }
 
-/* GenericEulerKernelTest3D.cpp:84-107 with _setNcpAndMatrixBToZero set, the
- * state the predictor test runs it in: BgradQ = 0. */
+/* Euler: GenericEulerKernelTest3D.cpp:84-107 with _setNcpAndMatrixBToZero
+ * set, the state the predictor test runs it in (BgradQ = 0).  Elastic: above. */
 void ora_ncp(const ora_solver* s, const double* Q, const double* gradQ, double* BgradQ) {
+  if (s->pde == ORA_PDE_ELASTIC) {
+    elastic_ncp(Q, gradQ, BgradQ);
+    return;
+  }
   (void)Q;
   (void)gradQ;
   for (int m = 0; m < s->nvar; m++) BgradQ[m] = 0.0;
@@ -156,7 +207,7 @@ ORA_INLINE int picard_loop(const ora_solver* s, const int order, const int V, co
               F[0] = lFi + I_LFI(i, j, k, l, 0, 0);
               F[1] = lFi + I_LFI(i, j, k, l, 1, 0);
               F[2] = lFi + I_LFI(i, j, k, l, 2, 0);
-              euler_flux(lQi + I_LQI(j, k, l, i, 0), F);
+              ora_flux(s, lQi + I_LQI(j, k, l, i, 0), F);
             }
       }
       /* rhs0 (:124-141) */
@@ -605,8 +656,8 @@ ORA_INLINE void riemann_body(const ora_solver* s, const int order, const int V, 
       }
     }
   double LL[64] = {0.0}, LR[64] = {0.0};
-  euler_eigenvalues(QavL, direction, LL);
-  euler_eigenvalues(QavR, direction, LR);
+  ora_eigenvalues(s, QavL, direction, LL);
+  ora_eigenvalues(s, QavR, direction, LR);
   for (int k = 0; k < V; k++) {
     LL[k] = fabs(LL[k]);
     LR[k] = fabs(LR[k]);
@@ -696,7 +747,7 @@ ORA_INLINE double dt_body(const ora_solver* s, const int order, const int V, con
       for (int k = 0; k < B; k++) {
         double denominator = 0.0;
         for (int d = 0; d < 3; d++) {
-          euler_eigenvalues(luh + (((i * B) + j) * B + k) * D, d, lambda);
+          ora_eigenvalues(s, luh + (((i * B) + j) * B + k) * D, d, lambda);
           double maxEigenvalue = 0.0;
           for (int ivar = 0; ivar < V; ivar++) {
             const double a = fabs(lambda[ivar]);

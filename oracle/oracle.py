@@ -51,6 +51,7 @@ def lib():
         L.ora_stable_time_step_size.restype = ctypes.c_double
         L.ora_flux.argtypes = [S, P, ctypes.POINTER(P)]
         L.ora_eigenvalues.argtypes = [S, P, ctypes.c_int, P]
+        L.ora_ncp.argtypes = [S, P, P, P]
         L.ora_table.argtypes = [ctypes.c_int] * 4
         L.ora_table.restype = ctypes.c_double
         L.ora_grid_create.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]
@@ -115,6 +116,24 @@ def _p(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def elastic_solver(order, cfl=0.9):
+    """Linear elasticity in non-conservative form: 9 variables, 3 parameters, NCP only."""
+    return make_solver(order, 9, 3, pde=PDE_ELASTIC, use_flux=False, use_source=False, use_ncp=True, cfl=cfl)
+
+
+def ncp(s, Q, gradQ):
+    out = np.zeros(s.nvar)
+    lib().ora_ncp(ctypes.byref(s), _p(np.ascontiguousarray(Q, dtype=np.float64)),
+                  _p(np.ascontiguousarray(gradQ, dtype=np.float64).ravel()), _p(out))
+    return out
+
+
+def eigenvalues(s, Q, d):
+    out = np.zeros(s.nvar)
+    lib().ora_eigenvalues(ctypes.byref(s), _p(np.ascontiguousarray(Q, dtype=np.float64)), d, _p(out))
+    return out
 
 
 def make_solver(order, nvar=5, npar=0, pde=PDE_EULER, use_flux=True, use_source=False,

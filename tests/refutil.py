@@ -172,3 +172,54 @@ def blast_cells(order, n, seed=7, nodes=None, radius=0.2, p_in=10.0, p_out=0.1, 
     p = np.where(r < radius, p_in, p_out)
     Q = np.stack([rho, 0 * rho, 0 * rho, 0 * rho, p / (GAMMA - 1.0)], -1)
     return np.ascontiguousarray(Q.reshape(-1))
+
+
+def elastic_material(X, seed=3):
+    """Seeded piecewise-smooth material at points X[..., 3] (x, y, z): two
+    layers split by z0(x, y) = 0.5 + 0.1 sin(2 pi x + a) sin(2 pi y + b); per
+    layer rho in [2.05, 2.95], cp in [4.1, 5.9], each modulated smoothly by
+    +-0.05 / +-0.1; cs = cp / sqrt(3).  Returns (lambda, mu, rho)."""
+    u = splitmix64_uniform(np.arange(6, dtype=np.uint64), seed)
+    a, b = 2 * np.pi * u[0], 2 * np.pi * u[1]
+    x, y, z = X[..., 0], X[..., 1], X[..., 2]
+    z0 = 0.5 + 0.1 * np.sin(2 * np.pi * x + a) * np.sin(2 * np.pi * y + b)
+    upper = z > z0
+    rho = np.where(upper, 2.05 + 0.9 * u[2], 2.05 + 0.9 * u[3]) + 0.05 * np.sin(2 * np.pi * ((x + y) + z))
+    cp = np.where(upper, 4.1 + 1.8 * u[4], 4.1 + 1.8 * u[5]) + 0.1 * np.sin(2 * np.pi * ((x - y) + z))
+    cs = cp / np.sqrt(3.0)
+    mu = rho * cs * cs
+    lam = rho * cp * cp - 2.0 * mu
+    return lam, mu, rho
+
+
+def elastic_cells(order, n, seed=3, nodes=None, width=0.05, amp=1.0):
+    """Elastic workload state at the GL nodes: zero stresses, a Gaussian
+    velocity pulse u = v = w = amp exp(-|x - c|^2 / (2 width^2)) at the cube
+    centre, parameters (lambda, mu, rho) of elastic_material.  [ncells*B^3*12]."""
+    X = node_coords(order, n, nodes=nodes)
+    lam, mu, rho = elastic_material(X, seed)
+    r2 = (((X[..., 0] - 0.5) ** 2 + (X[..., 1] - 0.5) ** 2) + (X[..., 2] - 0.5) ** 2)
+    g = amp * np.exp(-r2 / (2.0 * width * width))
+    z = np.zeros_like(g)
+    Q = np.stack([z, z, z, z, z, z, g, g, g, lam, mu, rho], -1)
+    return np.ascontiguousarray(Q.reshape(-1))
+
+
+def elastic_matrices(lam, mu, rho):
+    """Coefficient matrices B_x, B_y, B_z of dQ/dt + sum_d B_d dQ/dx_d = 0 for the
+    9 elastic variables (sxx, syy, szz, sxy, syz, sxz, u, v, w), written from the
+    velocity-stress equations directly."""
+    Bx, By, Bz = np.zeros((9, 9)), np.zeros((9, 9)), np.zeros((9, 9))
+    ir = 1.0 / rho
+    # stresses from velocity gradients
+    Bx[0, 6], By[0, 7], Bz[0, 8] = -(lam + 2 * mu), -lam, -lam
+    Bx[1, 6], By[1, 7], Bz[1, 8] = -lam, -(lam + 2 * mu), -lam
+    Bx[2, 6], By[2, 7], Bz[2, 8] = -lam, -lam, -(lam + 2 * mu)
+    Bx[3, 7], By[3, 6] = -mu, -mu          # sxy: dv/dx + du/dy
+    By[4, 8], Bz[4, 7] = -mu, -mu          # syz: dw/dy + dv/dz
+    Bx[5, 8], Bz[5, 6] = -mu, -mu          # sxz: dw/dx + du/dz
+    # velocities from stress divergence
+    Bx[6, 0], By[6, 3], Bz[6, 5] = -ir, -ir, -ir
+    Bx[7, 3], By[7, 1], Bz[7, 4] = -ir, -ir, -ir
+    Bx[8, 5], By[8, 4], Bz[8, 2] = -ir, -ir, -ir
+    return Bx, By, Bz

@@ -33,7 +33,7 @@ EXPORTED = (
     "exa_grid_download_solution", "exa_grid_init_euler", "exa_grid_step", "exa_grid_synchronize",
     "exa_grid_time", "exa_grid_dt_history", "exa_grid_picard_iterations",
     "exa_grid_set_profiling", "exa_grid_stage_times", "exa_grid_download_cells", "exa_grid_picard_totals",
-    "exa_project_on_fv_limiter_space", "exa_project_on_dg_space", "exa_find_cell_local_min_and_max",
+    "exa_grid_init_elastic", "exa_project_on_fv_limiter_space", "exa_project_on_dg_space", "exa_find_cell_local_min_and_max",
     "exa_discrete_maximum_principle_and_min_and_max_search", "exa_musclhancock_solution_update",
     "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
@@ -70,6 +70,11 @@ def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9, precision=PRECISION_FP
     return Solver(PDE_EULER, order, 5, npar, terms, 0, cfl, precision)
 
 
+def elastic_solver(order, cfl=0.9, precision=PRECISION_FP64):
+    """Linear elasticity: 9 variables + (lambda, mu, rho), non-conservative product only."""
+    return Solver(PDE_ELASTIC, order, 9, 3, TERM_NCP, 0, cfl, precision)
+
+
 _lib = None
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -104,6 +109,7 @@ def lib():
     L.exa_grid_upload_solution.argtypes = [V, _D]
     L.exa_grid_download_solution.argtypes = [V, _D]
     L.exa_grid_init_euler.argtypes = [V, ctypes.POINTER(EulerIC)]
+    L.exa_grid_init_elastic.argtypes = [V, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double]
     L.exa_grid_step.argtypes = [V, ctypes.c_int]
     L.exa_grid_synchronize.argtypes = [V]
     L.exa_grid_time.argtypes = [V, _D, _D, ctypes.POINTER(ctypes.c_long)]
@@ -264,6 +270,9 @@ class Grid:
         ic = EulerIC(rho0, amp, (ctypes.c_double * 3)(*k), (ctypes.c_double * 3)(*vel), p, nmodes, mode_amp,
                      seed)
         check(lib().exa_grid_init_euler(self._h, ctypes.byref(ic)), "exa_grid_init_euler")
+
+    def init_elastic(self, seed=3, width=0.05, amp=1.0):
+        check(lib().exa_grid_init_elastic(self._h, seed, width, amp), "exa_grid_init_elastic")
 
     def upload(self, luh):
         check(lib().exa_grid_upload_solution(self._h, _dp(_f64(luh))), "upload")

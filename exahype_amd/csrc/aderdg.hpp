@@ -129,6 +129,22 @@ struct ICArgs {
   double mode_phase[kMaxModes][4];
 };
 
+// Elastic workload state: zero stresses, Gaussian velocity pulse u = v = w =
+// amp exp(-|x - c|^2 / (2 width^2)) at the centre c of [0,1]^3, and a seeded
+// two-layer material (uniform draws r[6] from the seed): interface
+// z0 = 0.5 + 0.1 sin(2 pi x + 2 pi r0) sin(2 pi y + 2 pi r1); rho = 2.05 + 0.9 r2|r3
+// + 0.05 sin(2 pi (x+y+z)); cp = 4.1 + 1.8 r4|r5 + 0.1 sin(2 pi (x-y+z)),
+// cs = cp / sqrt 3; parameters (lambda, mu, rho).
+struct ElasticICArgs {
+  double* luh;
+  long ncells;
+  int nx, ny, nz, order;
+  double lo[3], dx[3];
+  double r[6];
+  double width, amp;
+};
+hipError_t launch_elastic_ic(const ElasticICArgs& a, hipStream_t s);
+
 bool supported(const SolverDesc& sd);
 // Kernels that only depend on (order, nvar): volume and surface integrals.
 bool supported_sizes(const SolverDesc& sd);

@@ -18,6 +18,7 @@
 #include "exa_device.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace exa {
 
@@ -91,8 +92,9 @@ struct PredictorLayout {
   static constexpr long kFree = kLdsBudget - kOther;
   static constexpr int NQL = kFree <= 0 ? 0 : (kFree / kSliceBytes >= B ? B : (int)(kFree / kSliceBytes));
   static constexpr int NQR = B - NQL;
-  // Rolled slices: the large sizes, whose unrolled slices exhaust the registers.
-  static constexpr bool ROLLED = B >= 7 && (IS_FP64 || B >= 8);
+  // Rolled slices: the large sizes and the 9-variable PDE, whose unrolled
+  // slices exhaust the registers.
+  static constexpr bool ROLLED = (B >= 7 && (IS_FP64 || B >= 8)) || V >= 9;
   // 8+ waves per workgroup leave 256 registers per lane, no room for slices;
   // a rolled loop cannot index registers.
   static constexpr bool WSQ = NQR > 0 && (NT >= 8 * kWave || ROLLED);
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     fh[2][m] = T(0);
     sh[m] = T(0);
   }
-#pragma unroll
+#pragma unroll UNROLL_T
   for (int t = 0; t < B; ++t) {
     T Q[D];
 #pragma unroll
@@ -569,16 +571,17 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
   if (USE_NCP) {
     T gradQ[3][V];
     T Qavg[D];
+    // only the normal row is non-zero; rows selected without a run-time
+    // index into the private array (which would place it in scratch memory)
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-      gradQ[0][k] = T(0);
-      gradQ[1][k] = T(0);
-      gradQ[2][k] = T(0);
+      const T jump = qR[k] - qL[k];
+      gradQ[0][k] = d == 0 ? jump : T(0);
+      gradQ[1][k] = d == 1 ? jump : T(0);
+      gradQ[2][k] = d == 2 ? jump : T(0);
     }
 #pragma unroll
     for (int k = 0; k < D; ++k) Qavg[k] = T(0.5) * (qR[k] + qL[k]);
-#pragma unroll
-    for (int k = 0; k < V; ++k) gradQ[d][k] = qR[k] - qL[k];
     PDE::ncp(Qavg, gradQ, ncp);
   }
   if (!active) return;
@@ -783,6 +786,44 @@ __global__ void euler_ic_kernel(ICArgs a) {
   for (int m = 5; m < D; ++m) Q[m] = 0.0;
 }
 
+__global__ __launch_bounds__(256) void elastic_ic_kernel(ElasticICArgs a) {
+  const int B = a.order + 1;
+  const int B3 = B * B * B;
+  const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (size_t)a.ncells * B3) return;
+  const size_t cell = gid / B3;
+  const int node = (int)(gid % B3);
+  const int iz = node / (B * B), iy = (node / B) % B, ix = node % B;
+  const int ci = (int)(cell % a.nx), cj = (int)((cell / a.nx) % a.ny), ck = (int)(cell / ((size_t)a.nx * a.ny));
+  const double x = (a.lo[0] + (ci + 0.5) * a.dx[0]) + a.dx[0] * (exa_gl_nodes[a.order][ix] - 0.5);
+  const double y = (a.lo[1] + (cj + 0.5) * a.dx[1]) + a.dx[1] * (exa_gl_nodes[a.order][iy] - 0.5);
+  const double z = (a.lo[2] + (ck + 0.5) * a.dx[2]) + a.dx[2] * (exa_gl_nodes[a.order][iz] - 0.5);
+  const double twopi = 6.283185307179586476925286766559;
+  const double z0 = 0.5 + 0.1 * sin(twopi * x + twopi * a.r[0]) * sin(twopi * y + twopi * a.r[1]);
+  const bool upper = z > z0;
+  const double rho = (upper ? 2.05 + 0.9 * a.r[2] : 2.05 + 0.9 * a.r[3]) + 0.05 * sin(twopi * ((x + y) + z));
+  const double cp = (upper ? 4.1 + 1.8 * a.r[4] : 4.1 + 1.8 * a.r[5]) + 0.1 * sin(twopi * ((x - y) + z));
+  const double cs = cp / sqrt(3.0);
+  const double mu = rho * cs * cs;
+  const double lam = rho * cp * cp - 2.0 * mu;
+  const double r2 = ((x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5)) + (z - 0.5) * (z - 0.5);
+  const double g = a.amp * exp(-r2 / (2.0 * a.width * a.width));
+  double* Q = a.luh + gid * 12;
+  for (int m = 0; m < 6; ++m) Q[m] = 0.0;
+  Q[6] = g;
+  Q[7] = g;
+  Q[8] = g;
+  Q[9] = lam;
+  Q[10] = mu;
+  Q[11] = rho;
+}
+
+hipError_t launch_elastic_ic(const ElasticICArgs& a, hipStream_t s) {
+  const size_t total = (size_t)a.ncells * (a.order + 1) * (a.order + 1) * (a.order + 1);
+  hipLaunchKernelGGL(elastic_ic_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // Sum of the per-cell Picard counts of one predictor launch (for the roofline).
 __global__ __launch_bounds__(256) void count_iterations_kernel(const int* iters, long ncells, StepState* st) {
   __shared__ double sRed[256 / kWave];
@@ -810,6 +851,15 @@ static hipError_t dispatch_pde(const SolverDesc& sd, F&& f) {
 #undef EXA_CASE
   }
   if (sd.pde == kPdeEuler && sd.npar == 1 && sd.order == 3) return f.template operator()<4, Euler<1>>();
+  if (sd.pde == kPdeElastic && sd.nvar == Elastic::V && sd.npar == Elastic::P) {
+    switch (sd.order) {
+      case 2: return f.template operator()<3, Elastic>();
+      case 3: return f.template operator()<4, Elastic>();
+      case 4: return f.template operator()<5, Elastic>();
+      case 5: return f.template operator()<6, Elastic>();
+      default: break;
+    }
+  }
   return hipErrorNotSupported;
 }
 
@@ -826,7 +876,15 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
       PredictorArgs ac = a;
       ac.cell0 = a.cell0 + c0;
       const dim3 grid((unsigned)(ncells - c0 < chunk ? ncells - c0 : chunk)), block(node_threads(B));
-      if (sd.precision == kFP32Bulk) {
+      if constexpr (std::is_same_v<PDE, Elastic>) {  // non-conservative product only
+        if (sd.terms != kNCP) return hipErrorNotSupported;
+        if (sd.precision == kFP32Bulk)
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kNCP, false, float>), grid, block, 0, s, ac);
+        else if (export_temps)
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kNCP, true, double>), grid, block, 0, s, ac);
+        else
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kNCP, false, double>), grid, block, 0, s, ac);
+      } else if (sd.precision == kFP32Bulk) {
         if (sd.terms != kFlux) return hipErrorNotSupported;
         hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float>), grid, block, 0, s, ac);
       } else if (sd.terms == kFlux) {
@@ -855,7 +913,8 @@ size_t predictor_ws_bytes_per_cell(const SolverDesc& sd) {
   (void)dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
     constexpr int V = PDE::V, D = PDE::V + PDE::P;
     if (sd.precision == kFP32Bulk)
-      bytes = PredictorLayout<B, V, D, false, float>::ws_bytes;
+      bytes = (sd.terms & kNCP) ? PredictorLayout<B, V, D, true, float>::ws_bytes
+                                : PredictorLayout<B, V, D, false, float>::ws_bytes;
     else if (sd.terms & kNCP)
       bytes = PredictorLayout<B, V, D, true, double>::ws_bytes;
     else
@@ -870,7 +929,12 @@ hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_
   return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
     const dim3 grid((unsigned)((a.nfaces + 3) / 4)), block(256);
     const bool ncp = (sd.terms & kNCP) != 0;
-    if (sd.precision == kFP32Bulk) {
+    if constexpr (std::is_same_v<PDE, Elastic>) {
+      if (sd.precision == kFP32Bulk)
+        hipLaunchKernelGGL((riemann_kernel<B, PDE, true, float>), grid, block, 0, s, a);
+      else
+        hipLaunchKernelGGL((riemann_kernel<B, PDE, true, double>), grid, block, 0, s, a);
+    } else if (sd.precision == kFP32Bulk) {
       if (ncp) return hipErrorNotSupported;
       hipLaunchKernelGGL((riemann_kernel<B, PDE, false, float>), grid, block, 0, s, a);
     } else if (ncp) {
@@ -963,8 +1027,11 @@ bool supported_sizes(const SolverDesc& sd) {
 }
 
 bool supported(const SolverDesc& sd) {
-  if (sd.pde != kPdeEuler || sd.nvar != 5) return false;
   if (sd.no_time_averaging) return false;
+  if (sd.pde == kPdeElastic)
+    return sd.nvar == Elastic::V && sd.npar == Elastic::P && sd.terms == kNCP && sd.order >= 2 && sd.order <= 5 &&
+           (sd.precision == kFP64 || sd.precision == kFP32Bulk);
+  if (sd.pde != kPdeEuler || sd.nvar != 5) return false;
   if (sd.precision != kFP64 && sd.precision != kFP32Bulk) return false;
   if (sd.npar == 0) return sd.order >= 2 && sd.order <= 7 && sd.terms == kFlux;
   if (sd.npar == 1)

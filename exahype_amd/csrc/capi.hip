@@ -491,6 +491,30 @@ extern "C" int exa_grid_download_solution(exa_grid* g, double* luh) {
   return EXA_OK;
 }
 
+extern "C" int exa_grid_init_elastic(exa_grid* g, unsigned long long seed, double width, double amp) {
+  if (!g || !(width > 0)) return EXA_E_INVALID;
+  if (g->sd.pde != EXA_PDE_ELASTIC) return EXA_E_UNSUPPORTED;
+  exa::ElasticICArgs a{};
+  a.luh = g->luh;
+  a.ncells = g->ncells;
+  a.nx = g->nx;
+  a.ny = g->ny;
+  a.nz = g->nz;
+  a.order = g->sd.order;
+  for (int d = 0; d < 3; ++d) {
+    a.lo[d] = g->lo[d];
+    a.dx[d] = g->dx[d];
+  }
+  for (int c = 0; c < 6; ++c) {  // counter-based: state = seed + (c+1) * golden, one mixing round
+    uint64_t st = seed + (uint64_t)c * 0x9E3779B97F4A7C15ull;
+    a.r[c] = (double)(splitmix64(st) >> 11) * 0x1.0p-53;
+  }
+  a.width = width;
+  a.amp = amp;
+  EXA_TRY(exa::launch_elastic_ic(a, g->stream));
+  return reset_time(g);
+}
+
 extern "C" int exa_grid_init_euler(exa_grid* g, const exa_euler_ic* ic) {
   if (!g || !ic || ic->nmodes < 0 || ic->nmodes > exa::kMaxModes) return EXA_E_INVALID;
   if (g->sd.pde != EXA_PDE_EULER) return EXA_E_UNSUPPORTED;

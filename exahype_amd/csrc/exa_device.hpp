@@ -147,6 +147,52 @@ struct Euler {
   }
 };
 
+// Linear elasticity, velocity-stress form: variables (sxx, syy, szz, sxy,
+// syz, sxz, u, v, w), parameters (lambda, mu, rho), in the layout of the
+// reference's elasticity test solver (ElasticityKernelTest2D.cpp:101-182,
+// extended to 3-D by its z-block).  Pure non-conservative product: no flux,
+// no source; the largest |eigenvalue| in any direction is cp.
+struct Elastic {
+  static constexpr int V = 9;
+  static constexpr int P = 3;
+
+  template <typename T>
+  __device__ static inline void flux(const T*, T (*F)[V]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int m = 0; m < V; ++m) F[d][m] = T(0);
+  }
+
+  template <typename T>
+  __device__ static inline T max_abs_eigenvalue(const T* Q, int) {
+    const T lam = Q[9], mu = Q[10], rho0 = Q[11];
+    return sqrt((lam + T(2) * mu) / rho0);
+  }
+
+  template <typename T>
+  __device__ static inline void algebraic_source(const T*, T* S) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) S[m] = T(0);
+  }
+
+  // B(Q) grad Q summed over the three directions (x + y) + z.
+  template <typename T>
+  __device__ static inline void ncp(const T* Q, const T (*g)[V], T* BgradQ) {
+    const T lam = Q[9], mu = Q[10], irho = T(1) / Q[11];
+    const T l2m = lam + T(2) * mu;
+    BgradQ[0] = (-l2m * g[0][6] + -lam * g[1][7]) + -lam * g[2][8];
+    BgradQ[1] = (-lam * g[0][6] + -l2m * g[1][7]) + -lam * g[2][8];
+    BgradQ[2] = (-lam * g[0][6] + -lam * g[1][7]) + -l2m * g[2][8];
+    BgradQ[3] = -mu * g[0][7] + -mu * g[1][6];
+    BgradQ[4] = -mu * g[1][8] + -mu * g[2][7];
+    BgradQ[5] = -mu * g[0][8] + -mu * g[2][6];
+    BgradQ[6] = (-irho * g[0][0] + -irho * g[1][3]) + -irho * g[2][5];
+    BgradQ[7] = (-irho * g[0][3] + -irho * g[1][1]) + -irho * g[2][4];
+    BgradQ[8] = (-irho * g[0][5] + -irho * g[1][4]) + -irho * g[2][2];
+  }
+};
+
 // Stable time step of one node, stableTimeStepSize.cpph:37-57 (the caller takes
 // the minimum).  maxEigenvalue = max over variables of |lambda|.
 template <class PDE>

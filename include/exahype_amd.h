@@ -148,6 +148,11 @@ typedef struct exa_euler_ic {
   unsigned long long seed;
 } exa_euler_ic;
 int exa_grid_init_euler(exa_grid* g, const exa_euler_ic* ic);
+/* Elastic initial data (EXA_PDE_ELASTIC, 9 variables + lambda, mu, rho): zero
+ * stresses, a Gaussian velocity pulse u = v = w = amp exp(-|x-c|^2/(2 width^2))
+ * at the centre of [0,1]^3, and a seeded two-layer, smoothly modulated
+ * material with rho in [2,3], cp in [4,6], cs = cp/sqrt(3). */
+int exa_grid_init_elastic(exa_grid* g, unsigned long long seed, double width, double amp);
 
 /* Advance nsteps time steps (asynchronous on the grid's stream). */
 int exa_grid_step(exa_grid* g, int nsteps);

@@ -353,6 +353,7 @@ extern "C" long exa_lgrid_num_cells(const exa_lgrid* g) { return g ? g->ncells :
 
 static int lgrid_init_from_device(exa_lgrid* g, const double* d_lim_ic) {
   const exa::LimArgs a = largs(g);
+  EXA_TRY(hipMemsetAsync(g->flags, 0, sizeof(int) * 4, g->stream));  // new run: counters from zero
   EXA_TRY(exa::launch_lim_init(g->sd, a, d_lim_ic, g->stream));
   EXA_TRY(lstart(g));
   EXA_TRY(hipStreamSynchronize(g->stream));

@@ -226,8 +226,9 @@ __device__ inline void load_tables(double* sUh2lim, double* sUh2lob, double* sLi
 // ---------------------------------------------------------------------------
 // Step begin: flags cleared, pst = st, list of FV-active cells (st >= A).
 // ---------------------------------------------------------------------------
+// flags[0..2] are per step; flags[3] counts steps with an irregular change
 __global__ void lim_clear_flags_kernel(int* flags) {
-  if (threadIdx.x < 4) flags[threadIdx.x] = 0;
+  if (threadIdx.x < 3) flags[threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(256) void lim_begin_kernel(LimArgs a) {

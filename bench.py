@@ -174,6 +174,135 @@ def bench_limiter(capi, args):
     g.close()
 
 
+def cpu_baseline_batch(specs, threads, seconds):
+    """The oracle's grid time step per order on a 9^3 patch of that order,
+    weighted by the workload's DOF mix: the estimated CPU rate on the same
+    patch set (rate per DOF-update, an intensive figure)."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refutil import euler_state, node_coords
+    O.lib()
+    per_dof_s = {}
+    budget = seconds / 6.0
+    for order in sorted({o for _, o, _, _ in specs}):
+        m = 9
+        g = O.Grid(O.make_solver(order, 5, 0), m)
+        X = node_coords(order, m)
+        g.set_solution(euler_state(1.0 + 0.2 * np.sin(2 * np.pi * X.sum(-1)), (1.0, 1.0, 1.0), 1.0).reshape(-1))
+        steps, t0 = 0, time.perf_counter()
+        while steps < 1 or (time.perf_counter() - t0 < budget and steps < 50):
+            g.step(1, nthreads=threads)
+            steps += 1
+        per_dof_s[order] = (time.perf_counter() - t0) / (steps * m ** 3 * (order + 1) ** 3)
+    dof = [n ** 3 * (o + 1) ** 3 for n, o, _, _ in specs]
+    est = sum(d * per_dof_s[o] for d, (_, o, _, _) in zip(dof, specs))
+    return dict(value=sum(dof) / est, unit="DOF-updates/s", cores=threads, kind="port",
+                sample="oracle grid steps of one 9^3 patch per order (entropy wave, periodic), OpenMP "
+                       f"{threads} threads, about {budget:.1f} s per order; per-order seconds per DOF-update "
+                       "weighted by the patch set's DOF mix")
+
+
+def bench_batch(capi, args):
+    """Patch-set workload: 96 seeded periodic Euler patches, sides in
+    {9,15,27,45,81}, orders 2..7, entropy waves, per-patch dt; FP32-bulk vs FP64."""
+    specs = capi.draw_patches(args.seed)
+    keep = 27 ** 3
+    res = {}
+    for form in ("fp64", "fp32"):
+        prec = capi.PRECISION_FP64 if form == "fp64" else capi.PRECISION_FP32_BULK
+        t0 = time.perf_counter()
+        ps = capi.PatchSet(specs, precision=prec, hbm_budget_gb=args.hbm_budget, keep_max_cells=keep)
+        ps.run(args.steps, use_graph=True, autotune=True, warmup=args.warmup)
+        wall = time.perf_counter() - t0
+        sec, dof, nwaves = ps.stats()
+        times = [ps.time(j) for j in range(len(specs))]
+        if not all(np.isfinite(dt) and dt > 0 for _, dt in times):
+            raise RuntimeError(f"{form}: non-finite time step in the patch set")
+        sols = {j: ps.solution(j) for j, sp in enumerate(specs) if sp[0] ** 3 <= keep}
+        variants = {o: ps.lds_variant(o) for o in range(2, 8)}
+        ps.close()
+        res[form] = dict(sec=sec, dof=dof, nwaves=nwaves, wall=wall, sols=sols, variants=variants, times=times)
+        print(f"batch {form}: {nwaves} wave(s), {dof / sec:.3e} DOF/s, {wall:.1f} s with setup", file=sys.stderr,
+              flush=True)
+    num = sum(float(np.sum((res["fp32"]["sols"][j] - res["fp64"]["sols"][j]) ** 2)) for j in res["fp64"]["sols"])
+    den = sum(float(np.sum(res["fp64"]["sols"][j] ** 2)) for j in res["fp64"]["sols"])
+    m, m64 = res["fp32"], res["fp64"]
+    cells = sum(n ** 3 for n, *_ in specs)
+    dof_step = sum(n ** 3 * (o + 1) ** 3 for n, o, _, _ in specs)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    out = {
+        "metric": "DOF-updates/s, 96 uneven 3D Euler patches (orders 2..7), per-patch dt, FP32-bulk vs FP64",
+        "value": m["dof"] / m["sec"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": m["sec"] / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic (96 seeded patches, entropy waves generated on the device)",
+        "config": {"workload": f"euler3d-patchset 96 patches, sides {{9,15,27,45,81}}, orders 2..7, "
+                               f"{cells} cells, {dof_step} DOF per step, periodic CFL0.9",
+                   "seed": args.seed, "hbm_budget_gb": args.hbm_budget, "waves": m["nwaves"],
+                   "launches": "one per stage and order group, captured once per wave in a hipGraph",
+                   "lds_variant_by_order": m["variants"]},
+        "wall_s_including_setup": round(m["wall"], 3),
+        "fp64": {"value": m64["dof"] / m64["sec"], "ms_per_step": m64["sec"] / args.steps * 1e3,
+                 "waves": m64["nwaves"], "lds_variant_by_order": m64["variants"],
+                 "wall_s_including_setup": round(m64["wall"], 3)},
+        "fp32_vs_fp64": {"speedup": m64["sec"] / m["sec"],
+                         "l2_rel_err": (num / den) ** 0.5 if den > 0 else None,
+                         "l2_over": f"the {len(m64['sols'])} patches with at most {keep} cells",
+                         "max_rel_time_diff": max(abs(a[0] - b[0]) / b[0] for a, b in zip(m["times"], m64["times"]))},
+        "roofline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_batch(specs, threads, args.cpu_seconds),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_elastic(capi, args):
+    """Elastic workload: velocity-stress NCP form, N=4, 81^3, Gaussian pulse
+    of width 0.05 in a seeded two-layer material; FP32-bulk vs FP64."""
+    order, n = 4, args.cells
+    B = order + 1
+    res = {}
+    for form in ("fp64", "fp32"):
+        prec = capi.PRECISION_FP64 if form == "fp64" else capi.PRECISION_FP32_BULK
+        g = capi.Grid(capi.elastic_solver(order, precision=prec), n)
+        g.init_elastic(seed=args.seed, width=0.05)
+        g.step(args.warmup)
+        g.synchronize()
+        g.set_profiling(True)
+        g.stage_times()
+        t0 = time.perf_counter()
+        g.step(args.steps)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        stages = g.stage_times()
+        t_now, dt_now, _ = g.time()
+        if not np.isfinite(dt_now) or dt_now <= 0:
+            raise RuntimeError(f"{form}: non-finite time step")
+        res[form] = dict(wall=wall, stages=stages, t=t_now, sol=g.solution().reshape(-1, 12)[:, :9])
+        print(f"elastic {form}: {wall / max(args.steps, 1) * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+        g.close()
+    dof = n ** 3 * B ** 3
+    a, b = res["fp32"]["sol"], res["fp64"]["sol"]
+    l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    m, m64 = res["fp32"], res["fp64"]
+    out = {
+        "metric": "DOF-updates/s, 3D linear elasticity (NCP form) N=4 81^3: FP32-bulk vs FP64",
+        "value": dof * args.steps / m["wall"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": m["wall"] / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic (Gaussian velocity pulse, seeded two-layer material, generated on the device)",
+        "config": {"workload": f"elastic3d {n}^3 cells N={order} periodic CFL0.9, 9 variables + 3 parameters",
+                   "seed": args.seed, "dof_per_step": dof},
+        "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in m["stages"].items()},
+        "fp64": {"value": dof * args.steps / m64["wall"], "ms_per_step": m64["wall"] / args.steps * 1e3,
+                 "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in m64["stages"].items()}},
+        "fp32_vs_fp64": {"speedup": m64["wall"] / m["wall"], "l2_rel_err": l2,
+                         "steps_compared": args.warmup + args.steps},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,9 +315,12 @@ def main():
                     help="both: time FP64 and FP32-bulk and report the error between them")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("precision", "limiter"), default="precision",
+    ap.add_argument("--workload", choices=("precision", "limiter", "batch", "elastic"), default="precision",
                     help="precision: Euler N=5 81^3, FP32-bulk vs FP64 (the headline line); "
-                         "limiter: Euler blast N=3 81^3 with the subcell limiter, FP64")
+                         "limiter: Euler blast N=3 81^3 with the subcell limiter, FP64; "
+                         "batch: 96 seeded uneven patches (orders 2..7), FP32-bulk vs FP64; "
+                         "elastic: linear elasticity N=4 81^3, FP32-bulk vs FP64")
+    ap.add_argument("--hbm-budget", type=float, default=200.0, help="batch: GB of patch state per wave")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times): the known-size "
                          "reference for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -206,6 +338,12 @@ def main():
     capi.lib()
     if args.workload == "limiter":
         bench_limiter(capi, args)
+        return
+    if args.workload == "batch":
+        bench_batch(capi, args)
+        return
+    if args.workload == "elastic":
+        bench_elastic(capi, args)
         return
     order, n = args.order, args.cells
     B, V, D = order + 1, 5, 5

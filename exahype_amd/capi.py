@@ -38,6 +38,8 @@ EXPORTED = (
     "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
+    "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_lds_variant",
+    "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy",
 )
 
 
@@ -64,6 +66,11 @@ class EulerIC(ctypes.Structure):
     _fields_ = [("rho0", ctypes.c_double), ("amp", ctypes.c_double), ("k", ctypes.c_double * 3),
                 ("vel", ctypes.c_double * 3), ("p", ctypes.c_double), ("nmodes", ctypes.c_int),
                 ("mode_amp", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
+
+
+class PatchSpec(ctypes.Structure):
+    """exa_patch_spec: one periodic unit-cube Euler patch with an entropy wave."""
+    _fields_ = [("n", ctypes.c_int), ("order", ctypes.c_int), ("k", ctypes.c_double * 3), ("amp", ctypes.c_double)]
 
 
 def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9, precision=PRECISION_FP64):
@@ -140,6 +147,13 @@ def lib():
     L.exa_lgrid_min_max.argtypes = [V, _D, _D]
     L.exa_lgrid_set_profiling.argtypes = [V, i]
     L.exa_lgrid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
+    L.exa_patchset_create.argtypes = [ctypes.POINTER(PatchSpec), i, i, d_, ctypes.c_long, ctypes.POINTER(V)]
+    L.exa_patchset_run.argtypes = [V, i, i, i, i]
+    L.exa_patchset_stats.argtypes = [V, _D, _D, _I]
+    L.exa_patchset_lds_variant.argtypes = [V, i]
+    L.exa_patchset_solution.argtypes = [V, i, _D]
+    L.exa_patchset_time.argtypes = [V, i, _D, _D]
+    L.exa_patchset_destroy.argtypes = [V]
     _lib = L
     return L
 
@@ -447,3 +461,68 @@ class LimitingGrid:
         check(lib().exa_lgrid_stage_times(self._h, _dp(ms), n), "stage_times")
         names = ("predictor", "riemann", "update", "dt", "limiter")
         return {names[i]: (float(ms[i]), int(n[i])) for i in range(5)}
+
+
+class PatchSet:
+    """Many independent periodic Euler patches of uneven size and order,
+    advanced together with per-patch dt (exa_patchset)."""
+
+    def __init__(self, specs, precision=PRECISION_FP64, hbm_budget_gb=200.0, keep_max_cells=0):
+        arr = (PatchSpec * len(specs))()
+        for j, (n, order, k, amp) in enumerate(specs):
+            arr[j].n, arr[j].order, arr[j].amp = int(n), int(order), float(amp)
+            for d in range(3):
+                arr[j].k[d] = float(k[d])
+        self.specs = list(specs)
+        h = ctypes.c_void_p()
+        check(lib().exa_patchset_create(arr, len(specs), precision, hbm_budget_gb, int(keep_max_cells),
+                                        ctypes.byref(h)), "exa_patchset_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().exa_patchset_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps, use_graph=True, autotune=True, warmup=0):
+        check(lib().exa_patchset_run(self._h, warmup, nsteps, int(use_graph), int(autotune)), "exa_patchset_run")
+
+    def stats(self):
+        sec, dof, nw = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(lib().exa_patchset_stats(self._h, ctypes.byref(sec), ctypes.byref(dof), ctypes.byref(nw)), "stats")
+        return sec.value, dof.value, nw.value
+
+    def lds_variant(self, order):
+        return lib().exa_patchset_lds_variant(self._h, order)
+
+    def solution(self, j):
+        n, order = self.specs[j][0], self.specs[j][1]
+        out = np.zeros(n ** 3 * (order + 1) ** 3 * 5)
+        check(lib().exa_patchset_solution(self._h, j, _dp(out)), "solution")
+        return out
+
+    def time(self, j):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        check(lib().exa_patchset_time(self._h, j, ctypes.byref(t), ctypes.byref(dt)), "time")
+        return t.value, dt.value
+
+
+def draw_patches(seed, npatches=96, sides=(9, 15, 27, 45, 81), orders=(2, 3, 4, 5, 6, 7)):
+    """The seeded patch set: side uniform in `sides`, order uniform in
+    `orders`, wave vector uniform in {1,2,3}^3, amplitude uniform in
+    [0.05, 0.3] (numpy PCG64 stream of `seed`, drawn in this order per patch)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(npatches):
+        n = int(rng.choice(sides))
+        order = int(rng.choice(orders))
+        k = tuple(int(x) for x in rng.integers(1, 4, size=3))
+        amp = float(rng.uniform(0.05, 0.3))
+        out.append((n, order, k, amp))
+    return out

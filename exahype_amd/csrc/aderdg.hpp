@@ -78,6 +78,12 @@ struct PredictorArgs {
   // limiter's troubled cells); null status = every cell
   const int* status;
   int status_limit;
+  // optional batch of independent patches: cell -> patch; dt and dx are then
+  // per patch (dt[patch], dx[3 * patch])
+  const int* cell_patch;
+  // LDS budget variant of the predictor: 0 = 144 KB, 1 = 64 KB (see
+  // PredictorLayout); the batch autotuner picks one per order
+  int lds_variant;
 };
 
 struct RiemannArgs {
@@ -102,6 +108,10 @@ struct UpdateArgs {
   unsigned long long* dt_slots;   // null: no time step computation
   double* dt_cell;                // [cells] per-cell stable dt, may be null
   long cell0;
+  // optional batch of independent patches: dt[patch], dx[3 * patch] and
+  // slots_per_patch dt slots per patch
+  const int* cell_patch;
+  int slots_per_patch;
 };
 
 struct VolumeArgs {
@@ -150,8 +160,10 @@ bool supported(const SolverDesc& sd);
 bool supported_sizes(const SolverDesc& sd);
 hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a,
                             long ncells, hipStream_t s);
-// Bytes of PredictorArgs::ws one cell needs (0: no workspace).
-size_t predictor_ws_bytes_per_cell(const SolverDesc& sd);
+// Bytes of PredictorArgs::ws one cell needs (0: no workspace), for an LDS variant.
+size_t predictor_ws_bytes_per_cell(const SolverDesc& sd, int lds_variant = 0);
+// Whether the 64 KB LDS variant is compiled for this solver.
+bool predictor_has_lds_variant(const SolverDesc& sd);
 // Cells per workspace chunk the host allocates for (bounds the workspace size).
 constexpr long kPredictorWsCells = 4096;
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s);
@@ -160,6 +172,10 @@ enum : int { kUpdSurface = 1, kUpdUpdate = 2, kUpdWriteLduh = 4 };
 hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, long ncells,
                          hipStream_t s);
 hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s);
+// Per-patch time steps of a batch: dt[p] = min of the patch's slots (then
+// reset); with advance_time, t[p] += the old dt[p] first.
+hipError_t launch_patch_dt_finalize(double* dt, double* t, unsigned long long* slots, int npatches,
+                                    int slots_per_patch, bool advance_time, hipStream_t s);
 hipError_t launch_volume_integral(const SolverDesc& sd, const VolumeArgs& a, long ncells, hipStream_t s);
 hipError_t launch_euler_ic(const ICArgs& a, hipStream_t s);
 // st->picard_total += sum(iters[0..ncells)), st->predictor_cells += ncells

@@ -78,14 +78,14 @@ __device__ inline void source_ncp(const T* Q, const T* sQ, const T* sDu, const T
 // registers or, when registers run out (WSQ), in a per-cell device workspace
 // that each thread reads and writes only at its own nodes.  Large sizes run
 // the loop over time slices rolled (ROLLED).
-template <int B, int V, int D, bool USE_NCP, typename T>
+template <int B, int V, int D, bool USE_NCP, typename T, int LDSK = 144>
 struct PredictorLayout {
   static constexpr int B2 = B * B, B3 = B2 * B;
   static constexpr int NT = node_threads(B);
   static constexpr bool IS_FP64 = sizeof(T) == sizeof(double);
   // LDS: one slice of fluxes [dir][var][node] (reused for lQhi / lFhi staging)
   static constexpr int SF = (3 * V > D ? 3 * V : D) * B3;
-  static constexpr long kLdsBudget = 144 * 1024;
+  static constexpr long kLdsBudget = (long)LDSK * 1024;
   static constexpr long kSliceBytes = (long)V * B3 * sizeof(T);
   static constexpr long kOther =
       (long)(SF + (USE_NCP ? V * B3 : 0) + 2 * B2 + B + 3 + 2 * (NT / kWave)) * sizeof(T);
@@ -116,7 +116,7 @@ struct PredictorLayout {
 //     into the next iterate through iK1 (the reference's n order), so rhs is
 //     never stored for all t.
 // ---------------------------------------------------------------------------
-template <int B, class PDE, int TERMS, bool EXPORT, typename T>
+template <int B, class PDE, int TERMS, bool EXPORT, typename T, int LDSK = 144>
 __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   constexpr bool USE_SRC_NCP = USE_SOURCE || USE_NCP;
   constexpr bool IS_FP64 = sizeof(T) == sizeof(double);
   constexpr int MAX_ITER = 2 * B;  // spaceTimePredictorNonlinear.cpph:85
-  using L = PredictorLayout<B, V, D, USE_NCP, T>;
+  using L = PredictorLayout<B, V, D, USE_NCP, T, LDSK>;
   constexpr int SF = L::SF, NQL = L::NQL, NQR = L::NQR, UNROLL_T = L::UNROLL_T;
   constexpr bool WSQ = L::WSQ;
   __shared__ T sF[SF];
@@ -165,9 +165,10 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   }
   for (int i = tid; i < B; i += NT) sW[i] = tc<T>(exa_gl_weights[N][i]);
 
-  const double* dxp = a.dx + cell * a.dx_stride;
+  const int patch = a.cell_patch ? a.cell_patch[cell] : 0;
+  const double* dxp = a.cell_patch ? a.dx + (size_t)patch * 3 : a.dx + cell * a.dx_stride;
   const T invDx[3] = {T(1.0) / T(dxp[0]), T(1.0) / T(dxp[1]), T(1.0) / T(dxp[2])};
-  const T dt = T(*a.dt);
+  const T dt = T(a.dt[patch]);
   // Every lane writes the same values: no divergent branch here (and no
   // lane-indexed private array, which would live in scratch memory).
   sInvDx[0] = invDx[0];
@@ -620,7 +621,8 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
   const bool active = tid < B3;
   const int node = active ? tid : 0;
   const int z = node / B2, y = (node / B) % B, x = node % B;
-  const double* dxp = a.dx + cell * a.dx_stride;
+  const int patch = a.cell_patch ? a.cell_patch[cell] : 0;
+  const double* dxp = a.cell_patch ? a.dx + (size_t)patch * 3 : a.dx + cell * a.dx_stride;
   const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
   double* luh = a.luh + cell * (size_t)(B3 * D) + node * D;
   double q[D];
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
       }
     }
     if (UPDATE) {
-      const double dt = *a.dt;
+      const double dt = a.dt[patch];
       const double updateSize = dt / (wz * wy * wx);
 #pragma unroll
       for (int m = 0; m < V; ++m) q[m] += du[m] * updateSize;
@@ -677,7 +679,10 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
     dtn = block_min<NT>(dtn, sRed);
     if (tid == 0) {
       if (a.dt_cell) a.dt_cell[cell] = dtn;
-      if (a.dt_slots) atomic_min_positive(a.dt_slots + (cell % kDtSlots), dtn);
+      if (a.dt_slots)
+        atomic_min_positive(a.dt_slots + (a.cell_patch ? (size_t)patch * a.slots_per_patch + cell % a.slots_per_patch
+                                                       : cell % kDtSlots),
+                            dtn);
     }
   }
 }
@@ -867,7 +872,7 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
                             hipStream_t s) {
   if (ncells <= 0) return hipSuccess;
   if (export_temps && sd.precision != kFP64) return hipErrorNotSupported;
-  const size_t ws = predictor_ws_bytes_per_cell(sd);
+  const size_t ws = predictor_ws_bytes_per_cell(sd, a.lds_variant);
   if (ws > 0 && (a.ws == nullptr || a.ws_cells <= 0)) return hipErrorInvalidValue;
   const long chunk = ws > 0 ? a.ws_cells : ncells;
   return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
@@ -886,9 +891,14 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kNCP, false, double>), grid, block, 0, s, ac);
       } else if (sd.precision == kFP32Bulk) {
         if (sd.terms != kFlux) return hipErrorNotSupported;
-        hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float>), grid, block, 0, s, ac);
+        if (a.lds_variant == 1 && PDE::P == 0)
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float, 64>), grid, block, 0, s, ac);
+        else
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float>), grid, block, 0, s, ac);
       } else if (sd.terms == kFlux) {
-        if (export_temps)
+        if (a.lds_variant == 1 && PDE::P == 0 && !export_temps)
+          hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, double, 64>), grid, block, 0, s, ac);
+        else if (export_temps)
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, true, double>), grid, block, 0, s, ac);
         else
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, double>), grid, block, 0, s, ac);
@@ -908,10 +918,19 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
   });
 }
 
-size_t predictor_ws_bytes_per_cell(const SolverDesc& sd) {
+bool predictor_has_lds_variant(const SolverDesc& sd) {
+  return sd.pde == kPdeEuler && sd.npar == 0 && sd.terms == kFlux;
+}
+
+size_t predictor_ws_bytes_per_cell(const SolverDesc& sd, int lds_variant) {
   size_t bytes = 0;
   (void)dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
     constexpr int V = PDE::V, D = PDE::V + PDE::P;
+    if (lds_variant == 1 && predictor_has_lds_variant(sd)) {
+      bytes = sd.precision == kFP32Bulk ? PredictorLayout<B, V, D, false, float, 64>::ws_bytes
+                                        : PredictorLayout<B, V, D, false, double, 64>::ws_bytes;
+      return hipSuccess;
+    }
     if (sd.precision == kFP32Bulk)
       bytes = (sd.terms & kNCP) ? PredictorLayout<B, V, D, true, float>::ws_bytes
                                 : PredictorLayout<B, V, D, false, float>::ws_bytes;
@@ -977,6 +996,27 @@ hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, lo
     }
     return hipGetLastError();
   });
+}
+
+__global__ __launch_bounds__(256) void patch_dt_finalize_kernel(double* dt, double* t, unsigned long long* slots,
+                                                                int npatches, int sp, int advance) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= npatches) return;
+  double m = DBL_MAX;
+  for (int i = 0; i < sp; ++i) {
+    const double v = __longlong_as_double((long long)slots[(size_t)p * sp + i]);
+    m = v < m ? v : m;
+    slots[(size_t)p * sp + i] = kDtSlotEmpty;
+  }
+  if (advance) t[p] += dt[p];
+  dt[p] = m;
+}
+
+hipError_t launch_patch_dt_finalize(double* dt, double* t, unsigned long long* slots, int npatches,
+                                    int slots_per_patch, bool advance_time, hipStream_t s) {
+  hipLaunchKernelGGL(patch_dt_finalize_kernel, dim3((npatches + 255) / 256), dim3(256), 0, s, dt, t, slots, npatches,
+                     slots_per_patch, advance_time ? 1 : 0);
+  return hipGetLastError();
 }
 
 hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s) {

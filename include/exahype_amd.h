@@ -246,6 +246,49 @@ enum { EXA_STAGE_LIMITER = 4, EXA_NUM_LSTAGES = 5 };
 int exa_lgrid_set_profiling(exa_lgrid* g, int on);
 int exa_lgrid_stage_times(exa_lgrid* g, double* ms, long* launches);
 
+/* ---------------------------------------------------------------------------
+ * Patch set: many independent periodic Euler grids ("patches") of uneven size
+ * and order advanced together, each with its own time step size.  Every patch
+ * is the unit cube with n^3 cells and the entropy wave rho = 1 + amp
+ * sin(2 pi k.x), u = v = w = 1, p = 1.  Each of them follows the non-fused
+ * global time step of exa_grid (ADERDGSolver::updateSolution,
+ * performPredictionAndVolumeIntegral, solveRiemannProblemAtInterface,
+ * ADERDGSolver.cpp:1881-2290; Runner.cpp:1116-1162) with its own dt.  Patches
+ * of one order share every launch (cell -> patch map); the patches run in
+ * waves whose state fits hbm_budget_gb; each wave's time step is captured
+ * once in a hipGraph and replayed.  The reference has no batching of
+ * independent grids: this scheduler replaces running one ExaHyPE process per
+ * patch.
+ * ------------------------------------------------------------------------- */
+typedef struct exa_patch_spec {
+  int n;       /* cells per side */
+  int order;   /* polynomial order 2..7 */
+  double k[3]; /* wave vector of the entropy wave */
+  double amp;  /* density amplitude of the entropy wave */
+} exa_patch_spec;
+typedef struct exa_patchset exa_patchset;
+/* precision: EXA_PRECISION_*; the solution of every patch with at most
+ * keep_max_cells cells is copied to host memory at the end of its wave (for
+ * exa_patchset_solution; 0 keeps none).  EXA_E_NOMEM when one patch alone
+ * exceeds the budget. */
+int exa_patchset_create(const exa_patch_spec* patches, int npatches, int precision, double hbm_budget_gb,
+                        long keep_max_cells, exa_patchset** out);
+/* Set up every wave (initial condition, first dt and predictor), optionally
+ * autotune the predictor's LDS variant per order, and advance each patch
+ * warmup_steps + nsteps time steps, replaying a captured hipGraph when
+ * use_graph != 0; the last nsteps are timed.  Synchronous. */
+int exa_patchset_run(exa_patchset* b, int warmup_steps, int nsteps, int use_graph, int autotune_variants);
+/* Device seconds of the timed steps (HIP events around each wave's timed steps),
+ * degrees of freedom updated in them, and the number of waves. */
+int exa_patchset_stats(exa_patchset* b, double* seconds, double* dof_updates, int* nwaves);
+/* LDS variant the autotuner chose for an order (0: 144 KB, 1: 64 KB; -1 none). */
+int exa_patchset_lds_variant(exa_patchset* b, int order);
+/* Solution [n^3][B^3][5] of one patch after the run (kept patches only). */
+int exa_patchset_solution(exa_patchset* b, int patch, double* luh);
+/* Time stamp and the dt of the next step of one patch after the run. */
+int exa_patchset_time(exa_patchset* b, int patch, double* t, double* dt);
+int exa_patchset_destroy(exa_patchset* b);
+
 #ifdef __cplusplus
 }
 #endif

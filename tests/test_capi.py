@@ -107,3 +107,22 @@ def test_cpp_dropin_header_compiles_and_links(exa, tmp_path):
                           str(tmp_path / "glue"), "-L", libdir, "-lexahype_amd", "-Wl,-rpath," + libdir],
                          capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_patch_sets_rejected_before_any_device_work(exa):
+    with pytest.raises(exa.ExaError, match="status 2"):
+        exa.PatchSet([(3, 8, (1, 1, 1), 0.1)])  # order outside 2..7
+    with pytest.raises(exa.ExaError, match="status 4"):
+        exa.PatchSet([(81, 7, (1, 1, 1), 0.1)], hbm_budget_gb=1.0)  # one patch exceeds the budget
+    with pytest.raises(exa.ExaError, match="status 3"):
+        exa.PatchSet([(3, 3, (1, 1, 1), 0.1)], precision=5)
+
+
+def test_seeded_patch_set_draw(exa):
+    a = exa.draw_patches(2024)
+    assert a == exa.draw_patches(2024) and a != exa.draw_patches(2025)
+    assert len(a) == 96
+    for n, order, k, amp in a:
+        assert n in (9, 15, 27, 45, 81) and 2 <= order <= 7
+        assert all(x in (1, 2, 3) for x in k) and 0.05 <= amp <= 0.3
+    assert {o for _, o, _, _ in a} == set(range(2, 8))

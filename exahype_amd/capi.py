@@ -131,7 +131,7 @@ def lib():
     L.exa_project_on_dg_space.argtypes = [_D, i, i, i, _D]
     L.exa_find_cell_local_min_and_max.argtypes = [_D, i, i, _D, _D]
     L.exa_discrete_maximum_principle_and_min_and_max_search.argtypes = [_D, i, i, d_, d_, _D, _D, _I]
-    L.exa_musclhancock_solution_update.argtypes = [S, _D, _D, _D, d_, i, i]
+    L.exa_musclhancock_solution_update.argtypes = [S, _D, _D, _D, d_, i, i, _D]
     L.exa_lgrid_create.argtypes = [S, i, i, i, _D, _D, i, i, i, d_, d_, ctypes.POINTER(V)]
     L.exa_lgrid_destroy.argtypes = [V]
     L.exa_lgrid_num_cells.argtypes = [V]
@@ -373,14 +373,16 @@ def dmp(luh, basis_size, bmin, bmax, relax=1e-4, scaling=1e-3, num_variables=5):
     return bool(ok[0]), bmin, bmax
 
 
-def muscl_update(s, patch, dx, dt):
-    """One MUSCL-Hancock update of a patch with filled ghost layers (interior written)."""
+def muscl_update(s, patch, dx, dt, return_dt=False):
+    """One MUSCL-Hancock update of a patch with filled ghost layers (interior
+    written); with return_dt also the admissible dt the reference returns."""
     old = _f64(patch).ravel()
     new = old.copy()
     L = 2 * s.order + 1
-    check(lib().exa_musclhancock_solution_update(ctypes.byref(s), _dp(new), _dp(old), _dp(_dx3(dx)), dt, L, 2),
-          "musclhancock")
-    return new
+    dtmax = ctypes.c_double()
+    check(lib().exa_musclhancock_solution_update(ctypes.byref(s), _dp(new), _dp(old), _dp(_dx3(dx)), dt, L, 2,
+                                                 ctypes.byref(dtmax)), "musclhancock")
+    return (new, dtmax.value) if return_dt else new
 
 
 class LimitingGrid:

@@ -54,6 +54,9 @@ struct LimArgs {
   double relax, scaling;
   int T, A, cure;
   int ncells;
+  // optional: min over every FV face of 0.99/3 h_d / s_max (the admissible dt
+  // musclhancock's solutionUpdate returns), as the bits of a positive double
+  unsigned long long* fv_dt_max;
 };
 
 enum : int { kLimFlagIrregular = 0, kLimFlagFvCount = 1 };

@@ -3,6 +3,8 @@
 // update, and the device-resident limiting grid.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
@@ -565,14 +567,14 @@ extern "C" int exa_discrete_maximum_principle_and_min_and_max_search(const doubl
 
 extern "C" int exa_musclhancock_solution_update(const exa_solver* s, double* luh_new, const double* luh,
                                                 const double* dx, double dt, int patch_size,
-                                                int ghost_layer_width) {
+                                                int ghost_layer_width, double* dt_max_allowed) {
   SolverDesc sd;
   if (!to_desc(s, &sd) || !luh_new || !luh || !dx) return EXA_E_INVALID;
   if (!exa::limiter_supported(sd)) return EXA_E_UNSUPPORTED;
   if (patch_size != exa::lim_L(sd.order) || ghost_layer_width != exa::kLimGhost) return EXA_E_UNSUPPORTED;
   EXA_TRY(exa::ensure_limiter_tables());
   const long S = exa::lim_S(sd.order), patch = S * S * S * 5, B3 = (long)(sd.order + 1) * (sd.order + 1) * (sd.order + 1);
-  DevBuf d_lim, d_luh, d_mm, d_cur, d_list, d_flags, d_dx, d_state;
+  DevBuf d_lim, d_luh, d_mm, d_cur, d_list, d_flags, d_dx, d_state, d_dtmax;
   EXA_TRY(d_lim.alloc(sizeof(double) * 2 * patch));
   EXA_TRY(d_luh.alloc(sizeof(double) * B3 * 5));
   EXA_TRY(d_mm.alloc(sizeof(double) * 10));
@@ -581,6 +583,9 @@ extern "C" int exa_musclhancock_solution_update(const exa_solver* s, double* luh
   EXA_TRY(d_flags.alloc(sizeof(int) * 4));
   EXA_TRY(d_dx.alloc(sizeof(double) * 3));
   EXA_TRY(d_state.alloc(sizeof(exa::StepState)));
+  EXA_TRY(d_dtmax.alloc(sizeof(unsigned long long)));
+  const double dmax0 = DBL_MAX;
+  EXA_TRY(hipMemcpy(d_dtmax.p, &dmax0, sizeof(double), hipMemcpyHostToDevice));
   EXA_TRY(hipMemcpy(d_lim.p, luh, sizeof(double) * patch, hipMemcpyHostToDevice));
   EXA_TRY(hipMemcpy(d_lim.d() + patch, luh_new, sizeof(double) * patch, hipMemcpyHostToDevice));
   EXA_TRY(hipMemset(d_cur.p, 0, 1));
@@ -597,9 +602,11 @@ extern "C" int exa_musclhancock_solution_update(const exa_solver* s, double* luh
   a.dx = d_dx.d();
   a.st_state = static_cast<exa::StepState*>(d_state.p);
   a.ncells = 1;
+  a.fv_dt_max = static_cast<unsigned long long*>(d_dtmax.p);
   EXA_TRY(exa::launch_lim_iota(a.fv_list, 1, a.flags, nullptr));
   EXA_TRY(exa::launch_lim_fv_update(sd, a, false, nullptr));
   EXA_TRY(hipDeviceSynchronize());
   EXA_TRY(hipMemcpy(luh_new, d_lim.d() + patch, sizeof(double) * patch, hipMemcpyDeviceToHost));
+  if (dt_max_allowed) EXA_TRY(hipMemcpy(dt_max_allowed, d_dtmax.p, sizeof(double), hipMemcpyDeviceToHost));
   return EXA_OK;
 }

@@ -430,7 +430,9 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
       }
     }
     __syncthreads();
-    // Rusanov on every face (RUS:98-156), flux stored over the left state
+    // Rusanov on every face (RUS:98-156), flux stored over the left state;
+    // dt_max_allowed = min over faces of cflFactor/3 h_d / s_max (MH:267-329)
+    double dt_allowed = DBL_MAX;
     for (int j = threadIdx.x; j < 3 * NF; j += kMusclNT) {
       const int d = j / NF, r = j % NF;
       const int fa = r / (L * (L + 1)), fb = (r / (L + 1)) % L, f = r % (L + 1);
@@ -441,6 +443,8 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
         qR[v] = sF[fidx(d, 1, v, fa, fb, f)];
       }
       const double smax = dmax(Euler<0>::max_abs_eigenvalue(qR, d), Euler<0>::max_abs_eigenvalue(qL, d));
+      const double dta = 0.99 / 3 * h[d] / smax;
+      dt_allowed = dta < dt_allowed ? dta : dt_allowed;
       euler_flux_dir(qL, d, FL);
       euler_flux_dir(qR, d, FR);
 #pragma unroll
@@ -449,6 +453,10 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
         fn += 0.5 * (FL[v] + FR[v]);
         sF[fidx(d, 0, v, fa, fb, f)] = fn;
       }
+    }
+    if (a.fv_dt_max) {
+      const double m = wave_min(dt_allowed);
+      if ((threadIdx.x & (kWave - 1)) == 0) atomicMin(a.fv_dt_max, (unsigned long long)__double_as_longlong(m));
     }
     __syncthreads();
     // flux differences in the reference's accumulation order (MH:267-329)

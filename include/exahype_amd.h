@@ -202,10 +202,12 @@ int exa_discrete_maximum_principle_and_min_and_max_search(const double* luh, int
 /* musclhancock::c::solutionUpdate (ExaHyPE/kernels/finitevolumes/musclhancock/
  * c/3d/musclhancock.cpph:49) for Euler: luh_new = update(luh) on one patch
  * with filled ghost layers (patch_size L, ghost_layer_width 2); dx is the
- * DG cell size.  Only the interior of luh_new is written. */
+ * DG cell size.  Only the interior of luh_new is written.  *dt_max_allowed
+ * (may be NULL) receives the function's return value: the minimum over the
+ * patch's faces of 0.99/3 * h_d / s_max. */
 int exa_musclhancock_solution_update(const exa_solver* s, double* luh_new, const double* luh,
                                      const double* dx, double dt, int patch_size,
-                                     int ghost_layer_width);
+                                     int ghost_layer_width, double* dt_max_allowed);
 
 /* Device-resident limiting grid: LimitingADERDGSolver's non-fused time step
  * (ExaHyPE/exahype/solvers/LimitingADERDGSolver.cpp:806-1742, Runner.cpp:866-946,

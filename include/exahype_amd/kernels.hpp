@@ -1,8 +1,12 @@
 // exahype_amd/kernels.hpp -- the kernels::aderdg::generic::c templates of the
-// reference (ExaHyPE/kernels/aderdg/generic/Kernels.h:85-213) re-declared on
-// top of the C-ABI, so that generated solver glue
-// (Toolkit/src/eu/exahype/solvers/templates/AbstractGenericADERDGSolverImplementation.template:46-161)
-// compiles unchanged against the HIP implementation.
+// reference (ExaHyPE/kernels/aderdg/generic/Kernels.h:85-213), the limiter
+// kernels (ExaHyPE/kernels/limiter/generic/Limiter.h:48-141) and the
+// MUSCL-Hancock update (ExaHyPE/kernels/finitevolumes/musclhancock/c/
+// musclhancock.h:38-41) re-declared on top of the C-ABI, so that generated
+// solver glue (Toolkit/src/eu/exahype/solvers/templates/
+// AbstractGenericADERDGSolverImplementation.template:46-161,
+// AbstractGenericFiniteVolumesSolverInCImplementation.template:48-49) and the
+// limiting solver's calls compile unchanged against the HIP implementation.
 //
 // Differences a caller must know:
 //  * PDE callbacks cannot cross into device code, so SolverType names one of
@@ -10,6 +14,10 @@
 //        static constexpr int ExaDevicePde = EXA_PDE_EULER;
 //    (NumberOfVariables, NumberOfParameters, Order, CFL as in the reference).
 //  * dx may be any type with operator[] (tarch::la::Vector<3,double> works).
+//  * The limiter functions take any solver type with getNumberOfVariables(),
+//    getNodesPerCoordinateAxis() and getDMPObservables() (ADERDGSolver has
+//    them); the library implements the default observables (the first
+//    getDMPObservables() == getNumberOfVariables() variables, Euler).
 //  * Calls are synchronous and take host arrays; errors throw exahype_amd::Error.
 //  * For whole grid levels use exa_grid_* instead: these single-cell calls pay
 //    a host<->device copy each.
@@ -139,4 +147,82 @@ double stableTimeStepSize(SolverType& /*solver*/, const double* const luh, const
 }  // namespace c
 }  // namespace generic
 }  // namespace aderdg
+}  // namespace kernels
+
+namespace kernels {
+namespace limiter {
+namespace generic {
+namespace c {
+
+inline void projectOnFVLimiterSpace(const double* const luh, const int numberOfVariables, const int basisSize,
+                                    const int ghostLayerWidth, double* const lim) {
+  exahype_amd::check(exa_project_on_fv_limiter_space(luh, numberOfVariables, basisSize, ghostLayerWidth, lim),
+                     "projectOnFVLimiterSpace");
+}
+
+inline void projectOnDGSpace(const double* const lim, const int numberOfVariables, const int basisSize,
+                             const int ghostLayerWidth, double* const luh) {
+  exahype_amd::check(exa_project_on_dg_space(lim, numberOfVariables, basisSize, ghostLayerWidth, luh),
+                     "projectOnDGSpace");
+}
+
+template <typename ADERDGSolverType>
+inline void findCellLocalMinAndMax(const double* const luh, const ADERDGSolverType* solver,
+                                   double* const localMinPerVariables, double* const localMaxPerVariable) {
+  if (solver->getDMPObservables() != solver->getNumberOfVariables())
+    throw exahype_amd::Error("findCellLocalMinAndMax: only the default observables are implemented");
+  exahype_amd::check(exa_find_cell_local_min_and_max(luh, solver->getNumberOfVariables(),
+                                                     solver->getNodesPerCoordinateAxis(), localMinPerVariables,
+                                                     localMaxPerVariable),
+                     "findCellLocalMinAndMax");
+}
+
+template <typename ADERDGSolverType>
+inline bool discreteMaximumPrincipleAndMinAndMaxSearch(const double* const luh, const ADERDGSolverType* solver,
+                                                       const double relaxationParameter,
+                                                       const double differenceScaling,
+                                                       double* boundaryMinPerVariables,
+                                                       double* boundaryMaxPerVariables) {
+  if (solver->getDMPObservables() != solver->getNumberOfVariables())
+    throw exahype_amd::Error("discreteMaximumPrincipleAndMinAndMaxSearch: only the default observables are implemented");
+  int ok = 0;
+  exahype_amd::check(exa_discrete_maximum_principle_and_min_and_max_search(
+                         luh, solver->getNumberOfVariables(), solver->getNodesPerCoordinateAxis(),
+                         relaxationParameter, differenceScaling, boundaryMinPerVariables, boundaryMaxPerVariables,
+                         &ok),
+                     "discreteMaximumPrincipleAndMinAndMaxSearch");
+  return ok != 0;
+}
+
+}  // namespace c
+}  // namespace generic
+}  // namespace limiter
+
+namespace finitevolumes {
+namespace musclhancock {
+namespace c {
+
+// Returns the admissible time step size (min over faces of 0.99/3 h / s_max).
+template <bool useSource, bool useNCP, bool useFlux, class SolverType, typename DxVector>
+double solutionUpdate(SolverType& /*solver*/, double* luh_new, const double* luh, const DxVector& dx, double dt) {
+  exa_solver s;
+  s.pde = SolverType::ExaDevicePde;
+  s.order = (SolverType::PatchSize - 1) / 2;
+  s.num_variables = SolverType::NumberOfVariables;
+  s.num_parameters = SolverType::NumberOfParameters;
+  s.terms = exahype_amd::terms(useSource, useFlux, useNCP);
+  s.no_time_averaging = 0;
+  s.cfl = SolverType::CFL;
+  s.precision = EXA_PRECISION_FP64;
+  double d[3], dt_max = 0.0;
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(exa_musclhancock_solution_update(&s, luh_new, luh, d, dt, SolverType::PatchSize,
+                                                      SolverType::GhostLayerWidth, &dt_max),
+                     "musclhancock::solutionUpdate");
+  return dt_max;
+}
+
+}  // namespace c
+}  // namespace musclhancock
+}  // namespace finitevolumes
 }  // namespace kernels

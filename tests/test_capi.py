@@ -93,7 +93,40 @@ void glue(MyEulerSolver& solver, double* luh, double* lduh, double* lQhbnd, doub
   kernels::aderdg::generic::c::solutionUpdate<MyEulerSolver>(solver, luh, lduh, 1e-3);
   (void)kernels::aderdg::generic::c::stableTimeStepSize<MyEulerSolver>(solver, luh, dx);
 }
-int main(int argc, char**) { if (argc > 99) glue(*(MyEulerSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0); return 0; }
+// The limiting solver's calls (Limiter.h:48-141) and the FV glue's MUSCL call.
+struct MyADERDGSolver {  // the accessors of exahype::solvers::ADERDGSolver used by the limiter
+  int getNumberOfVariables() const { return 5; }
+  int getNodesPerCoordinateAxis() const { return 4; }
+  int getDMPObservables() const { return 5; }
+};
+struct MyFVSolver {  // the constants a generated Abstract<FVSolver> carries
+  static constexpr int NumberOfVariables = 5;
+  static constexpr int NumberOfParameters = 0;
+  static constexpr int PatchSize = 7;
+  static constexpr int GhostLayerWidth = 2;
+  static constexpr double CFL = 0.9;
+  static constexpr int ExaDevicePde = EXA_PDE_EULER;
+};
+double limiter_glue(const MyADERDGSolver* solver, MyFVSolver& fv, double* luh, double* lim, double* limNew,
+                    double* mn, double* mx) {
+  const Vec3 dx{{0.1, 0.1, 0.1}};
+  kernels::limiter::generic::c::projectOnFVLimiterSpace(luh, 5, 4, 2, lim);
+  kernels::limiter::generic::c::projectOnDGSpace(lim, 5, 4, 2, luh);
+  kernels::limiter::generic::c::findCellLocalMinAndMax(luh, solver, mn, mx);
+  const bool ok = kernels::limiter::generic::c::discreteMaximumPrincipleAndMinAndMaxSearch(luh, solver, 1e-4, 1e-3,
+                                                                                          mn, mx);
+  double maxAdmissibleDt = kernels::finitevolumes::musclhancock::c::solutionUpdate<false, false, true, MyFVSolver>(
+      fv, limNew, lim, dx, 1e-3);
+  return ok ? maxAdmissibleDt : 0.0;
+}
+int main(int argc, char**) {
+  if (argc > 99) {
+    glue(*(MyEulerSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
+    MyFVSolver fv;
+    return (int)limiter_glue(nullptr, fv, 0, 0, 0, 0, 0);
+  }
+  return 0;
+}
 '''
 
 

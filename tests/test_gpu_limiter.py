@@ -80,9 +80,11 @@ def test_muscl_patch_vs_oracle(exa, oracle, order):
     s_exa, s_ora = exa.euler_solver(order), oracle.make_solver(order)
     for dt in (1e-4, 1e-3):
         P = _blast_patch(rng, L, g)
-        new = exa.muscl_update(s_exa, P, 0.1, dt).reshape(P.shape)
-        onew, _ = oracle.muscl_update(s_ora, L, g, P, 0.1, dt)
+        new, dtmax = exa.muscl_update(s_exa, P, 0.1, dt, return_dt=True)
+        new = new.reshape(P.shape)
+        onew, odtmax = oracle.muscl_update(s_ora, L, g, P, 0.1, dt)
         onew = onew.reshape(P.shape)
+        assert abs(dtmax - odtmax) <= 1e-14 * odtmax
         inner = (slice(g, g + L),) * 3
         assert np.isfinite(new[inner]).all()
         assert np.abs(new[inner] - P[inner]).max() > 1e-8  # it evolved

@@ -170,7 +170,12 @@ def test_muscl_constant_state_is_steady(oracle):
     new, dtm = oracle.muscl_update(s, L, g, P, 0.1, 1e-3)
     new = new.reshape(P.shape)
     assert np.array_equal(new[g:g + L, g:g + L, g:g + L], P[g:g + L, g:g + L, g:g + L])
-    assert np.isfinite(dtm) and dtm > 0
+    # every face sees the uniform state (all minmod slopes vanish, also next to
+    # the zero edge ghosts), so the admissible dt is 0.99/3 h / (|u_x| + c)
+    rho, u = q[0], q[1:4] / q[0]
+    p = 0.4 * (q[4] - 0.5 * rho * (u @ u))
+    c = np.sqrt(1.4 * p / rho)
+    assert abs(dtm - 0.99 / 3 * (0.1 / L) / (np.abs(u).max() + c)) <= 1e-14 * dtm
 
 
 def test_muscl_one_dimensional_and_conservative(oracle):

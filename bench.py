@@ -89,6 +89,23 @@ def run_form(capi, order, n, steps, warmup, seed, precision):
     return dict(grid=g, wall=wall, stages=stages, picard_total=picard_total, pred_cells=pred_cells, t=t_now)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "predictor_euler_n5_81cube_pmc_traffic.txt")
+
+
+def measured_traffic(form, order, n):
+    """HBM bytes per predictor launch measured with rocprofv3 --pmc FETCH_SIZE
+    and WRITE_SIZE (each scaled by the same counter's reading for a copy of
+    known size), as recorded in the profile summary for this workload; None
+    when no measurement of this form and size is recorded."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    key = f"traffic_bytes_per_launch {form} N={order} {n}^3"
+    for line in open(TRAFFIC_FILE):
+        if line.startswith(key + " "):
+            return int(line.split()[-1])
+    return None
+
+
 def roofline_for(r, B, V, D, peak, kernel):
     pred_ms, pred_n = r["stages"]["predictor"]
     avg_ms = pred_ms / max(pred_n, 1)
@@ -370,6 +387,10 @@ def main():
         roof = roofline_for(m, B, V, D, FP32_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,float> (predictor + volume integral)")
     else:
         roof = roofline_for(m, B, V, D, FP64_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,double> (predictor + volume integral)")
+    roof["traffic"] = measured_traffic(main_form, order, n)
+    # algorithmic HBM bytes per launch: read luh (double), write lduh, lQhbnd, lFhbnd (bulk type)
+    bulk = 4 if main_form == "fp32" else 8
+    roof["algorithmic_bytes_per_launch"] = ncells * (8 * B ** 3 * D + bulk * (B ** 3 * V + 6 * B * B * (D + V)))
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
@@ -400,8 +421,9 @@ def main():
         r64 = res["fp64"]
         out["fp64"] = {"value": dof * args.steps / r64["wall"], "ms_per_step": r64["wall"] / args.steps * 1e3,
                        "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in r64["stages"].items()},
-                       "roofline": roofline_for(r64, B, V, D, FP64_PEAK_TFLOPS,
-                                                "predictor_kernel<6,Euler,flux,double>")}
+                       "roofline": dict(roofline_for(r64, B, V, D, FP64_PEAK_TFLOPS,
+                                                     "predictor_kernel<6,Euler,flux,double>"),
+                                        traffic=measured_traffic("fp64", order, n))}
     if "fp32" in res and "fp64" in res:
         out["fp32_vs_fp64"] = {"speedup": out["fp64"]["ms_per_step"] / out["ms_per_step"],
                                "l2_rel_err": res["fp32"]["l2"], "linf_rel_err": res["fp32"]["linf"],

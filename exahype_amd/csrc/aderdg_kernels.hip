@@ -100,6 +100,7 @@ struct PredictorLayout {
   static constexpr bool WSQ = NQR > 0 && (NT >= 8 * kWave || ROLLED);
   // unroll factor of the loop over time slices
   static constexpr int UNROLL_T = ROLLED ? 1 : B;
+  static constexpr bool SPLIT = ROLLED && WSQ && NQL > 0;
   static constexpr size_t ws_bytes = WSQ ? (size_t)NQR * V * B3 * sizeof(T) : 0;
 };
 
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   constexpr int MAX_ITER = 2 * B;  // spaceTimePredictorNonlinear.cpph:85
   using L = PredictorLayout<B, V, D, USE_NCP, T, LDSK>;
   constexpr int SF = L::SF, NQL = L::NQL, NQR = L::NQR, UNROLL_T = L::UNROLL_T;
+  constexpr bool SPLIT = L::SPLIT;
   constexpr bool WSQ = L::WSQ;
   __shared__ T sF[SF];
   __shared__ T sQi[NQL > 0 ? NQL * V * B3 : 1];  // lQi^k slices t < NQL, [t][var][node]
@@ -137,7 +139,6 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   __shared__ T sK[B2];   // Kxi[n][l]
   __shared__ T sDu[B2];  // dudx[l][n]
   __shared__ T sW[B];    // Gauss-Legendre weights
-  __shared__ T sInvDx[3];
   __shared__ T sRed[2 * (NT / kWave)];
 
   T* const out_lduh = static_cast<T*>(a.lduh);
@@ -169,11 +170,6 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   const double* dxp = a.cell_patch ? a.dx + (size_t)patch * 3 : a.dx + cell * a.dx_stride;
   const T invDx[3] = {T(1.0) / T(dxp[0]), T(1.0) / T(dxp[1]), T(1.0) / T(dxp[2])};
   const T dt = T(a.dt[patch]);
-  // Every lane writes the same values: no divergent branch here (and no
-  // lane-indexed private array, which would live in scratch memory).
-  sInvDx[0] = invDx[0];
-  sInvDx[1] = invDx[1];
-  sInvDx[2] = invDx[2];
 
   T q0[D];
   {
@@ -190,8 +186,10 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   // slice by slice (iK1 applied in the reference's n order).
   T qi_reg[(NQR > 0 && !WSQ) ? NQR : 1][V];
   T* const ws = WSQ ? static_cast<T*>(a.ws) + (size_t)blockIdx.x * (NQR * V * B3) : nullptr;
-  // register slices are only used with compile-time t (unrolled loops)
-  auto qi = [&](int t, int m) -> T& {
+  // Register slices are only used with compile-time t (unrolled loops).  Loads
+  // and stores go through separate accessors with a uniform branch on t, so no
+  // pointer that may address either LDS or global memory is ever formed.
+  auto qi = [&](int t, int m) __device__ -> T {
     if (t < NQL)
       return sQi[(t * V + m) * B3 + node];
     else if (WSQ)
@@ -199,15 +197,28 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     else
       return qi_reg[t - NQL][m];
   };
+  auto qi_set = [&](int t, int m, T v) __device__ {
+    if (t < NQL)
+      sQi[(t * V + m) * B3 + node] = v;
+    else if (WSQ)
+      ws[((t - NQL) * V + m) * B3 + node] = v;
+    else
+      qi_reg[t - NQL][m] = v;
+  };
 #pragma unroll
   for (int t = 0; t < B; ++t)
 #pragma unroll
     for (int m = 0; m < V; ++m)
-      qi(t, m) = q0[m];
+      qi_set(t, m, q0[m]);
   T qn[B][V];
   int iters = MAX_ITER;
   const T iweight = T(1.0) / w3;
   __syncthreads();
+
+  // Phase 2 tasks per direction: one (variable m, line) each, NTD of them in
+  // RD rounds over the workgroup.
+  constexpr int NTD = B2 * V;
+  constexpr int RD = (NTD + NT - 1) / NT;
 
 #pragma unroll 1
   for (int iter = 0; iter < MAX_ITER; ++iter) {
@@ -221,8 +232,12 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       for (int m = 0; m < V; ++m) qn[l][m] = T(0);
     // Rolled (large sizes): table entries come from constant memory with a
     // uniform index; unrolled: they are immediates.
-#pragma unroll UNROLL_T
-    for (int t = 0; t < B; ++t) {
+    // One time slice; with SPLIT (rolled loops over slices that live partly
+    // in LDS and partly in the workspace) the LDS slices and the workspace
+    // slices run in two loops, so that within each loop a slice's location is
+    // known and every access is a plain LDS or global one (never a generic
+    // pointer that may address either).
+    auto picard_slice = [&](const int t) __device__ __attribute__((always_inline)) {
       const T wt = tc<T>(exa_gl_weights[N][t]);
       T Q[D];
 #pragma unroll
@@ -253,41 +268,39 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       }
       __syncthreads();
 
-      // phase 2 (line owners): one task = one (direction, variable, line);
-      // out(l) = sum_n (us * F(n)) * Kxi[n][l] along the line, with
-      // us = w_t w_a w_b dt / dx_dir (:152-236).  Kxi comes in as constants.
-      // A task reads its whole line before writing the results back to the same
-      // positions, and no other task touches them, so the contraction is in place.
+      // phase 2 (line owners): per direction d, one task = one (variable,
+      // line); out(l) = sum_n (us * F(n)) * Kxi[n][l] along the line, with
+      // us = w_t w_a w_b dt / dx_d (:152-236).  Directions run one after the
+      // other with compile-time strides; Kxi comes in as constants.  A task
+      // reads its whole line before writing the results back to the same
+      // positions, and no other task touches them: the contraction is in place.
       if (USE_FLUX) {
-        constexpr int NTASK = 3 * B2 * V;
-#pragma unroll 1
-        for (int round = 0; round < (NTASK + NT - 1) / NT; ++round) {
-          // Surplus lanes of the last round compute the last task again but
-          // must not store: the contraction is in place, and a duplicate in
-          // another wavefront could read the line after the owner rewrote it.
-          const int raw = tid + round * NT;
-          const int task = min(raw, NTASK - 1);
-          const int d = task / (B2 * V);
-          const int rem = task - d * (B2 * V);
-          const int m = rem / B2;
-          const int line = rem - m * B2;
-          const int la = line / B, lb = line - (line / B) * B;
-          // x-lines (z,y): base (z B + y) B, stride 1; y-lines (z,x): z B^2 + x, stride B;
-          // z-lines (y,x): y B + x, stride B^2
-          const int base = d == 0 ? line * B : (d == 1 ? la * B2 + lb : line);
-          const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
-          const T us = wt * sW[la] * sW[lb] * dt * sInvDx[d];
-          const T* src = sF + (d * V + m) * B3 + base;
-          T f[B];
 #pragma unroll
-          for (int n = 0; n < B; ++n) f[n] = us * src[n * stride];
-          T* dst = sF + (d * V + m) * B3 + base;
+        for (int r = 0; r < RD; ++r) {
+          if (tid + r * NT < NTD) {
+            const int task = tid + r * NT;
+            const int m = task / B2, line = task - (task / B2) * B2;
+            const int la = line / B, lb = line - (line / B) * B;
+            // line starts: x-lines (z,y) stride 1, y-lines (z,x) stride B,
+            // z-lines (y,x) stride B^2; the weights are w_la, w_lb
+            const int off0 = m * B3 + line * B, off1 = m * B3 + la * B2 + lb, off2 = m * B3 + line;
+            const T wa = sW[la], wb = sW[lb];
 #pragma unroll
-          for (int l = 0; l < B; ++l) {
-            T acc = T(0);
+            for (int d = 0; d < 3; ++d) {
+              const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
+              const int o = d * V * B3 + (d == 0 ? off0 : (d == 1 ? off1 : off2));
+              const T us = wt * wa * wb * dt * invDx[d];
+              T f[B];
 #pragma unroll
-            for (int n = 0; n < B; ++n) acc += f[n] * tc<T>(exa_kxi[N][n][l]);
-            if (raw < NTASK) dst[l * stride] = acc;
+              for (int n = 0; n < B; ++n) f[n] = us * sF[o + n * stride];
+#pragma unroll
+              for (int l = 0; l < B; ++l) {
+                T acc = T(0);
+#pragma unroll
+                for (int n = 0; n < B; ++n) acc += f[n] * tc<T>(exa_kxi[N][n][l]);
+                sF[o + l * stride] = acc;
+              }
+            }
           }
         }
         __syncthreads();
@@ -325,6 +338,15 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
         for (int l = 0; l < B; ++l) qn[l][m] += ir * tc<T>(exa_ik1[N][l][t]);
       }
       if (USE_FLUX || USE_NCP) __syncthreads();  // the next slice rewrites sF / sQ
+    };
+    if constexpr (SPLIT) {
+#pragma unroll 1
+      for (int t = 0; t < NQL; ++t) picard_slice(t);
+#pragma unroll 1
+      for (int t = NQL; t < B; ++t) picard_slice(t);
+    } else {
+#pragma unroll UNROLL_T
+      for (int t = 0; t < B; ++t) picard_slice(t);
     }
 
     // squared residual (:303) and exit test (:314-318)
@@ -358,7 +380,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     for (int l = 0; l < B; ++l)
 #pragma unroll
       for (int m = 0; m < V; ++m)
-        qi(l, m) = qn[l][m];
+        qi_set(l, m, qn[l][m]);
   }
 
   // ---- epilogue: time average, extrapolation, volume integral ----
@@ -421,8 +443,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     fh[2][m] = T(0);
     sh[m] = T(0);
   }
-#pragma unroll UNROLL_T
-  for (int t = 0; t < B; ++t) {
+  auto average_slice = [&](const int t) __device__ __attribute__((always_inline)) {
     T Q[D];
 #pragma unroll
     for (int m = 0; m < V; ++m) Q[m] = qi(t, m);
@@ -449,6 +470,15 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       for (int m = 0; m < V; ++m) sh[m] += S[m] * wt;
       if (USE_NCP) __syncthreads();
     }
+  };
+  if constexpr (SPLIT) {
+#pragma unroll 1
+    for (int t = 0; t < NQL; ++t) average_slice(t);
+#pragma unroll 1
+    for (int t = NQL; t < B; ++t) average_slice(t);
+  } else {
+#pragma unroll UNROLL_T
+    for (int t = 0; t < B; ++t) average_slice(t);
   }
   if (EXPORT) {
     // lFhi blocks: x (z,y,x), y (z,x,y), z (y,x,z), src (z,y,x)  (:375-393)

@@ -46,8 +46,11 @@ struct LimArgs {
   double* mn;         // [cells][6][V] solution minima per face block
   double* mx;         // [cells][6][V]
   double* limmm;      // [cells][2][V] min/max over the FV patch interior
-  int* flags;         // [0] irregular change this step; [1] FV list size; [2] helper list size
+  int* flags;         // [0] irregular change this step; [1] FV list size; [2] recompute list size
   int* fv_list;       // [cells]
+  int* rc_list;       // [cells] cells with status > 0 after an irregular change (rolled back)
+  double* loc_mm;     // [cells][2][V] local min/max of the DG solution after the update
+  double* dt_cell;    // [cells] stable dt candidate of the DG solution after the update
   const double* dx;   // device [3]
   StepState* st_state;  // dt, dt slots
   double cfl_pnpm;
@@ -59,7 +62,7 @@ struct LimArgs {
   unsigned long long* fv_dt_max;
 };
 
-enum : int { kLimFlagIrregular = 0, kLimFlagFvCount = 1 };
+enum : int { kLimFlagIrregular = 0, kLimFlagFvCount = 1, kLimFlagRcCount = 2 };
 
 // The launch_* functions below take the solver order (2..kLimMaxOrder, Euler,
 // FP64).  "all" launches cover every cell; the FV launches walk fv_list.

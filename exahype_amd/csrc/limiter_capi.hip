@@ -82,7 +82,8 @@ struct exa_lgrid {
   double *lim = nullptr, *mn = nullptr, *mx = nullptr, *limmm = nullptr, *d_dx = nullptr;
   uint8_t *cur = nullptr, *has_patch = nullptr;
   int *st = nullptr, *pst = nullptr, *st2 = nullptr, *itc = nullptr, *nbr = nullptr, *flags = nullptr;
-  int *fv_list = nullptr, *iters = nullptr;
+  int *fv_list = nullptr, *iters = nullptr, *rc_list = nullptr;
+  double *loc_mm = nullptr, *dt_cell = nullptr;
   FaceDesc* faces = nullptr;
   long nfaces = 0;
   exa::StepState* state = nullptr;
@@ -115,6 +116,9 @@ exa::LimArgs largs(exa_lgrid* g) {
   a.limmm = g->limmm;
   a.flags = g->flags;
   a.fv_list = g->fv_list;
+  a.rc_list = g->rc_list;
+  a.loc_mm = g->loc_mm;
+  a.dt_cell = g->dt_cell;
   a.dx = g->d_dx;
   a.st_state = g->state;
   a.cfl_pnpm = g->sd.cfl * kPNPM[g->sd.order];
@@ -148,7 +152,8 @@ void lfree(exa_lgrid* g) {
   }
   void* ptrs[] = {g->luh,  g->luh_prev, g->lduh, g->lQhbnd, g->lFhbnd, g->lim,     g->mn,      g->mx,
                   g->limmm, g->d_dx,    g->cur,  g->has_patch, g->st,  g->pst,     g->st2,     g->itc,
-                  g->nbr,  g->flags,    g->fv_list, g->iters, g->faces, g->state, g->ws};
+                  g->nbr,  g->flags,    g->fv_list, g->iters, g->faces, g->state, g->ws,
+                  g->rc_list, g->loc_mm, g->dt_cell};
   for (void* p : ptrs) (void)hipFree(p);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
@@ -297,6 +302,9 @@ extern "C" int exa_lgrid_create(const exa_solver* s, int nx, int ny, int nz, con
   A((void**)&g->nbr, sizeof(int) * ncells * 6);
   A((void**)&g->flags, sizeof(int) * 4);
   A((void**)&g->fv_list, sizeof(int) * ncells);
+  A((void**)&g->rc_list, sizeof(int) * ncells);
+  A((void**)&g->loc_mm, sizeof(double) * ncells * 10);
+  A((void**)&g->dt_cell, sizeof(double) * ncells);
   A((void**)&g->iters, sizeof(int) * ncells);
   A((void**)&g->state, sizeof(exa::StepState));
   if (const size_t wsb = exa::predictor_ws_bytes_per_cell(sd)) {

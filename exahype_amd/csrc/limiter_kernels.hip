@@ -571,12 +571,22 @@ __global__ __launch_bounds__(kLimNT) void lim_update_status_kernel(LimArgs a) {
     double dtn = active ? node_stable_dt<Euler<0>>(q, invDx, a.cfl_pnpm) : DBL_MAX;
     dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
     dtn = block_min<kLimNT>(dtn, red);
-    if (tid == 0) atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
+    if (tid == 0) {
+      atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
+      a.dt_cell[c] = dtn;
+    }
   }
   __syncthreads();
   // DMP (LIM:114-165) against the merged boundary extrema
   double mn[V], mx[V];
   dg_local_minmax<N>(sQ, sLob, sSub, t1, t2, sUh2lim, sUh2lob, red, mn, mx);
+  if (tid == 0) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      a.loc_mm[(size_t)c * 2 * V + v] = mn[v];
+      a.loc_mm[(size_t)c * 2 * V + V + v] = mx[v];
+    }
+  }
   double* bmn = a.mn + (size_t)c * 6 * V;
   double* bmx = a.mx + (size_t)c * 6 * V;
   bool ok = true;
@@ -688,7 +698,10 @@ __global__ __launch_bounds__(256) void lim_spread_kernel(LimArgs a, const int* i
 
 __global__ void lim_reset_fv_count_kernel(int* flags, int only_if_irregular) {
   if (only_if_irregular && !flags[kLimFlagIrregular]) return;
-  if (threadIdx.x == 0) flags[kLimFlagFvCount] = 0;
+  if (threadIdx.x == 0) {
+    flags[kLimFlagFvCount] = 0;
+    flags[kLimFlagRcCount] = 0;
+  }
 }
 
 // reinitialiseSolversLocally (LS:1343-1381): roll cells with status > 0 back
@@ -705,6 +718,7 @@ __global__ __launch_bounds__(kLimNT) void lim_rollback_kernel(LimArgs a) {
   const int c = blockIdx.x;
   const int s = a.st[c];
   if (threadIdx.x == 0 && s >= a.A) a.fv_list[atomicAdd(&a.flags[kLimFlagFvCount], 1)] = c;
+  if (threadIdx.x == 0 && s > 0) a.rc_list[atomicAdd(&a.flags[kLimFlagRcCount], 1)] = c;
   if (s > 0) {
     load_tables<N>(sUh2lim, nullptr, sLim2uh);
     if (a.pst[c] >= a.A) {
@@ -733,9 +747,12 @@ __global__ __launch_bounds__(kLimNT) void lim_rollback_kernel(LimArgs a) {
   }
 }
 
-// LocalRecomputation enterCell (LocalRecomputation.cpp:236-285) for every
-// cell after the FV cells were recomputed: recomputeSolutionLocally of the
-// helper cells (LS:1396-1436), startNewTimeStep, determineMinAndMax.
+// LocalRecomputation enterCell (LocalRecomputation.cpp:236-285) after the FV
+// cells were recomputed: recomputeSolutionLocally of the helper cells
+// (LS:1396-1436), startNewTimeStep, determineMinAndMax.  Cells with status 0
+// were not rolled back: their solution is the one the status kernel saw, so
+// its dt candidate and local min/max are reused (lim_recompute_light_kernel);
+// this kernel walks the cells with status > 0 (the rollback's list).
 template <int N>
 __global__ __launch_bounds__(kLimNT) void lim_recompute_kernel(LimArgs a) {
   using Sh = LimShape<N>;
@@ -745,58 +762,80 @@ __global__ __launch_bounds__(kLimNT) void lim_recompute_kernel(LimArgs a) {
   __shared__ double sUh2lim[B * L], sUh2lob[B * B], sLim2uh[B * L];
   __shared__ double red[2 * V * (kLimNT / kWave)];
   if (!a.flags[kLimFlagIrregular]) return;
-  const int c = blockIdx.x;
   const int tid = threadIdx.x;
-  const int s = a.st[c];
+  const int count = a.flags[kLimFlagRcCount];
   load_tables<N>(sUh2lim, sUh2lob, sLim2uh);
-  __syncthreads();
-  if (s > 0 && s < a.A) {
-    if (a.pst[c] < a.A) {
-      double* luh = a.luh + (size_t)c * B3 * V;
-      double* prev = a.luh_prev + (size_t)c * B3 * V;
-      for (int i = tid; i < B3 * V; i += kLimNT) {
-        const double t = luh[i];
-        luh[i] = prev[i];
-        prev[i] = t;
+  const double invDx[3] = {1.0 / a.dx[0], 1.0 / a.dx[1], 1.0 / a.dx[2]};
+  for (int it = blockIdx.x; it < count; it += gridDim.x) {
+    const int c = a.rc_list[it];
+    const int s = a.st[c];
+    __syncthreads();  // tables loaded / the previous cell's LDS use is over
+    if (s > 0 && s < a.A) {
+      if (a.pst[c] < a.A) {
+        double* luh = a.luh + (size_t)c * B3 * V;
+        double* prev = a.luh_prev + (size_t)c * B3 * V;
+        for (int i = tid; i < B3 * V; i += kLimNT) {
+          const double t = luh[i];
+          luh[i] = prev[i];
+          prev[i] = t;
+        }
+        // the FV projection follows from the min/max pass below (sSub)
+      } else {
+        const int buf = a.cur[c] ^ 1;
+        double mn[V], mx[V];
+        fv_to_dg<N, kLimNT>(patch_ptr(a, c, buf, PATCH), a.luh + (size_t)c * B3 * V, sSub, t1, t2, sQ, sLim2uh, red,
+                            mn, mx);
+        if (tid == 0) a.cur[c] = (uint8_t)buf;
       }
-      // the FV projection follows from the DMP pass below (sSub)
-    } else {
-      const int buf = a.cur[c] ^ 1;
-      double mn[V], mx[V];
-      fv_to_dg<N, kLimNT>(patch_ptr(a, c, buf, PATCH), a.luh + (size_t)c * B3 * V, sSub, t1, t2, sQ, sLim2uh, red, mn,
-                          mx);
-      if (tid == 0) a.cur[c] = (uint8_t)buf;
+    }
+    __syncthreads();
+    const double* luh = a.luh + (size_t)c * B3 * V;
+    for (int i = tid; i < B3 * V; i += kLimNT) sQ[(i % V) * B3 + i / V] = luh[i];
+    __syncthreads();
+    const bool active = tid < B3;
+    const int node = active ? tid : 0;
+    {
+      double q[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) q[m] = sQ[m * B3 + node];
+      double dtn = active ? node_stable_dt<Euler<0>>(q, invDx, a.cfl_pnpm) : DBL_MAX;
+      dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
+      dtn = block_min<kLimNT>(dtn, red);
+      if (tid == 0) atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
+    }
+    __syncthreads();
+    double mn[V], mx[V];
+    dg_local_minmax<N>(sQ, sLob, sSub, t1, t2, sUh2lim, sUh2lob, red, mn, mx);
+    if (s > 0 && s < a.A && a.pst[c] < a.A) store_interior<N>(patch_ptr(a, c, a.cur[c], PATCH), sSub);
+    if (tid == 0) {
+      double* bmn = a.mn + (size_t)c * 6 * V;
+      double* bmx = a.mx + (size_t)c * 6 * V;
+      for (int f = 0; f < 6; ++f)
+        for (int v = 0; v < V; ++v) {
+          bmn[f * V + v] = (s < a.A) ? mn[v] : a.limmm[(size_t)c * 2 * V + v];
+          bmx[f * V + v] = (s < a.A) ? mx[v] : a.limmm[(size_t)c * 2 * V + V + v];
+        }
     }
   }
-  __syncthreads();
-  const double* luh = a.luh + (size_t)c * B3 * V;
-  for (int i = tid; i < B3 * V; i += kLimNT) sQ[(i % V) * B3 + i / V] = luh[i];
-  __syncthreads();
-  const bool active = tid < B3;
-  const int node = active ? tid : 0;
-  {
-    double q[V];
-#pragma unroll
-    for (int m = 0; m < V; ++m) q[m] = sQ[m * B3 + node];
-    const double invDx[3] = {1.0 / a.dx[0], 1.0 / a.dx[1], 1.0 / a.dx[2]};
-    double dtn = active ? node_stable_dt<Euler<0>>(q, invDx, a.cfl_pnpm) : DBL_MAX;
-    dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
-    dtn = block_min<kLimNT>(dtn, red);
-    if (tid == 0) atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
-  }
-  __syncthreads();
-  double mn[V], mx[V];
-  dg_local_minmax<N>(sQ, sLob, sSub, t1, t2, sUh2lim, sUh2lob, red, mn, mx);
-  if (s > 0 && s < a.A && a.pst[c] < a.A) store_interior<N>(patch_ptr(a, c, a.cur[c], PATCH), sSub);
-  if (tid == 0) {
-    double* bmn = a.mn + (size_t)c * 6 * V;
-    double* bmx = a.mx + (size_t)c * 6 * V;
-    for (int f = 0; f < 6; ++f)
-      for (int v = 0; v < V; ++v) {
-        bmn[f * V + v] = (s < a.A) ? mn[v] : a.limmm[(size_t)c * 2 * V + v];
-        bmx[f * V + v] = (s < a.A) ? mx[v] : a.limmm[(size_t)c * 2 * V + V + v];
-      }
-  }
+}
+
+// Cells with status 0 after an irregular change: determineMinAndMax writes
+// the local extrema on all faces, startNewTimeStep the dt candidate, both as
+// the status kernel computed them for the same solution.
+__global__ __launch_bounds__(256) void lim_recompute_light_kernel(LimArgs a) {
+  if (!a.flags[kLimFlagIrregular]) return;
+  constexpr int V = 5;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= a.ncells || a.st[c] != 0) return;
+  const double* lm = a.loc_mm + (size_t)c * 2 * V;
+  double* bmn = a.mn + (size_t)c * 6 * V;
+  double* bmx = a.mx + (size_t)c * 6 * V;
+  for (int f = 0; f < 6; ++f)
+    for (int v = 0; v < V; ++v) {
+      bmn[f * V + v] = lm[v];
+      bmx[f * V + v] = lm[V + v];
+    }
+  atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(a.dt_cell[c]));
 }
 
 // Initial state (LS:1031-1043 and the limiter patches' own initial condition,
@@ -1178,8 +1217,14 @@ hipError_t launch_lim_spread_rollback(const SolverDesc& sd, const LimArgs& a, hi
 }
 
 hipError_t launch_lim_recompute_finish(const SolverDesc& sd, const LimArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(lim_recompute_light_kernel, dim3(blocks256(a.ncells)), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   return dispatch_lim(sd.order, [&]<int N>() -> hipError_t {
-    hipLaunchKernelGGL(lim_recompute_kernel<N>, dim3(a.ncells), dim3(kLimNT), 0, s, a);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const long g = 8L * cus;
+    hipLaunchKernelGGL(lim_recompute_kernel<N>, dim3((unsigned)(g < a.ncells ? g : a.ncells)), dim3(kLimNT), 0, s, a);
     return hipGetLastError();
   });
 }

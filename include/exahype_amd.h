@@ -237,7 +237,11 @@ int exa_lgrid_time(exa_lgrid* g, double* t, double* dt, long* steps);
 int exa_lgrid_download(exa_lgrid* g, double* luh);
 /* Limiter status per cell, and the number of steps with an irregular change. */
 int exa_lgrid_status(exa_lgrid* g, int* status, long* irregular_steps);
-/* Current FV patch of one cell, (L+4)^3 * V. */
+/* Current FV patch of one cell, (L+4)^3 * V.  The interior is the cell's FV
+ * solution; the ghost layers are scratch (refilled from the neighbours before
+ * every update).  The reference's update also adds the boundary-face fluxes
+ * to the first ghost layer of its new patch (musclhancock.cpph:267-329), values
+ * nothing reads before the next refill; the device does not write them. */
 int exa_lgrid_limiter_solution(exa_lgrid* g, long cell, double* lim);
 /* Solution min/max face blocks [cells][6][V]. */
 int exa_lgrid_min_max(exa_lgrid* g, double* mn, double* mx);

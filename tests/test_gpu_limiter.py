@@ -138,10 +138,18 @@ def test_limiting_grid_vs_oracle(exa, oracle, n, periodic, radius, steps):
             err = np.abs(sol[:, v] - osol[:, v]).max() / max(np.abs(osol[:, v]).max(), 1e-300)
             assert err <= 1e-12 or np.abs(sol[:, v] - osol[:, v]).max() <= 1e-14, f"var {v} step {k}: {err:.3e}"
     assert irr_gpu_prev > 0  # the run exercised rollback and recomputation
-    # FV patches of limited cells agree too
+    # FV patches of limited cells agree in their interior.  Ghost layers are
+    # refilled from the neighbours before every update; the reference's update
+    # also adds the boundary-face fluxes to the first ghost layer of its new
+    # patch (musclhancock.cpph:267-329), values nothing reads before the next
+    # refill and the device does not write.
     st, _ = g.status()
+    S = 2 * order + 1 + 4
+    inner = (slice(2, S - 2),) * 3
     for c in np.nonzero(st >= 2)[0][:8]:
-        assert rel_max(g.limiter_solution(int(c)), o.limiter_solution(int(c))) <= 1e-11
+        a = g.limiter_solution(int(c)).reshape(S, S, S, 5)[inner]
+        b = o.limiter_solution(int(c)).reshape(S, S, S, 5)[inner]
+        assert rel_max(a, b) <= 1e-11
     g.close()
 
 

@@ -176,6 +176,24 @@ def test_muscl_constant_state_is_steady(oracle):
     p = 0.4 * (q[4] - 0.5 * rho * (u @ u))
     c = np.sqrt(1.4 * p / rho)
     assert abs(dtm - 0.99 / 3 * (0.1 / L) / (np.abs(u).max() + c)) <= 1e-14 * dtm
+    # the face loops also apply the boundary-face flux to the first ghost layer
+    # of luh_new (musclhancock.cpph:267-329): low ghost q - dt/h F_d(q), high
+    # ghost q + dt/h F_d(q); the second layer and edge/corner ghosts are untouched
+    from refutil import euler_flux
+    c_dt = 1e-3 / (0.1 / L)
+    inn = slice(g, g + L)
+    for d in range(3):
+        F = euler_flux(q, d)
+        lo, hi = [slice(None)] * 3, [slice(None)] * 3
+        for axis in range(3):  # array axes (z, y, x); direction d is axis 2 - d
+            if axis != 2 - d:
+                lo[axis] = hi[axis] = inn
+        lo[2 - d], hi[2 - d] = g - 1, g + L
+        assert np.allclose(new[tuple(lo)], q - c_dt * F, rtol=0, atol=1e-14)
+        assert np.allclose(new[tuple(hi)], q + c_dt * F, rtol=0, atol=1e-14)
+        lo[2 - d], hi[2 - d] = g - 2, g + L + 1
+        assert np.array_equal(new[tuple(lo)], P[tuple(lo)]) and np.array_equal(new[tuple(hi)], P[tuple(hi)])
+    assert np.array_equal(new[0, 0], P[0, 0]) and np.array_equal(new[g - 1, g - 1], P[g - 1, g - 1])
 
 
 def test_muscl_one_dimensional_and_conservative(oracle):

@@ -135,12 +135,12 @@ def cpu_baseline_limiter(n, seconds):
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refutil import blast_cells
-    order, m = 3, 6
+    order, m = 3, 12
     s = O.make_solver(order)
     nodes = np.array([O.table(O.T_NODES, order, i) for i in range(order + 1)])
-    g = O.LimitingGrid(s, m)
-    g.set_solution(blast_cells(order, m, nodes=nodes, radius=0.3),
-                   blast_cells(order, m, subcells=True, radius=0.3))
+    g = O.LimitingGrid(s, m, periodic=False)
+    g.set_solution(blast_cells(order, m, nodes=nodes, radius=0.2),
+                   blast_cells(order, m, subcells=True, radius=0.2))
     steps, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < seconds:
         g.step(1)
@@ -148,7 +148,7 @@ def cpu_baseline_limiter(n, seconds):
     el = time.perf_counter() - t0
     st = g.status()
     return dict(value=m ** 3 * (order + 1) ** 3 * steps / el, unit="DOF-updates/s", cores=1, kind="port",
-                sample=f"{steps} steps of the limiting driver (oracle) on a {m}^3 N=3 blast (r=0.3, periodic), "
+                sample=f"{steps} steps of the limiting driver (oracle) on a {m}^3 N=3 blast (r=0.2, transmissive), "
                        f"{(st >= 2).mean():.2f} of the cells FV-active at the end, 1 thread, {el:.1f} s")
 
 

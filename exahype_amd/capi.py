@@ -203,6 +203,8 @@ def space_time_predictor_nonlinear(s, luh, dx, dt, temps=False):
 
 
 def volume_integral_nonlinear(s, lFhi, dx):
+    """volumeIntegralNonlinear of lFhi (4 B^3 V), or of the space-time fluxes
+    lFi (4 B^4 V) when s.no_time_averaging is set."""
     lduh = np.zeros(s.B ** 3 * s.num_variables)
     check(lib().exa_volume_integral_nonlinear(ctypes.byref(s), _dp(lduh), _dp(_f64(lFhi)), _dp(_dx3(dx))),
           "volumeIntegralNonlinear")

@@ -177,6 +177,8 @@ hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s);
 hipError_t launch_patch_dt_finalize(double* dt, double* t, unsigned long long* slots, int npatches,
                                     int slots_per_patch, bool advance_time, hipStream_t s);
 hipError_t launch_volume_integral(const SolverDesc& sd, const VolumeArgs& a, long ncells, hipStream_t s);
+// One cell: lFhi = the time average of the space-time fluxes lFi.
+hipError_t launch_time_average_lfi(int order, int V, const double* lFi, double* lFhi, hipStream_t s);
 hipError_t launch_euler_ic(const ICArgs& a, hipStream_t s);
 // st->picard_total += sum(iters[0..ncells)), st->predictor_cells += ncells
 hipError_t launch_count_iterations(const int* iters, long ncells, StepState* st, hipStream_t s);

@@ -215,6 +215,10 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   const T iweight = T(1.0) / w3;
   __syncthreads();
 
+  // Float form: the quadrature weights and dt/dx of the line contraction are
+  // applied per node in phase 3 instead of per line value in phase 2:
+  // us_d / w^3 = w_t dt / (dx_d w_d), w_d the node's weight along d.
+  const T g_node[3] = {dt * invDx[0] / wx, dt * invDx[1] / wy, dt * invDx[2] / wz};
   // Phase 2 tasks per direction: one (variable m, line) each, NTD of them in
   // RD rounds over the workgroup.
   constexpr int NTD = B2 * V;
@@ -226,6 +230,12 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     // across iterations costs more registers than recomputing it per slice.
     T w3i = w3;
     asm volatile("" : "+v"(w3i));
+    T q0i[V];  // the same for the float form's FLCoeff[t] luh
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      q0i[m] = q0[m];
+      asm volatile("" : "+v"(q0i[m]));
+    }
 #pragma unroll
     for (int l = 0; l < B; ++l)
 #pragma unroll
@@ -289,10 +299,15 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
             for (int d = 0; d < 3; ++d) {
               const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
               const int o = d * V * B3 + (d == 0 ? off0 : (d == 1 ? off1 : off2));
-              const T us = wt * wa * wb * dt * invDx[d];
               T f[B];
+              if constexpr (IS_FP64) {
+                const T us = wt * wa * wb * dt * invDx[d];
 #pragma unroll
-              for (int n = 0; n < B; ++n) f[n] = us * sF[o + n * stride];
+                for (int n = 0; n < B; ++n) f[n] = us * sF[o + n * stride];
+              } else {
+#pragma unroll
+                for (int n = 0; n < B; ++n) f[n] = sF[o + n * stride];
+              }
 #pragma unroll
               for (int l = 0; l < B; ++l) {
                 T acc = T(0);
@@ -308,8 +323,35 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 
       // phase 3 (node owners): rhs = w^3 FLCoeff[t] luh - contributions (+ S dt w^4)
       // (:124-141, :238-276); lQi_new(l) += iweight * rhs * iK1[l][t] (:292-299, n = t)
-      T r[V];
       const T fl_t = tc<T>(exa_fl_coeff[N][t]);
+      if constexpr (!IS_FP64) {
+        // float form: ir = rhs / w^3 directly, weights folded per node
+        T ir[V];
+        const T wg[3] = {wt * g_node[0], wt * g_node[1], wt * g_node[2]};
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+          ir[m] = fl_t * q0i[m];
+          if (USE_FLUX) {
+            ir[m] -= wg[0] * sF[(0 * V + m) * B3 + node];
+            ir[m] -= wg[1] * sF[(1 * V + m) * B3 + node];
+            ir[m] -= wg[2] * sF[(2 * V + m) * B3 + node];
+          }
+        }
+        if (USE_SRC_NCP) {
+          T S[V];
+          source_ncp<B, PDE, USE_SOURCE, USE_NCP>(Q, sQ, sDu, invDx, z, y, x, S);
+          const T us = wt * dt;
+#pragma unroll
+          for (int m = 0; m < V; ++m) ir[m] += us * S[m];
+        }
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+#pragma unroll
+          for (int l = 0; l < B; ++l) qn[l][m] += ir[m] * tc<T>(exa_ik1[N][l][t]);
+        if (USE_FLUX || USE_NCP) __syncthreads();  // the next slice rewrites sF / sQ
+        return;
+      }
+      T r[V];
 #pragma unroll
       for (int m = 0; m < V; ++m) {
         r[m] = w3i * fl_t * q0[m];
@@ -410,25 +452,38 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     for (int m = 0; m < D; ++m) sF[m * B3 + node] = qh[m];
   }
   __syncthreads();
-  // lQhbnd (:434-527): face f, face node (a,b), datum k; reduce the normal line.
-#pragma unroll 1
-  for (int round = 0; round < (6 * B2 * D + NT - 1) / NT; ++round) {
-    const int o = min(tid + round * NT, 6 * B2 * D - 1);
-    const int f = o / (B2 * D);
-    const int rem = o - f * (B2 * D);
-    const int fa = rem / (B * D);
-    const int fb = (rem / D) % B;
-    const int k = rem % D;
-    const int dir = f >> 1;
-    const double* coef = (f & 1) ? exa_fr_coeff[N] : exa_fl_coeff[N];
-    // x-faces: (z=a,y=b,x=l); y-faces: (z=a,y=l,x=b); z-faces: (z=l,y=a,x=b)
-    const int base = dir == 0 ? (fa * B + fb) * B : (dir == 1 ? fa * B2 + fb : fa * B + fb);
-    const int stride = dir == 0 ? 1 : (dir == 1 ? B : B2);
-    T acc = T(0);
+  // Face extrapolation (lQhbnd / lFhbnd, :434-527) as line tasks: per
+  // direction d, one task = (face node (la, lb), datum k), k fastest so that
+  // the stores of a wavefront are contiguous; a task reads its normal line
+  // once and forms both faces of the direction (FL and FR coefficients, as
+  // immediates).  Source values: sF[k][node] for every direction, or with
+  // PER_DIR the direction's own block sF[d][k][node].
+  auto extrapolate = [&]<int NK, bool PER_DIR>(T* out, size_t coff) __device__ {
+    constexpr int NTK = B2 * NK;
 #pragma unroll
-    for (int l = 0; l < B; ++l) acc += sF[k * B3 + base + l * stride] * T(coef[l]);
-    out_lQhbnd[cQb + o] = acc;
-  }
+    for (int r = 0; r < (NTK + NT - 1) / NT; ++r) {
+      const int task = tid + r * NT;
+      if (task < NTK) {
+        const int k = task % NK, line = task / NK;
+        const int la = line / B, lb = line - (line / B) * B;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
+          const int o = (PER_DIR ? d * V + k : k) * B3 + (d == 0 ? line * B : (d == 1 ? la * B2 + lb : line));
+          T accL = T(0), accR = T(0);
+#pragma unroll
+          for (int l = 0; l < B; ++l) {
+            const T v = sF[o + l * stride];
+            accL += v * tc<T>(exa_fl_coeff[N][l]);
+            accR += v * tc<T>(exa_fr_coeff[N][l]);
+          }
+          out[coff + (size_t)(2 * d) * NTK + task] = accL;
+          out[coff + (size_t)(2 * d + 1) * NTK + task] = accR;
+        }
+      }
+    }
+  };
+  extrapolate.template operator()<D, false>(out_lQhbnd, cQb);
   __syncthreads();
 
   // Time-averaged fluxes of the last flux evaluation, i.e. of lQi^k = qi (the
@@ -498,23 +553,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   __syncthreads();
   // lFhbnd from lFhi_{x,y,z} (:434-527)
   if (USE_FLUX) {
-#pragma unroll 1
-    for (int round = 0; round < (6 * B2 * V + NT - 1) / NT; ++round) {
-      const int o = min(tid + round * NT, 6 * B2 * V - 1);
-      const int f = o / (B2 * V);
-      const int rem = o - f * (B2 * V);
-      const int fa = rem / (B * V);
-      const int fb = (rem / V) % B;
-      const int k = rem % V;
-      const int dir = f >> 1;
-      const double* coef = (f & 1) ? exa_fr_coeff[N] : exa_fl_coeff[N];
-      const int base = dir == 0 ? (fa * B + fb) * B : (dir == 1 ? fa * B2 + fb : fa * B + fb);
-      const int stride = dir == 0 ? 1 : (dir == 1 ? B : B2);
-      T acc = T(0);
-#pragma unroll
-      for (int l = 0; l < B; ++l) acc += sF[(dir * V + k) * B3 + base + l * stride] * T(coef[l]);
-      out_lFhbnd[cFb + o] = acc;
-    }
+    extrapolate.template operator()<V, true>(out_lFhbnd, cFb);
   } else {
 #pragma unroll 1
     for (int round = 0; round < (6 * B2 * V + NT - 1) / NT; ++round)
@@ -525,7 +564,43 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     T acc[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) acc[m] = T(0);
-    if (USE_FLUX) {
+    if constexpr (!IS_FP64) {
+      // float form: the directional contractions sum_n Kxi[l][n] lFhi_d(n) as
+      // line tasks in place (as in phase 2), then each node sums its three
+      // directions with their weights
+      if (USE_FLUX) {
+        __syncthreads();  // the extrapolation has read sF
+#pragma unroll
+        for (int r = 0; r < RD; ++r) {
+          if (tid + r * NT < NTD) {
+            const int task = tid + r * NT;
+            const int m = task / B2, line = task - (task / B2) * B2;
+            const int la = line / B, lb = line - (line / B) * B;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
+              const int o = (d * V + m) * B3 + (d == 0 ? line * B : (d == 1 ? la * B2 + lb : line));
+              T f[B];
+#pragma unroll
+              for (int n = 0; n < B; ++n) f[n] = sF[o + n * stride];
+#pragma unroll
+              for (int l = 0; l < B; ++l) {
+                T c = T(0);
+#pragma unroll
+                for (int n = 0; n < B; ++n) c += f[n] * tc<T>(exa_kxi[N][l][n]);
+                sF[o + l * stride] = c;
+              }
+            }
+          }
+        }
+        __syncthreads();
+        const T us[3] = {wz * wy * invDx[0], wz * wx * invDx[1], wy * wx * invDx[2]};
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+          acc[m] = us[0] * sF[(0 * V + m) * B3 + node] + us[1] * sF[(1 * V + m) * B3 + node] +
+                   us[2] * sF[(2 * V + m) * B3 + node];
+      }
+    } else if (USE_FLUX) {
       {
         const T us = wz * wy * invDx[0];
         const T* row = sF + (0 * V) * B3 + (z * B + y) * B;
@@ -1054,6 +1129,36 @@ hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s) {
   return hipGetLastError();
 }
 
+// lFhi (x (z,y,x), y (z,x,y), z (y,x,z), src (z,y,x)) = sum_t w_t lFi(t,z,y,x,{x,y,z,src})
+// (aderPredictorNonlinear, spaceTimePredictorNonlinear.cpph:367-396): the
+// time average the noTimeAveraging form leaves to the volume integral.
+template <int B>
+__global__ __launch_bounds__(node_threads(B)) void time_average_lfi_kernel(const double* lFi, double* lFhi, int V) {
+  constexpr int N = B - 1, B2 = B * B, B3 = B2 * B;
+  const int node = threadIdx.x;
+  if (node >= B3) return;
+  const int z = node / B2, y = (node / B) % B, x = node % B;
+  const int dst[4] = {(z * B + y) * B + x, (z * B + x) * B + y, (y * B + x) * B + z, (z * B + y) * B + x};
+  for (int blk = 0; blk < 4; ++blk)
+    for (int m = 0; m < V; ++m) {
+      double acc = 0.0;
+      for (int t = 0; t < B; ++t) acc += lFi[(((size_t)t * B3 + node) * 4 + blk) * V + m] * exa_gl_weights[N][t];
+      lFhi[(size_t)blk * B3 * V + dst[blk] * V + m] = acc;
+    }
+}
+
+hipError_t launch_time_average_lfi(int order, int V, const double* lFi, double* lFhi, hipStream_t s) {
+  switch (order + 1) {
+#define EXA_TA(Bv) \
+  case Bv:         \
+    hipLaunchKernelGGL(time_average_lfi_kernel<Bv>, dim3(1), dim3(node_threads(Bv)), 0, s, lFi, lFhi, V); break;
+    EXA_TA(3) EXA_TA(4) EXA_TA(5) EXA_TA(6) EXA_TA(7) EXA_TA(8)
+#undef EXA_TA
+    default: return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
 template <int B, int V>
 static hipError_t launch_volume_t(int terms, const VolumeArgs& a, long ncells, hipStream_t s) {
   const dim3 grid((unsigned)ncells), block(node_threads(B));
@@ -1092,12 +1197,14 @@ hipError_t launch_count_iterations(const int* iters, long ncells, StepState* st,
   return hipGetLastError();
 }
 
-bool supported_sizes(const SolverDesc& sd) {
-  return sd.nvar == 5 && sd.order >= 2 && sd.order <= 7 && !sd.no_time_averaging;
-}
+// noTimeAveraging (spaceTimePredictorNonlinear.cpph:690-695) selects where
+// the time average is taken: the reference extrapolates and integrates the
+// space-time data directly; the device averages first.  The operations are
+// the same linear maps summed in another order, so both settings run the same
+// kernels.
+bool supported_sizes(const SolverDesc& sd) { return sd.nvar == 5 && sd.order >= 2 && sd.order <= 7; }
 
 bool supported(const SolverDesc& sd) {
-  if (sd.no_time_averaging) return false;
   if (sd.pde == kPdeElastic)
     return sd.nvar == Elastic::V && sd.npar == Elastic::P && sd.terms == kNCP && sd.order >= 2 && sd.order <= 5 &&
            (sd.precision == kFP64 || sd.precision == kFP32Bulk);

@@ -125,11 +125,17 @@ extern "C" int exa_volume_integral_nonlinear(const exa_solver* s, double* lduh, 
   if (!to_desc(s, &sd) || !lduh || !lFhi || !dx) return EXA_E_INVALID;
   if (!exa::supported_sizes(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
-  DevBuf d_du, d_fhi, d_dx;
+  DevBuf d_du, d_fhi, d_dx, d_fi;
   EXA_TRY(d_du.alloc(sizeof(double) * z.B3 * z.V));
   EXA_TRY(d_fhi.alloc(sizeof(double) * 4 * z.B3 * z.V));
   EXA_TRY(d_dx.alloc(sizeof(double) * 3));
-  EXA_TRY(hipMemcpy(d_fhi.p, lFhi, sizeof(double) * 4 * z.B3 * z.V, hipMemcpyHostToDevice));
+  if (sd.no_time_averaging) {  // the argument is lFi: average it over time first
+    EXA_TRY(d_fi.alloc(sizeof(double) * 4 * z.B3 * z.B * z.V));
+    EXA_TRY(hipMemcpy(d_fi.p, lFhi, sizeof(double) * 4 * z.B3 * z.B * z.V, hipMemcpyHostToDevice));
+    EXA_TRY(exa::launch_time_average_lfi(sd.order, z.V, d_fi.d(), d_fhi.d(), nullptr));
+  } else {
+    EXA_TRY(hipMemcpy(d_fhi.p, lFhi, sizeof(double) * 4 * z.B3 * z.V, hipMemcpyHostToDevice));
+  }
   EXA_TRY(hipMemcpy(d_dx.p, dx, sizeof(double) * 3, hipMemcpyHostToDevice));
   exa::VolumeArgs a{d_du.d(), d_fhi.d(), d_dx.d(), 0};
   EXA_TRY(exa::launch_volume_integral(sd, a, 1, nullptr));

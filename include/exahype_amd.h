@@ -82,7 +82,9 @@ int exa_space_time_predictor_nonlinear(const exa_solver* s, double* lQhbnd, doub
                                        const double* luh, const double* dx, double dt,
                                        int* picard_iterations);
 
-/* volumeIntegralNonlinear (K/c/3d/volumeIntegralNonlinear.cpph:31), time-averaged form. */
+/* volumeIntegralNonlinear (K/c/3d/volumeIntegralNonlinear.cpph:31).  lFhi is the
+ * time-averaged flux (4 B^3 V); with s->no_time_averaging the reference passes the
+ * space-time fluxes lFi (4 B^4 V) instead, and so does this argument. */
 int exa_volume_integral_nonlinear(const exa_solver* s, double* lduh, const double* lFhi,
                                   const double* dx);
 

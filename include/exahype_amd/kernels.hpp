@@ -103,14 +103,16 @@ void spaceTimePredictorNonlinear(SolverType& /*solver*/, double* lQhbnd, double*
 
 template <bool useSourceOrNCP, bool useFlux, bool noTimeAveraging, int numberOfVariables, int basisSize,
           typename DxVector>
-void volumeIntegralNonlinear(double* lduh, const double* const /*lFi*/, const double* const lFhi,
+void volumeIntegralNonlinear(double* lduh, const double* const lFi, const double* const lFhi,
                              const DxVector& dx) {
   const exa_solver s = exahype_amd::size_descriptor(
       numberOfVariables, basisSize, (useFlux ? EXA_TERM_FLUX : 0) | (useSourceOrNCP ? EXA_TERM_SOURCE : 0),
       noTimeAveraging);
   double d[3];
   exahype_amd::dx3(dx, d);
-  exahype_amd::check(exa_volume_integral_nonlinear(&s, lduh, lFhi, d), "volumeIntegralNonlinear");
+  // the noTimeAveraging form integrates the space-time fluxes lFi
+  exahype_amd::check(exa_volume_integral_nonlinear(&s, lduh, noTimeAveraging ? lFi : lFhi, d),
+                     "volumeIntegralNonlinear");
 }
 
 template <int numberOfVariables, int basisSize, typename DxVector>

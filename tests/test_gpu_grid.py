@@ -183,3 +183,23 @@ def test_transmissive_grid_vs_oracle(exa, oracle, order, n, steps):
     assert rel_per_dof(gpu.reshape(-1, 5)[:, [0, 4]], cpu.reshape(-1, 5)[:, [0, 4]]) <= TOL
     assert rel_max(gpu, cpu) <= TOL
     g.close()
+
+
+def test_grid_no_time_averaging_vs_oracle(exa, oracle):
+    """A grid whose solver sets noTimeAveraging, against the oracle's own
+    noTimeAveraging branch, 5 steps."""
+    order, n, steps = 3, 5, 5
+    print(f"case: grid {n}^3 N={order} noTimeAveraging {steps} steps", flush=True)
+    s = exa.euler_solver(order)
+    s.no_time_averaging = 1
+    g = exa.Grid(s, n)
+    g.init_euler(amp=0.2, nmodes=8, mode_amp=0.01, seed=9)
+    luh0 = g.solution()
+    g.step(steps)
+    g.synchronize()
+    o = oracle.Grid(oracle.make_solver(order, 5, 0, no_time_averaging=True), n)
+    o.set_solution(luh0)
+    odts, _ = o.step(steps, nthreads=0)
+    assert rel_per_dof(g.solution(), o.solution()) < TOL
+    assert np.abs(g.dt_history(steps) - odts).max() <= 1e-13 * odts.max()
+    g.close()

@@ -141,3 +141,24 @@ def test_predictor_temporaries_vs_oracle(exa, oracle):
     B = 6
     assert rel_max(r["lFi"].reshape(B ** 4, 4, 5)[:, :3], o["lFi"].reshape(B ** 4, 4, 5)[:, :3]) < TOL
     assert rel_max(r["lFhi"][: 3 * B ** 3 * 5], o["lFhi"][: 3 * B ** 3 * 5]) < TOL
+
+
+@pytest.mark.parametrize("order", [3, 5])
+def test_no_time_averaging_vs_oracle(exa, oracle, order):
+    """noTimeAveraging: the device averages in time first; the reference's
+    branch integrates the space-time data -- the same linear maps, parity to
+    the usual bar; the volume integral takes lFi in this form."""
+    print(f"case: noTimeAveraging single-cell kernels N={order}", flush=True)
+    rng = np.random.default_rng(90 + order)
+    luh = smooth_euler_cells(rng, 1, order, scale=0.2)
+    dx, dt = np.array([0.05, 0.04, 0.03]), 1e-3
+    s = exa.euler_solver(order)
+    s.no_time_averaging = 1
+    so = oracle.make_solver(order, 5, 0, no_time_averaging=True)
+    r = exa.space_time_predictor_nonlinear(s, luh, dx, dt, temps=True)
+    o = oracle.space_time_predictor(so, luh, dx, dt)
+    assert r["iters"] == o["iters"]
+    for k in ("lQhbnd", "lFhbnd", "lQi"):
+        assert rel_max(r[k], o[k]) < TOL, k
+    lFi = o["lFi"]
+    assert rel_max(exa.volume_integral_nonlinear(s, lFi, dx), oracle.volume_integral(so, o["lFhi"], dx, lFi=lFi)) < TOL

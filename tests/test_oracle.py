@@ -297,3 +297,24 @@ def test_driver_conserves_and_converges(oracle):
         assert np.all(dts > 0)
     assert errs[0] < 2e-2
     assert errs[0] / errs[1] > 2 ** 3.5, errs
+
+
+@pytest.mark.parametrize("order", [2, 3, 5])
+def test_no_time_averaging_branch_matches_the_averaged_one(oracle, order):
+    """noTimeAveraging (spaceTimePredictorNonlinear.cpph:690-713 and the lFi
+    volume integral, volumeIntegralNonlinear.cpph:44-135) applies the same
+    linear maps as the time-averaged branch in another order: the face data
+    and the volume integral agree to rounding, the Picard loop is shared."""
+    rng = np.random.default_rng(40 + order)
+    luh = random_euler_cells(rng, 1, order)
+    dx, dt = np.array([0.1, 0.08, 0.06]), 2e-3
+    a = oracle.space_time_predictor(oracle.make_solver(order, 5, 0), luh, dx, dt)
+    s_nt = oracle.make_solver(order, 5, 0, no_time_averaging=True)
+    b = oracle.space_time_predictor(s_nt, luh, dx, dt)
+    assert a["iters"] == b["iters"]
+    assert np.array_equal(a["lQi"], b["lQi"]) and np.array_equal(a["lFi"], b["lFi"])
+    for k in ("lQhbnd", "lFhbnd"):
+        assert np.abs(a[k] - b[k]).max() <= 1e-13 * np.abs(a[k]).max(), k
+    va = oracle.volume_integral(oracle.make_solver(order, 5, 0), a["lFhi"], dx)
+    vb = oracle.volume_integral(s_nt, np.zeros_like(a["lFhi"]), dx, lFi=a["lFi"])
+    assert np.abs(va - vb).max() <= 1e-13 * np.abs(va).max()

@@ -224,6 +224,50 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   constexpr int NTD = B2 * V;
   constexpr int RD = (NTD + NT - 1) / NT;
 
+  // Phase 2 of a time slice (line owners): per direction d, one task = one
+  // (variable, line); out(l) = sum_n (us * F(n)) * Kxi[n][l] along the line,
+  // with us = w_t w_a w_b dt / dx_d (:152-236) -- in the float form us is
+  // applied per node in phase 3 instead.  Directions run one after the other
+  // with compile-time strides; Kxi comes in as constants.  A task reads its
+  // whole line before writing the results back to the same positions, and no
+  // other task touches them: the contraction is in place.  Ends with a barrier.
+  auto contract_lines = [&](const T wt) __device__ __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      if (tid + r * NT < NTD) {
+        const int task = tid + r * NT;
+        const int m = task / B2, line = task - (task / B2) * B2;
+        const int la = line / B, lb = line - (line / B) * B;
+        // line starts: x-lines (z,y) stride 1, y-lines (z,x) stride B,
+        // z-lines (y,x) stride B^2; the weights are w_la, w_lb
+        const int off0 = m * B3 + line * B, off1 = m * B3 + la * B2 + lb, off2 = m * B3 + line;
+        const T wa = sW[la], wb = sW[lb];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
+          const int o = d * V * B3 + (d == 0 ? off0 : (d == 1 ? off1 : off2));
+          T f[B];
+          if constexpr (IS_FP64) {
+            const T us = wt * wa * wb * dt * invDx[d];
+#pragma unroll
+            for (int n = 0; n < B; ++n) f[n] = us * sF[o + n * stride];
+          } else {
+#pragma unroll
+            for (int n = 0; n < B; ++n) f[n] = sF[o + n * stride];
+          }
+#pragma unroll
+          for (int l = 0; l < B; ++l) {
+            T acc = T(0);
+#pragma unroll
+            for (int n = 0; n < B; ++n) acc += f[n] * tc<T>(exa_kxi[N][n][l]);
+            sF[o + l * stride] = acc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  };
+
 #pragma unroll 1
   for (int iter = 0; iter < MAX_ITER; ++iter) {
     // rhs0 = w^3 FLCoeff[t] luh is loop invariant; keeping B*V copies of it live
@@ -254,6 +298,13 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       for (int m = 0; m < V; ++m) Q[m] = qi(t, m);
 #pragma unroll
       for (int m = V; m < D; ++m) Q[m] = q0[m];
+      // Float form, first iteration: every slice of the initial iterate equals
+      // luh, and phase 2 does not depend on t there, so the fluxes and line
+      // contractions of slice 0 (still in LDS) serve every slice (the same
+      // inputs give the same values).  The double form keeps w_t inside the
+      // contraction, as the reference, and evaluates every slice.
+      const bool reuse = !IS_FP64 && iter == 0 && t > 0;
+      if (!reuse) {
       // phase 1 (node owners): fluxes of this time slice -> sF[dir][var][node]
       if (USE_FLUX) {
         T F[3][V];
@@ -278,48 +329,8 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       }
       __syncthreads();
 
-      // phase 2 (line owners): per direction d, one task = one (variable,
-      // line); out(l) = sum_n (us * F(n)) * Kxi[n][l] along the line, with
-      // us = w_t w_a w_b dt / dx_d (:152-236).  Directions run one after the
-      // other with compile-time strides; Kxi comes in as constants.  A task
-      // reads its whole line before writing the results back to the same
-      // positions, and no other task touches them: the contraction is in place.
-      if (USE_FLUX) {
-#pragma unroll
-        for (int r = 0; r < RD; ++r) {
-          if (tid + r * NT < NTD) {
-            const int task = tid + r * NT;
-            const int m = task / B2, line = task - (task / B2) * B2;
-            const int la = line / B, lb = line - (line / B) * B;
-            // line starts: x-lines (z,y) stride 1, y-lines (z,x) stride B,
-            // z-lines (y,x) stride B^2; the weights are w_la, w_lb
-            const int off0 = m * B3 + line * B, off1 = m * B3 + la * B2 + lb, off2 = m * B3 + line;
-            const T wa = sW[la], wb = sW[lb];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
-              const int o = d * V * B3 + (d == 0 ? off0 : (d == 1 ? off1 : off2));
-              T f[B];
-              if constexpr (IS_FP64) {
-                const T us = wt * wa * wb * dt * invDx[d];
-#pragma unroll
-                for (int n = 0; n < B; ++n) f[n] = us * sF[o + n * stride];
-              } else {
-#pragma unroll
-                for (int n = 0; n < B; ++n) f[n] = sF[o + n * stride];
-              }
-#pragma unroll
-              for (int l = 0; l < B; ++l) {
-                T acc = T(0);
-#pragma unroll
-                for (int n = 0; n < B; ++n) acc += f[n] * tc<T>(exa_kxi[N][n][l]);
-                sF[o + l * stride] = acc;
-              }
-            }
-          }
-        }
-        __syncthreads();
-      }
+      if (USE_FLUX) contract_lines(wt);
+      }  // !reuse
 
       // phase 3 (node owners): rhs = w^3 FLCoeff[t] luh - contributions (+ S dt w^4)
       // (:124-141, :238-276); lQi_new(l) += iweight * rhs * iK1[l][t] (:292-299, n = t)
@@ -640,19 +651,32 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
 // ---------------------------------------------------------------------------
 // Rusanov flux on a list of faces.  One wavefront per face, lane = face node.
 // ---------------------------------------------------------------------------
+// Lanes per face in the Riemann kernel: several faces share a wavefront for
+// B^2 <= 16 (measured faster at N=3); from N=4 on one face per wavefront
+// (packing two N=4 faces was measured slower).
+__host__ __device__ constexpr int riemann_segment(int B) { return B * B <= 4 ? 4 : (B * B <= 16 ? 16 : 64); }
+
 template <int B, class PDE, bool USE_NCP, typename T>
 __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
   constexpr int B2 = B * B;
-  static_assert(B2 <= kWave, "one face per wavefront");
-  const long f = (long)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
-  if (f >= a.nfaces) return;  // whole wavefront exits together
+  static_assert(B2 <= kWave, "one face per wavefront at most");
+  // faces per wavefront: aligned segments of SEG lanes, one face node per lane
+  constexpr int SEG = riemann_segment(B);
+  constexpr int FPW = kWave / SEG;
   const int lane = threadIdx.x & (kWave - 1);
-  const bool active = lane < B2;
-  const int node = active ? lane : 0;
-  const FaceDesc fd = a.faces[f];
-  if (a.status && a.status[fd.left] >= a.status_limit && a.status[fd.right] >= a.status_limit) return;
+  const long f0 = ((long)blockIdx.x * (256 / kWave) + threadIdx.x / kWave) * FPW;
+  if (f0 >= a.nfaces) return;  // whole wavefront exits together
+  const long f = f0 + lane / SEG;
+  const int sl = lane & (SEG - 1);
+  // faces beyond the list, or between two FV-active limiter cells, take part
+  // in the segment sums with zeros and store nothing
+  bool live = f < a.nfaces;
+  const FaceDesc fd = a.faces[live ? f : f0];
+  if (live && a.status && a.status[fd.left] >= a.status_limit && a.status[fd.right] >= a.status_limit) live = false;
+  const bool active = live && sl < B2;
+  const int node = sl < B2 ? sl : 0;
   const int d = fd.dir;
   T* FL = static_cast<T*>(a.lFhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * V);
   T* FR = static_cast<T*>(a.lFhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
@@ -665,8 +689,8 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
   for (int k = 0; k < D; ++k) {
     qL[k] = QL[node * D + k];
     qR[k] = QR[node * D + k];
-    QavL[k] = wave_sum(active ? weight * qL[k] : T(0));  // :49-66
-    QavR[k] = wave_sum(active ? weight * qR[k] : T(0));
+    QavL[k] = segment_sum<SEG>(active ? weight * qL[k] : T(0));  // :49-66
+    QavR[k] = segment_sum<SEG>(active ? weight * qR[k] : T(0));
   }
   const T sL = PDE::max_abs_eigenvalue(QavL, d);  // :68-78
   const T sR = PDE::max_abs_eigenvalue(QavR, d);
@@ -1051,7 +1075,8 @@ size_t predictor_ws_bytes_per_cell(const SolverDesc& sd, int lds_variant) {
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s) {
   if (a.nfaces <= 0) return hipSuccess;
   return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
-    const dim3 grid((unsigned)((a.nfaces + 3) / 4)), block(256);
+    const long per_block = 4L * (kWave / riemann_segment(B));  // 4 wavefronts
+    const dim3 grid((unsigned)((a.nfaces + per_block - 1) / per_block)), block(256);
     const bool ncp = (sd.terms & kNCP) != 0;
     if constexpr (std::is_same_v<PDE, Elastic>) {
       if (sd.precision == kFP32Bulk)

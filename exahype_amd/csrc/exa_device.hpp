@@ -39,6 +39,16 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 
+// Sum over aligned segments of SEG lanes (SEG a power of two up to 64), the
+// same butterfly restricted to the segment: for data that is zero outside the
+// segment's first lanes it equals wave_sum bit for bit.
+template <int SEG, typename T>
+__device__ inline T segment_sum(T v) {
+#pragma unroll
+  for (int off = SEG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
 template <typename T>
 __device__ inline T wave_min(T v) {
 #pragma unroll

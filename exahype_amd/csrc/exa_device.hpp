@@ -39,6 +39,14 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 
+// Make a wavefront's LDS writes visible to all of its lanes (no workgroup
+// barrier: other wavefronts of the workgroup do not take part).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sum over aligned segments of SEG lanes (SEG a power of two up to 64), the
 // same butterfly restricted to the segment: for data that is zero outside the
 // segment's first lanes it equals wave_sum bit for bit.

@@ -849,91 +849,6 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
   }
 }
 
-// Surface integral + solution update + stable dt (the SURFACE | UPDATE form of
-// update_kernel, same arithmetic in the same order) with the cell's luh, lduh
-// and face-flux blocks moved between HBM and LDS by contiguous lanes; the
-// node owners then work from LDS.
-template <int B, class PDE, typename TF>
-__global__ __launch_bounds__(node_threads(B)) void update_staged_kernel(UpdateArgs a) {
-  constexpr int V = PDE::V, P = PDE::P, D = V + P;
-  constexpr int N = B - 1;
-  constexpr int B2 = B * B, B3 = B2 * B;
-  constexpr int NT = node_threads(B);
-  __shared__ double sQ[B3 * D];
-  __shared__ TF sDu[B3 * V];
-  __shared__ TF sF[6 * B2 * V];
-  __shared__ double sRed[NT / kWave];
-  const size_t cell = (size_t)blockIdx.x + a.cell0;
-  const int tid = threadIdx.x;
-  const bool active = tid < B3;
-  const int node = active ? tid : 0;
-  const int z = node / B2, y = (node / B) % B, x = node % B;
-  const int patch = a.cell_patch ? a.cell_patch[cell] : 0;
-  const double* dxp = a.cell_patch ? a.dx + (size_t)patch * 3 : a.dx + cell * a.dx_stride;
-  const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
-  double* const luh = a.luh + cell * (size_t)(B3 * D);
-  const TF* const lduh = static_cast<const TF*>(a.lduh) + cell * (size_t)(B3 * V);
-  const TF* const lF = static_cast<const TF*>(a.lFhbnd) + cell * (size_t)(6 * B2 * V);
-  for (int e = tid; e < B3 * D; e += NT) sQ[e] = luh[e];
-  for (int e = tid; e < B3 * V; e += NT) sDu[e] = lduh[e];
-  for (int e = tid; e < 6 * B2 * V; e += NT) sF[e] = lF[e];
-  __syncthreads();
-  double q[D];
-#pragma unroll
-  for (int m = 0; m < D; ++m) q[m] = sQ[node * D + m];
-  {
-    const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
-    double du[V];
-#pragma unroll
-    for (int m = 0; m < V; ++m) du[m] = sDu[node * V + m];
-    {  // x faces, (z,y)
-      const double us = wz * wy * invDx[0];
-      const double fr = exa_fr_coeff[N][x], fl = exa_fl_coeff[N][x];
-#pragma unroll
-      for (int m = 0; m < V; ++m)
-        du[m] -= ((double)sF[(1 * B2 + z * B + y) * V + m] * fr - (double)sF[(0 * B2 + z * B + y) * V + m] * fl) * us;
-    }
-    {  // y faces, (z,x)
-      const double us = wz * wx * invDx[1];
-      const double fr = exa_fr_coeff[N][y], fl = exa_fl_coeff[N][y];
-#pragma unroll
-      for (int m = 0; m < V; ++m)
-        du[m] -= ((double)sF[(3 * B2 + z * B + x) * V + m] * fr - (double)sF[(2 * B2 + z * B + x) * V + m] * fl) * us;
-    }
-    {  // z faces, (y,x)
-      const double us = wy * wx * invDx[2];
-      const double fr = exa_fr_coeff[N][z], fl = exa_fl_coeff[N][z];
-#pragma unroll
-      for (int m = 0; m < V; ++m)
-        du[m] -= ((double)sF[(5 * B2 + y * B + x) * V + m] * fr - (double)sF[(4 * B2 + y * B + x) * V + m] * fl) * us;
-    }
-    const double dt = a.dt[patch];
-    const double updateSize = dt / (wz * wy * wx);
-#pragma unroll
-    for (int m = 0; m < V; ++m) q[m] += du[m] * updateSize;
-  }
-  __syncthreads();  // every node owner has read sQ
-  if (active) {
-#pragma unroll
-    for (int m = 0; m < V; ++m) sQ[node * D + m] = q[m];
-  }
-  __syncthreads();
-  for (int e = tid; e < B3 * D; e += NT)
-    if (P == 0 || e % D < V) luh[e] = sQ[e];  // the parameters are unchanged
-  if (a.dt_slots || a.dt_cell) {
-    double dtn = active ? node_stable_dt<PDE>(q, invDx, a.cfl_pnpm) : DBL_MAX;
-    dtn = dtn < DBL_MAX ? dtn : DBL_MAX;  // a NaN or inf candidate never wins (std::min order)
-    dtn = block_min<NT>(dtn, sRed);
-    if (tid == 0) {
-      if (a.dt_cell) a.dt_cell[cell] = dtn;
-      if (a.dt_slots)
-        atomic_min_positive(a.dt_slots + (a.cell_patch ? (size_t)patch * a.slots_per_patch + cell % a.slots_per_patch
-                                                       : cell % kDtSlots),
-                            dtn);
-    }
-  }
-}
-
 // dt_next = min(slots); t += dt (unless initialising); slots reset.
 __global__ __launch_bounds__(256) void dt_finalize_kernel(StepState* st, int advance_time) {
   __shared__ double sRed[256 / kWave];
@@ -1219,7 +1134,7 @@ hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, lo
     const dim3 grid((unsigned)ncells), block(node_threads(B));
     if (sd.precision == kFP32Bulk) {
       if (mode == (kUpdSurface | kUpdUpdate))
-        hipLaunchKernelGGL((update_staged_kernel<B, PDE, float>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false, float>), grid, block, 0, s, a);
       else if (mode == 0)
         hipLaunchKernelGGL((update_kernel<B, PDE, false, false, false, float>), grid, block, 0, s, a);
       else
@@ -1228,7 +1143,7 @@ hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, lo
     }
     switch (mode) {
       case kUpdSurface | kUpdUpdate:
-        hipLaunchKernelGGL((update_staged_kernel<B, PDE, double>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false, double>), grid, block, 0, s, a);
         break;
       case kUpdSurface | kUpdWriteLduh:
         hipLaunchKernelGGL((update_kernel<B, PDE, true, false, true, double>), grid, block, 0, s, a);

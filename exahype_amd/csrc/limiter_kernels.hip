@@ -500,21 +500,68 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
 // DMP + admissibility (LS:996-1029, 1045-1065), the limiter min/max of
 // FV-active cells, the new status (LS:955-994) and patch allocation
 // (LS:940-953).
+//
+// One wavefront per cell (lane = node; N <= 3, so B^3 <= 64), four cells per
+// workgroup, synchronised per wavefront only.  The local min/max runs the
+// tensor contractions one variable at a time (tensor3_wave: the same sums in
+// the same order as tensor3), so a wavefront's LDS holds one variable's
+// temporaries; the subcell values of the FV projection are formed again, per
+// variable, only for the cells that need the projection.
 // ---------------------------------------------------------------------------
+template <int P, int M>
+__device__ inline void tensor3_wave(const double* in, const double* T, double* t1, double* t2, double* out,
+                                    int lane) {
+  for (int i = lane; i < P * P * M; i += kWave) {  // t1[z][y][X]
+    const int X = i % M, r = i / M;
+    const double* row = in + r * P;
+    double acc = 0.0;
+#pragma unroll
+    for (int x = 0; x < P; ++x) acc += row[x] * T[x * M + X];
+    t1[i] = acc;
+  }
+  wave_lds_sync();
+  for (int i = lane; i < P * M * M; i += kWave) {  // t2[z][Y][X]
+    const int X = i % M, Y = (i / M) % M, z = i / (M * M);
+    double acc = 0.0;
+#pragma unroll
+    for (int y = 0; y < P; ++y) acc += t1[(z * P + y) * M + X] * T[y * M + Y];
+    t2[i] = acc;
+  }
+  wave_lds_sync();
+  for (int i = lane; i < M * M * M; i += kWave) {  // out[Z][Y][X]
+    const int YX = i % (M * M), Z = i / (M * M);
+    double acc = 0.0;
+#pragma unroll
+    for (int z = 0; z < P; ++z) acc += t2[z * M * M + YX] * T[z * M + Z];
+    out[i] = acc;
+  }
+  wave_lds_sync();
+}
+
+constexpr int kStatusCells = 4;  // cells (wavefronts) per workgroup of the status kernel
+
 template <int N>
-__global__ __launch_bounds__(kLimNT) void lim_update_status_kernel(LimArgs a) {
+__global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel(LimArgs a) {
   using Sh = LimShape<N>;
-  constexpr int B = Sh::B, B2 = B * B, B3 = Sh::B3, L3 = Sh::L3, V = Sh::V, PATCH = Sh::PATCH;
-  __shared__ double sQ[V * B3], sLob[V * B3], sSub[V * L3];
-  __shared__ double t1[V * B * B * Sh::L], t2[V * B * Sh::L * Sh::L];
-  __shared__ double sUh2lim[B * Sh::L], sUh2lob[B * B];
-  __shared__ double red[2 * V * (kLimNT / kWave)];
-  const int c = blockIdx.x;
-  const int tid = threadIdx.x;
+  constexpr int B = Sh::B, B2 = B * B, B3 = Sh::B3, L = Sh::L, L3 = Sh::L3, V = Sh::V, PATCH = Sh::PATCH;
+  static_assert(B3 <= kWave, "one node per lane");
+  constexpr int T1 = B * B * L > B3 ? B * B * L : B3, T2 = B * L * L > B3 ? B * L * L : B3;
+  constexpr int PER = V * B3 + T1 + T2 + L3 + B3;  // sQ, t1, t2, subcells, Lobatto values
+  __shared__ double sUh2lim[B * L], sUh2lob[B * B];
+  __shared__ double sWave[kStatusCells * PER];
   load_tables<N>(sUh2lim, sUh2lob, nullptr);
+  __syncthreads();  // the only workgroup barrier: before any wavefront can leave
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int c = blockIdx.x * kStatusCells + wid;
+  if (c >= a.ncells) return;
+  double* const sQ = sWave + wid * PER;
+  double* const t1 = sQ + V * B3;
+  double* const t2 = t1 + T1;
+  double* const sSub = t2 + T2;
+  double* const sLob = sSub + L3;
   const int s = a.pst[c];
-  const bool active = tid < B3;
-  const int node = active ? tid : 0;
+  const bool active = lane < B3;
+  const int node = active ? lane : 0;
   const int z = node / B2, y = (node / B) % B, x = node % B;
   double* luh = a.luh + (size_t)c * B3 * V + node * V;
   double q[V];
@@ -570,23 +617,49 @@ __global__ __launch_bounds__(kLimNT) void lim_update_status_kernel(LimArgs a) {
   {
     double dtn = active ? node_stable_dt<Euler<0>>(q, invDx, a.cfl_pnpm) : DBL_MAX;
     dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
-    dtn = block_min<kLimNT>(dtn, red);
-    if (tid == 0) {
+    dtn = wave_min(dtn);
+    if (lane == 0) {
       atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
       a.dt_cell[c] = dtn;
     }
   }
-  __syncthreads();
-  // DMP (LIM:114-165) against the merged boundary extrema
+  wave_lds_sync();
+  // local min/max (LIM:210-368): Gauss-Legendre nodes, Lobatto nodes,
+  // subcell means, one variable at a time
   double mn[V], mx[V];
-  dg_local_minmax<N>(sQ, sLob, sSub, t1, t2, sUh2lim, sUh2lob, red, mn, mx);
-  if (tid == 0) {
+#pragma unroll 1
+  for (int v = 0; v < V; ++v) {
+    tensor3_wave<B, B>(sQ + v * B3, sUh2lob, t1, t2, sLob, lane);
+    tensor3_wave<B, L>(sQ + v * B3, sUh2lim, t1, t2, sSub, lane);
+    double lo = DBL_MAX, hi = -DBL_MAX;
+    for (int i = lane; i < B3; i += kWave) {
+      lo = dmin(lo, sQ[v * B3 + i]);
+      hi = dmax(hi, sQ[v * B3 + i]);
+      lo = dmin(lo, sLob[i]);
+      hi = dmax(hi, sLob[i]);
+    }
+    for (int i = lane; i < L3; i += kWave) {
+      lo = dmin(lo, sSub[i]);
+      hi = dmax(hi, sSub[i]);
+    }
+    lo = wave_min(lo);
+    hi = -wave_min(-hi);
+    // a register array indexed by the rolled v would live in scratch memory:
+    // select into the unrolled slots instead
+#pragma unroll
+    for (int w = 0; w < V; ++w) {
+      mn[w] = (w == v) ? lo : mn[w];
+      mx[w] = (w == v) ? hi : mx[w];
+    }
+  }
+  if (lane == 0) {
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       a.loc_mm[(size_t)c * 2 * V + v] = mn[v];
       a.loc_mm[(size_t)c * 2 * V + V + v] = mx[v];
     }
   }
+  // DMP (LIM:114-165) against the merged boundary extrema
   double* bmn = a.mn + (size_t)c * 6 * V;
   double* bmx = a.mx + (size_t)c * 6 * V;
   bool ok = true;
@@ -607,8 +680,7 @@ __global__ __launch_bounds__(kLimNT) void lim_update_status_kernel(LimArgs a) {
   }
   // isPhysicallyAdmissible of the Euler solver (face-0 block = local minima)
   if (ok) ok = !(mn[0] <= 0.0) && !(mn[4] <= 0.0);
-  __syncthreads();  // every thread has read the face blocks
-  if (tid == 0) {
+  if (lane == 0) {  // every lane has its face-block values in registers
     if (s >= a.A) {  // determineLimiterMinAndMax: patch extrema on all faces
       for (int f = 0; f < 6; ++f)
         for (int v = 0; v < V; ++v) {
@@ -631,7 +703,7 @@ __global__ __launch_bounds__(kLimNT) void lim_update_status_kernel(LimArgs a) {
   if (!ok) {
     itc = a.cure + 1;
     ns = a.T;
-    if (s <= a.A && tid == 0) a.flags[kLimFlagIrregular] = 1;
+    if (s <= a.A && lane == 0) a.flags[kLimFlagIrregular] = 1;
   } else if (s >= a.T) {
     ns = a.T;
     itc -= 1;
@@ -655,8 +727,20 @@ __global__ __launch_bounds__(kLimNT) void lim_update_status_kernel(LimArgs a) {
   const bool helper = s > 0 && s < a.A;
   const bool fresh = !patch && ns > 0;
   const int buf = a.cur[c] ^ (helper ? 1 : 0);
-  if (helper || (fresh && ns < a.A)) store_interior<N>(patch_ptr(a, c, buf, PATCH), sSub);
-  if (tid == 0) {
+  if (helper || (fresh && ns < a.A)) {
+    double* const dst = patch_ptr(a, c, buf, PATCH);
+    constexpr int G = Sh::G, S = Sh::S;
+#pragma unroll 1
+    for (int v = 0; v < V; ++v) {
+      tensor3_wave<B, L>(sQ + v * B3, sUh2lim, t1, t2, sSub, lane);
+      for (int n = lane; n < L3; n += kWave) {
+        const int zz = n / (L * L), yy = (n / L) % L, xx = n % L;
+        dst[(((zz + G) * S + (yy + G)) * S + (xx + G)) * V + v] = sSub[n];
+      }
+      wave_lds_sync();  // sSub is rewritten by the next variable
+    }
+  }
+  if (lane == 0) {
     a.st[c] = ns;
     a.itc[c] = itc;
     a.has_patch[c] = (uint8_t)((patch || fresh) ? 1 : 0);
@@ -1191,7 +1275,8 @@ hipError_t launch_lim_fv_update(const SolverDesc& sd, const LimArgs& a, bool onl
 
 hipError_t launch_lim_update_status(const SolverDesc& sd, const LimArgs& a, hipStream_t s) {
   return dispatch_lim(sd.order, [&]<int N>() -> hipError_t {
-    hipLaunchKernelGGL(lim_update_status_kernel<N>, dim3(a.ncells), dim3(kLimNT), 0, s, a);
+    hipLaunchKernelGGL(lim_update_status_kernel<N>, dim3((a.ncells + kStatusCells - 1) / kStatusCells),
+                       dim3(kStatusCells * kWave), 0, s, a);
     return hipGetLastError();
   });
 }

@@ -788,8 +788,27 @@ __global__ void lim_reset_fv_count_kernel(int* flags, int only_if_irregular) {
   }
 }
 
-// reinitialiseSolversLocally (LS:1343-1381): roll cells with status > 0 back
-// to the previous time step; rebuild the FV list.
+// reinitialiseSolversLocally (LS:1343-1381), list part (thread per cell):
+// the FV list and the list of cells with status > 0 (rolled back), the cure
+// counter of troubled cells; cells with status 0 release their patch.
+__global__ __launch_bounds__(256) void lim_rollback_lists_kernel(LimArgs a) {
+  if (!a.flags[kLimFlagIrregular]) return;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= a.ncells) return;
+  const int s = a.st[c];
+  if (s >= a.A) a.fv_list[atomicAdd(&a.flags[kLimFlagFvCount], 1)] = c;
+  if (s > 0) {
+    a.rc_list[atomicAdd(&a.flags[kLimFlagRcCount], 1)] = c;
+    if (s >= a.T) a.itc[c] = a.cure + 1;
+  } else {
+    a.has_patch[c] = 0;
+  }
+}
+
+// reinitialiseSolversLocally, work part: roll the listed cells (status > 0)
+// back to the previous time step -- previously FV-active cells to their
+// previous patch and its DG projection, the others to their previous DG
+// solution and its FV projection.  Persistent over the list.
 template <int N>
 __global__ __launch_bounds__(kLimNT) void lim_rollback_kernel(LimArgs a) {
   using Sh = LimShape<N>;
@@ -799,16 +818,14 @@ __global__ __launch_bounds__(kLimNT) void lim_rollback_kernel(LimArgs a) {
   __shared__ double sUh2lim[Sh::B * L], sLim2uh[Sh::B * L];
   __shared__ double red[2 * V * (kLimNT / kWave)];
   if (!a.flags[kLimFlagIrregular]) return;
-  const int c = blockIdx.x;
-  const int s = a.st[c];
-  if (threadIdx.x == 0 && s >= a.A) a.fv_list[atomicAdd(&a.flags[kLimFlagFvCount], 1)] = c;
-  if (threadIdx.x == 0 && s > 0) a.rc_list[atomicAdd(&a.flags[kLimFlagRcCount], 1)] = c;
-  if (s > 0) {
-    load_tables<N>(sUh2lim, nullptr, sLim2uh);
+  const int count = a.flags[kLimFlagRcCount];
+  load_tables<N>(sUh2lim, nullptr, sLim2uh);
+  for (int it = blockIdx.x; it < count; it += gridDim.x) {
+    const int c = a.rc_list[it];
+    __syncthreads();  // tables loaded / the previous cell's LDS use is over
     if (a.pst[c] >= a.A) {
       const int buf = a.cur[c] ^ 1;
       double mn[V], mx[V];
-      __syncthreads();
       fv_to_dg<N, kLimNT>(patch_ptr(a, c, buf, PATCH), a.luh + (size_t)c * B3 * V, sSub, t1, t2, sQ, sLim2uh, red, mn,
                           mx);
       if (threadIdx.x == 0) a.cur[c] = (uint8_t)buf;
@@ -825,9 +842,6 @@ __global__ __launch_bounds__(kLimNT) void lim_rollback_kernel(LimArgs a) {
       tensor3<Sh::B, L, V, kLimNT>(sQ, sUh2lim, t1, t2, sSub);
       store_interior<N>(patch_ptr(a, c, a.cur[c], PATCH), sSub);
     }
-    if (threadIdx.x == 0 && s >= a.T) a.itc[c] = a.cure + 1;
-  } else if (threadIdx.x == 0) {
-    a.has_patch[c] = 0;
   }
 }
 
@@ -905,21 +919,18 @@ __global__ __launch_bounds__(kLimNT) void lim_recompute_kernel(LimArgs a) {
 
 // Cells with status 0 after an irregular change: determineMinAndMax writes
 // the local extrema on all faces, startNewTimeStep the dt candidate, both as
-// the status kernel computed them for the same solution.
+// the status kernel computed them for the same solution.  One thread per
+// face-block value (cell, face, variable), so the stores are contiguous.
 __global__ __launch_bounds__(256) void lim_recompute_light_kernel(LimArgs a) {
   if (!a.flags[kLimFlagIrregular]) return;
   constexpr int V = 5;
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= a.ncells || a.st[c] != 0) return;
-  const double* lm = a.loc_mm + (size_t)c * 2 * V;
-  double* bmn = a.mn + (size_t)c * 6 * V;
-  double* bmx = a.mx + (size_t)c * 6 * V;
-  for (int f = 0; f < 6; ++f)
-    for (int v = 0; v < V; ++v) {
-      bmn[f * V + v] = lm[v];
-      bmx[f * V + v] = lm[V + v];
-    }
-  atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(a.dt_cell[c]));
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)a.ncells * 6 * V) return;
+  const int c = (int)(i / (6 * V)), r = (int)(i % (6 * V)), v = r % V;
+  if (a.st[c] != 0) return;
+  a.mn[i] = a.loc_mm[(size_t)c * 2 * V + v];
+  a.mx[i] = a.loc_mm[(size_t)c * 2 * V + V + v];
+  if (r == 0) atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(a.dt_cell[c]));
 }
 
 // Initial state (LS:1031-1043 and the limiter patches' own initial condition,
@@ -1295,14 +1306,20 @@ hipError_t launch_lim_spread_rollback(const SolverDesc& sd, const LimArgs& a, hi
   hipLaunchKernelGGL(lim_reset_fv_count_kernel, dim3(1), dim3(64), 0, s, a.flags, 1);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(lim_rollback_lists_kernel, dim3(blocks256(a.ncells)), dim3(256), 0, s, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
   return dispatch_lim(sd.order, [&]<int N>() -> hipError_t {
-    hipLaunchKernelGGL(lim_rollback_kernel<N>, dim3(a.ncells), dim3(kLimNT), 0, s, a);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const long g = 8L * cus;
+    hipLaunchKernelGGL(lim_rollback_kernel<N>, dim3((unsigned)(g < a.ncells ? g : a.ncells)), dim3(kLimNT), 0, s, a);
     return hipGetLastError();
   });
 }
 
 hipError_t launch_lim_recompute_finish(const SolverDesc& sd, const LimArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(lim_recompute_light_kernel, dim3(blocks256(a.ncells)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(lim_recompute_light_kernel, dim3(blocks256((long)a.ncells * 30)), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return dispatch_lim(sd.order, [&]<int N>() -> hipError_t {

@@ -93,10 +93,12 @@ struct PredictorLayout {
   static constexpr int NQL = kFree <= 0 ? 0 : (kFree / kSliceBytes >= B ? B : (int)(kFree / kSliceBytes));
   static constexpr int NQR = B - NQL;
   // Rolled slices: the large sizes and the 9-variable PDE, whose unrolled
-  // slices exhaust the registers, and the float form from N=4 on, where the
+  // slices exhaust the registers, and the float form at N=4 and N=5, where the
   // rolled loop needs fewer registers (N=5: 110 instead of 166, one more
-  // wavefront per SIMD) at the cost of uniform table loads per slice.
-  static constexpr bool ROLLED = (B >= 7 && (IS_FP64 || B >= 8)) || V >= 9 || (!IS_FP64 && B >= 5);
+  // wavefront per SIMD) at the cost of uniform table loads per slice
+  // (measured: N=4 -2%, N=5 -4% predictor time; at N=6 the rolled form was
+  // 19% slower, so N=6 stays unrolled).
+  static constexpr bool ROLLED = (B >= 7 && (IS_FP64 || B >= 8)) || V >= 9 || (!IS_FP64 && (B == 5 || B == 6));
   // 8+ waves per workgroup leave 256 registers per lane, no room for slices;
   // a rolled loop cannot index registers.
   static constexpr bool WSQ = NQR > 0 && (NT >= 8 * kWave || ROLLED);

@@ -191,6 +191,86 @@ def bench_limiter(capi, args):
     g.close()
 
 
+FV_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fv_godunov_54cube_L10_pmc_traffic.txt")
+
+
+def fv_measured_traffic(n, L):
+    """HBM bytes per FV update launch from rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (copy-calibrated), as recorded for this grid; None otherwise."""
+    if not os.path.exists(FV_TRAFFIC_FILE):
+        return None
+    key = f"traffic_bytes_per_launch fv_godunov {n}^3 L={L}"
+    for line in open(FV_TRAFFIC_FILE):
+        if line.startswith(key + " "):
+            return int(line.split()[-1])
+    return None
+
+
+def cpu_baseline_fv(L, seconds):
+    """The oracle's finite-volumes driver (patches with ghost layers, ghost
+    filling, Godunov update, stable dt), single thread, on a periodic grid of
+    6^3 cells of L^3 subcells (rate per subcell update, an intensive figure)."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refutil import fv_grid_state, fv_smooth
+    m = 6
+    g = O.FVGrid(O.make_solver(3, cfl=0.9), O.FV_GODUNOV, L, m, periodic=True)
+    g.set_solution(fv_grid_state((m, m, m), L, fv_smooth)[1])
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        g.step(1)
+        steps += 1
+    el = time.perf_counter() - t0
+    return dict(value=m ** 3 * L ** 3 * steps / el, unit="DOF-updates/s (subcells)", cores=1, kind="port",
+                sample=f"{steps} steps of the FV driver (oracle) on {m}^3 cells of {L}^3 subcells, periodic smooth "
+                       f"Euler state, 1 thread, {el:.1f} s")
+
+
+def bench_fv(capi, args):
+    """Finite-volumes workload: the EulerFV demonstrator's Godunov solver in
+    3-D, 54^3 cells of 10^3 subcells (540^3), periodic entropy wave + 8
+    seeded modes, FP64."""
+    n, L = args.fv_cells, args.patch_size
+    g = capi.FVGrid(capi.euler_solver(3), L, n)
+    g.init_euler(rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=8, mode_amp=0.01,
+                 seed=args.seed)
+    g.step(args.warmup)
+    g.synchronize()
+    g.set_profiling(True)
+    g.stage_times()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    stages = g.stage_times()
+    t_now, dt_now, _ = g.time()
+    if not np.isfinite(dt_now) or dt_now <= 0:
+        raise RuntimeError("non-finite time step")
+    sub = (n * L) ** 3
+    upd_ms, upd_n = stages["update"]
+    avg_ms = upd_ms / max(upd_n, 1)
+    alg_bytes = sub * 5 * 8 * 2  # read u, write u_new (variables in double)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "DOF-updates/s (subcell updates), 3D Euler finite volumes (Godunov/Rusanov), 54^3 cells x 10^3 "
+                  "subcells, FP64",
+        "value": sub * args.steps / wall, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (entropy wave + 8 seeded Fourier modes at subcell centres, generated on the device)",
+        "config": {"workload": f"euler3d-fv-godunov {n}^3 cells, patch size {L} ({n * L}^3 subcells), periodic, "
+                               "CFL 0.9", "subcells": sub, "seed": args.seed},
+        "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items()},
+        "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=fv_measured_traffic(n, L),
+                         kernel="fv_godunov_kernel (Godunov update + stable dt of the new solution)",
+                         avg_launch_ms=round(avg_ms, 4), algorithmic_bytes_per_launch=alg_bytes),
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_fv(L, args.cpu_seconds),
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
 def cpu_baseline_batch(specs, threads, seconds):
     """The oracle's grid time step per order on a 9^3 patch of that order,
     weighted by the workload's DOF mix: the estimated CPU rate on the same
@@ -332,11 +412,14 @@ def main():
                     help="both: time FP64 and FP32-bulk and report the error between them")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("precision", "limiter", "batch", "elastic"), default="precision",
+    ap.add_argument("--fv-cells", type=int, default=54, help="fv: cells per direction")
+    ap.add_argument("--patch-size", type=int, default=10, help="fv: subcells per cell and direction")
+    ap.add_argument("--workload", choices=("precision", "limiter", "batch", "elastic", "fv"), default="precision",
                     help="precision: Euler N=5 81^3, FP32-bulk vs FP64 (the headline line); "
                          "limiter: Euler blast N=3 81^3 with the subcell limiter, FP64; "
                          "batch: 96 seeded uneven patches (orders 2..7), FP32-bulk vs FP64; "
-                         "elastic: linear elasticity N=4 81^3, FP32-bulk vs FP64")
+                         "elastic: linear elasticity N=4 81^3, FP32-bulk vs FP64; "
+                         "fv: Godunov finite volumes, 54^3 cells of 10^3 subcells, FP64")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="batch: GB of patch state per wave")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times): the known-size "
@@ -361,6 +444,9 @@ def main():
         return
     if args.workload == "elastic":
         bench_elastic(capi, args)
+        return
+    if args.workload == "fv":
+        bench_fv(capi, args)
         return
     order, n = args.order, args.cells
     B, V, D = order + 1, 5, 5

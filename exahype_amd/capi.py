@@ -21,6 +21,7 @@ STAGE_PREDICTOR, STAGE_RIEMANN, STAGE_UPDATE, STAGE_DT = range(4)
 STAGE_NAMES = ("predictor", "riemann", "update", "dt")
 BC_PERIODIC = 0
 BC_TRANSMISSIVE = 1
+FV_GODUNOV, FV_MUSCLHANCOCK = 0, 1
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
 
 # Every symbol declared in include/exahype_amd.h.
@@ -40,6 +41,9 @@ EXPORTED = (
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
     "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_lds_variant",
     "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy",
+    "exa_godunov_solution_update", "exa_fvgrid_create", "exa_fvgrid_destroy", "exa_fvgrid_num_cells",
+    "exa_fvgrid_upload", "exa_fvgrid_download", "exa_fvgrid_init_euler", "exa_fvgrid_step",
+    "exa_fvgrid_synchronize", "exa_fvgrid_time", "exa_fvgrid_set_profiling", "exa_fvgrid_stage_times",
 )
 
 
@@ -132,7 +136,7 @@ def lib():
     L.exa_find_cell_local_min_and_max.argtypes = [_D, i, i, _D, _D]
     L.exa_discrete_maximum_principle_and_min_and_max_search.argtypes = [_D, i, i, d_, d_, _D, _D, _I]
     L.exa_musclhancock_solution_update.argtypes = [S, _D, _D, _D, d_, i, i, _D]
-    L.exa_lgrid_create.argtypes = [S, i, i, i, _D, _D, i, i, i, d_, d_, ctypes.POINTER(V)]
+    L.exa_lgrid_create.argtypes = [S, i, i, i, _D, _D, i, i, i, d_, d_, i, ctypes.POINTER(V)]
     L.exa_lgrid_destroy.argtypes = [V]
     L.exa_lgrid_num_cells.argtypes = [V]
     L.exa_lgrid_num_cells.restype = ctypes.c_long
@@ -154,6 +158,19 @@ def lib():
     L.exa_patchset_solution.argtypes = [V, i, _D]
     L.exa_patchset_time.argtypes = [V, i, _D, _D]
     L.exa_patchset_destroy.argtypes = [V]
+    L.exa_godunov_solution_update.argtypes = [S, _D, _D, _D, d_, i, i, _D]
+    L.exa_fvgrid_create.argtypes = [S, i, i, i, i, i, _D, _D, i, ctypes.POINTER(V)]
+    L.exa_fvgrid_destroy.argtypes = [V]
+    L.exa_fvgrid_num_cells.argtypes = [V]
+    L.exa_fvgrid_num_cells.restype = ctypes.c_long
+    L.exa_fvgrid_upload.argtypes = [V, _D]
+    L.exa_fvgrid_download.argtypes = [V, _D]
+    L.exa_fvgrid_init_euler.argtypes = [V, ctypes.POINTER(EulerIC)]
+    L.exa_fvgrid_step.argtypes = [V, i]
+    L.exa_fvgrid_synchronize.argtypes = [V]
+    L.exa_fvgrid_time.argtypes = [V, _D, _D, ctypes.POINTER(ctypes.c_long)]
+    L.exa_fvgrid_set_profiling.argtypes = [V, i]
+    L.exa_fvgrid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     _lib = L
     return L
 
@@ -387,23 +404,100 @@ def muscl_update(s, patch, dx, dt, return_dt=False):
     return (new, dtmax.value) if return_dt else new
 
 
+def godunov_update(s, patch, dx, dt, patch_size, ghost_layer_width=1, luh_new=None):
+    """godunov::c::solutionUpdate of one patch with filled ghost layers;
+    returns (new patch, dt_max_allowed).  luh_new defaults to a copy of the
+    patch (the reference's caller copies the solution first)."""
+    old = _f64(patch).ravel()
+    new = old.copy() if luh_new is None else _f64(luh_new).ravel().copy()
+    dtmax = ctypes.c_double()
+    check(lib().exa_godunov_solution_update(ctypes.byref(s), _dp(new), _dp(old), _dp(_dx3(dx)), dt, patch_size,
+                                            ghost_layer_width, ctypes.byref(dtmax)), "godunov")
+    return new, dtmax.value
+
+
+class FVGrid:
+    """Device-resident finite-volumes grid (exa_fvgrid): nx*ny*nz cells of
+    patch_size^3 subcells, Godunov, global time stepping."""
+
+    def __init__(self, s, patch_size, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), boundary=BC_PERIODIC,
+                 scheme=FV_GODUNOV):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.s, self.L, self.n = s, patch_size, tuple(n)
+        h = ctypes.c_void_p()
+        check(lib().exa_fvgrid_create(ctypes.byref(s), scheme, patch_size, n[0], n[1], n[2], _dp(_dx3(lo)),
+                                      _dp(_dx3(hi)), boundary, ctypes.byref(h)), "exa_fvgrid_create")
+        self._h = h
+        self.ncells = lib().exa_fvgrid_num_cells(h)
+        self.size = self.ncells * patch_size ** 3 * 5
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().exa_fvgrid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, interior):
+        a = _f64(interior).ravel()
+        assert a.size == self.size
+        check(lib().exa_fvgrid_upload(self._h, _dp(a)), "upload")
+
+    def init_euler(self, rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=0, mode_amp=0.0,
+                   seed=0):
+        ic = EulerIC(rho0, amp, (ctypes.c_double * 3)(*k), (ctypes.c_double * 3)(*vel), p, nmodes, mode_amp,
+                     seed)
+        check(lib().exa_fvgrid_init_euler(self._h, ctypes.byref(ic)), "exa_fvgrid_init_euler")
+
+    def solution(self):
+        out = np.zeros(self.size)
+        check(lib().exa_fvgrid_download(self._h, _dp(out)), "download")
+        return out
+
+    def step(self, n=1):
+        check(lib().exa_fvgrid_step(self._h, n), "step")
+
+    def synchronize(self):
+        check(lib().exa_fvgrid_synchronize(self._h), "synchronize")
+
+    def time(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
+        check(lib().exa_fvgrid_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n)), "time")
+        return t.value, dt.value, n.value
+
+    def set_profiling(self, on=True):
+        check(lib().exa_fvgrid_set_profiling(self._h, int(on)), "set_profiling")
+
+    def stage_times(self):
+        ms = np.zeros(2)
+        n = (ctypes.c_long * 2)()
+        check(lib().exa_fvgrid_stage_times(self._h, _dp(ms), n), "stage_times")
+        return {"update": (float(ms[0]), int(n[0])), "finalize": (float(ms[1]), int(n[1]))}
+
+
 class LimitingGrid:
     """A uniform grid with the a-posteriori subcell limiter (exa_lgrid)."""
 
     def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), boundary=BC_TRANSMISSIVE, helper_layers=1,
-                 steps_till_cured=0, relax=1e-4, scaling=1e-3):
+                 steps_till_cured=0, relax=1e-4, scaling=1e-3, fv_scheme=None):
         if isinstance(n, int):
             n = (n, n, n)
         self.s = s
         self.n = tuple(n)
+        self.fv_scheme = FV_MUSCLHANCOCK if fv_scheme is None else fv_scheme
         h = ctypes.c_void_p()
         check(lib().exa_lgrid_create(ctypes.byref(s), n[0], n[1], n[2], _dp(_dx3(lo)), _dp(_dx3(hi)), boundary,
-                                     helper_layers, steps_till_cured, relax, scaling, ctypes.byref(h)),
-              "exa_lgrid_create")
+                                     helper_layers, steps_till_cured, relax, scaling, self.fv_scheme,
+                                     ctypes.byref(h)), "exa_lgrid_create")
         self._h = h
         self.ncells = lib().exa_lgrid_num_cells(h)
         self.L = 2 * s.order + 1
-        self.S = self.L + 4
+        self.S = self.L + 4  # device patches keep two ghost layers for either scheme
 
     def close(self):
         if getattr(self, "_h", None):

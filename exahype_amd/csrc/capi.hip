@@ -536,24 +536,8 @@ extern "C" int exa_grid_init_euler(exa_grid* g, const exa_euler_ic* ic) {
   for (int d = 0; d < 3; ++d) {
     a.lo[d] = g->lo[d];
     a.dx[d] = g->dx[d];
-    a.k[d] = ic->k[d];
-    a.vel[d] = ic->vel[d];
   }
-  a.rho0 = ic->rho0;
-  a.amp = ic->amp;
-  a.p = ic->p;
-  a.nmodes = ic->nmodes;
-  a.mode_amp = ic->mode_amp;
-  uint64_t state = ic->seed;
-  for (int m = 0; m < ic->nmodes; ++m) {
-    int kk[3];
-    do {
-      for (int d = 0; d < 3; ++d) kk[d] = (int)(splitmix64(state) % 7) - 3;
-    } while (kk[0] == 0 && kk[1] == 0 && kk[2] == 0);
-    for (int d = 0; d < 3; ++d) a.mode_k[m][d] = kk[d];
-    for (int c = 0; c < 4; ++c)
-      a.mode_phase[m][c] = 6.283185307179586476925286766559 * (double)(splitmix64(state) >> 11) * 0x1.0p-53;
-  }
+  fill_euler_ic(ic, &a);
   EXA_TRY(exa::launch_euler_ic(a, g->stream));
   return reset_time(g);
 }

@@ -65,4 +65,29 @@ uint64_t splitmix64(uint64_t& x) {
   return z ^ (z >> 31);
 }
 
+// The formula fields of ICArgs from an exa_euler_ic: the seeded Fourier
+// modes draw integer wave vectors in [-3,3]^3 (not all zero) and four phases
+// in [0, 2 pi) each from splitmix64.
+void fill_euler_ic(const exa_euler_ic* ic, exa::ICArgs* a) {
+  for (int d = 0; d < 3; ++d) {
+    a->k[d] = ic->k[d];
+    a->vel[d] = ic->vel[d];
+  }
+  a->rho0 = ic->rho0;
+  a->amp = ic->amp;
+  a->p = ic->p;
+  a->nmodes = ic->nmodes;
+  a->mode_amp = ic->mode_amp;
+  uint64_t state = ic->seed;
+  for (int m = 0; m < ic->nmodes; ++m) {
+    int kk[3];
+    do {
+      for (int d = 0; d < 3; ++d) kk[d] = (int)(splitmix64(state) % 7) - 3;
+    } while (kk[0] == 0 && kk[1] == 0 && kk[2] == 0);
+    for (int d = 0; d < 3; ++d) a->mode_k[m][d] = kk[d];
+    for (int c = 0; c < 4; ++c)
+      a->mode_phase[m][c] = 6.283185307179586476925286766559 * (double)(splitmix64(state) >> 11) * 0x1.0p-53;
+  }
+}
+
 }  // namespace

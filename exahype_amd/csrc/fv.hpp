@@ -1,0 +1,74 @@
+// fv.hpp -- internal interface of the finite-volume kernels: the single-patch
+// Godunov update (reference layout) and the device-resident finite-volumes
+// grid (dense subcell planes, first-order Godunov with Rusanov fluxes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "aderdg.hpp"
+
+namespace exa {
+
+// One patch (S,S,S,5), S = L + 2g, variable fastest (godunov.cpph:47).
+// luh_new holds a copy of luh on entry and receives the reference's writes:
+// the interior update and the boundary-face fluxes added to the first ghost
+// layer.  dt_max (bits of a positive double, DBL_MAX on entry) receives the
+// minimum over the patch's faces of cfl/3 h_d / s_max.
+struct GodunovPatchArgs {
+  const double* luh;
+  double* luh_new;
+  int L, g;
+  double h[3];
+  double dt;
+  double cfl;
+  unsigned long long* dt_max;
+};
+hipError_t launch_godunov_patch(const GodunovPatchArgs& a, hipStream_t s);
+
+// Time-step state of a finite-volumes grid (device resident).
+struct FvState {
+  double t;                   // time stamp
+  double dt;                  // time step size of the next update
+  unsigned long long dt_next; // running minimum of the new solution's cell dts (double bits)
+  long long steps;
+  int bad;                    // a non-finite or non-positive dt was produced
+};
+
+// The grid's solution is one plane set per variable, u[v][Z][Y][X] with
+// X = nx L etc.: subcell (cell (i,j,k), local (x,y,z)) sits at global
+// (k L + z, j L + y, i L + x).  Ghost layers do not exist; neighbour reads
+// wrap (periodic) or clamp to the boundary subcell (transmissive with ghost
+// width 1, where the reference's mirror copies that subcell).
+struct FvGridArgs {
+  const double* u;
+  double* un;
+  int X, Y, Z;
+  int periodic;
+  double h[3];   // subcell sizes dx_d / L
+  double cfl;    // the solver's CFL (stableTimeStepSize's factor is cfl/3)
+  FvState* state;
+  int nsx, nry, nzc, zlen, ntiles;  // tiling of the update kernel
+};
+// One Godunov update u -> un with dt = state->dt, fused with the new
+// solution's stable dt (atomic minimum into state->dt_next).
+hipError_t launch_fv_godunov_step(const FvGridArgs& a, hipStream_t s);
+// t += dt; dt = dt_next; dt_next = max; steps += 1.
+hipError_t launch_fv_finalize(FvState* st, hipStream_t s);
+// Stable dt of u into state->dt_next, then t = 0, dt = that minimum.
+hipError_t launch_fv_initial_dt(const FvGridArgs& a, hipStream_t s);
+// Host layout [cells][L^3][5] (cell-major, subcells z,y,x) <-> planes.
+hipError_t launch_fv_pack(const double* cells, double* planes, int nx, int ny, int nz, int L, hipStream_t s);
+hipError_t launch_fv_unpack(const double* planes, double* cells, int nx, int ny, int nz, int L, hipStream_t s);
+// Euler initial data (ICArgs' formula) at the subcell centres lo + (g + 1/2) h.
+hipError_t launch_fv_euler_ic(const ICArgs& ic, double* planes, int X, int Y, int Z, const double* h,
+                              hipStream_t s);
+
+// Workgroup shape of the update kernel: 8 wavefronts, each a row of 64
+// consecutive subcells along x; rows 1..6 and lanes 1..62 produce output,
+// the outer rows and lanes supply neighbour values.
+constexpr int kFvRows = 8;
+constexpr int kFvOutRows = kFvRows - 2;
+constexpr int kFvOutX = 64 - 2;
+
+}  // namespace exa

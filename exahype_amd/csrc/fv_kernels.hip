@@ -1,0 +1,459 @@
+// fv_kernels.hip -- finite-volume kernels on the device (3-D, Euler, FP64):
+// the first-order Godunov update of one patch in the reference layout, and a
+// device-resident finite-volumes grid stored as dense subcell planes.
+//
+// Reference functions restated (paths relative to the reference checkout):
+//   GOD = ExaHyPE/kernels/finitevolumes/godunov/c/3d/godunov.cpph
+//   RUS = ExaHyPE/kernels/finitevolumes/riemannsolvers/c/rusanov.cpph
+//   FVC = ExaHyPE/kernels/finitevolumes/commons/c/commons.cpph
+//   FVS = ExaHyPE/exahype/solvers/FiniteVolumesSolver.cpp
+//
+// Every subcell's new value is formed in the reference's accumulation order:
+// start from the old value, then + dt/h F(x-1/2), - dt/h F(x+1/2), the same
+// for y, then for z (GOD:61-123 visits all x faces, then all y faces, then
+// all z faces, and each face adds to its right cell before the next face
+// subtracts from it).  A face flux is a function of its two cell states
+// only, so computing it once per face, or once per patch that shares it,
+// gives the same value.
+#include <cfloat>
+#include <cmath>
+
+#include "exa_device.hpp"
+#include "fv.hpp"
+#include "fv_device.hpp"
+
+namespace exa {
+
+namespace {
+
+constexpr double kGamma = 1.4;
+
+// Pressure and sound speed of one state, shared by its directional fluxes
+// and eigenvalues (the same expression trees as euler_flux_dir and
+// Euler::max_abs_eigenvalue, which compute them again per call).
+struct CellBase {
+  double irho, p, cs;
+};
+
+__device__ inline CellBase cell_base(const double* q) {
+  CellBase b;
+  b.irho = 1.0 / q[0];
+  b.p = (kGamma - 1.0) * (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * b.irho);
+  b.cs = sqrt(kGamma * b.p * b.irho);
+  return b;
+}
+
+// Largest |eigenvalue| in direction d: max_element of |(u-c, u, u, u, u+c)|.
+__device__ inline double cell_lambda(const double* q, const CellBase& b, int d) {
+  const double u_n = q[d + 1] * b.irho;
+  const double l0 = fabs(u_n - b.cs), l1 = fabs(u_n), l4 = fabs(u_n + b.cs);
+  double m = l0;
+  m = (m < l1) ? l1 : m;
+  m = (m < l4) ? l4 : m;
+  return m;
+}
+
+__device__ inline void cell_flux(const double* q, const CellBase& b, int d, double* F) {
+  const double qn = q[d + 1];
+  F[0] = qn;
+#pragma unroll
+  for (int m = 1; m < 4; ++m) F[m] = (m == d + 1) ? b.irho * qn * q[m] + b.p : b.irho * qn * q[m];
+  F[4] = b.irho * qn * (q[4] + b.p);
+}
+
+// Rusanov flux across a face between states qa (left) and qb (right) in
+// direction d; returns s_max.
+__device__ inline double face_flux(const double* qa, const double* qb, int d, double* fn) {
+  const CellBase ba = cell_base(qa), bb = cell_base(qb);
+  double Fa[5], Fb[5];
+  cell_flux(qa, ba, d, Fa);
+  cell_flux(qb, bb, d, Fb);
+  const double la = cell_lambda(qa, ba, d), lb = cell_lambda(qb, bb, d);
+  rusanov_from_parts(qa, Fa, la, qb, Fb, lb, fn);
+  return dmax(lb, la);
+}
+
+// FV stableTimeStepSize of one subcell (FVC:409-460): min over d of
+// cfl/3 * h_d / max|lambda_d|, with ch[d] = cfl/3 * h_d.
+__device__ inline double cell_stable_dt(const double* q, const double* ch) {
+  const CellBase b = cell_base(q);
+  double dt = DBL_MAX;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dt = dmin(dt, ch[d] / cell_lambda(q, b, d));
+  return dt;
+}
+
+__device__ inline void atomic_min_positive(unsigned long long* slot, double v) {
+  atomicMin(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+// Single patch (GOD:35-143).  One thread per subcell of the (S,S,S) patch:
+// interior subcells take their six faces in the reference's order; a
+// face-ghost subcell of the first layer takes the one face it shares with
+// the interior (GOD's loops start one layer early and end one layer late);
+// every other ghost is left as it is.  Every face's dt candidate is formed
+// once: by the subcell on its right, or for the last face of a line by the
+// subcell on its left.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void godunov_patch_kernel(GodunovPatchArgs a) {
+  const int L = a.L, g = a.g, S = L + 2 * g;
+  const int pb = g, pe = g + L;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  double dtc = DBL_MAX;
+  if (id < S * S * S) {
+    const int z = id / (S * S), y = (id / S) % S, x = id % S;
+    const int co[3] = {x, y, z};
+    const int stride[3] = {5, 5 * S, 5 * S * S};
+    const bool in[3] = {x >= pb && x < pe, y >= pb && y < pe, z >= pb && z < pe};
+    const double cdt[3] = {a.dt / a.h[0], a.dt / a.h[1], a.dt / a.h[2]};
+    const double ch[3] = {a.cfl / 3 * a.h[0], a.cfl / 3 * a.h[1], a.cfl / 3 * a.h[2]};
+    const double* q = a.luh + (size_t)id * 5;
+    double* out = a.luh_new + (size_t)id * 5;
+    if (in[0] && in[1] && in[2]) {
+      double un[5], fl[5], fr[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) un[v] = out[v];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double smax_l = face_flux(q - stride[d], q, d, fl);
+        const double smax_r = face_flux(q, q + stride[d], d, fr);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) un[v] += cdt[d] * fl[v];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) un[v] -= cdt[d] * fr[v];
+        dtc = dmin(dtc, ch[d] / smax_l);
+        if (co[d] == pe - 1) dtc = dmin(dtc, ch[d] / smax_r);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] = un[v];
+    } else {
+      // a face ghost: exactly one coordinate one layer outside, the others inside
+      int dir = -1, side = 0, nout = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (in[d]) continue;
+        ++nout;
+        if (co[d] == pb - 1) {
+          dir = d;
+          side = -1;
+        } else if (co[d] == pe) {
+          dir = d;
+          side = 1;
+        } else {
+          nout = 9;
+        }
+      }
+      if (nout == 1) {
+        double f[5];
+        if (side < 0) {  // left ghost: luh_new -= dt/h F(face to its right)
+          face_flux(q, q + stride[dir], dir, f);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) out[v] -= cdt[dir] * f[v];
+        } else {  // right ghost: luh_new += dt/h F(face to its left)
+          face_flux(q - stride[dir], q, dir, f);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) out[v] += cdt[dir] * f[v];
+        }
+      }
+    }
+  }
+  const double m = wave_min(dtc);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m < DBL_MAX) atomic_min_positive(a.dt_max, m);
+}
+
+// ---------------------------------------------------------------------------
+// Grid update.  A workgroup owns a tile of 62 x 6 subcell columns and walks a
+// range of z planes.  Wavefront w holds row y0 - 1 + w, lane l column
+// x0 - 1 + l, so the outer rows and lanes carry the neighbours of the tile.
+// Per plane and column the subcell's pressure and sound speed are formed
+// once, each face flux once:
+//   x faces: lane l forms the face to its right from lane l+1's state,
+//            directional flux and eigenvalue (cross-lane shuffles), and
+//            passes it to lane l+1 as that lane's left face;
+//   y faces: rows exchange state, y flux and eigenvalue through LDS; row w
+//            forms the face above it and leaves it in LDS for row w+1;
+//   z faces: carried along the walk (the face above plane z is the face
+//            below plane z+1).
+// The update is followed by the new subcell's stable dt (startNewTimeStep,
+// FVS:553-574), reduced over the wavefront and into state->dt_next.
+// Reads per step: each subcell once from the tile, plus the tile borders
+// (1/31 along x, 1/3 along y, read again by the neighbouring tile, mostly
+// from L2); writes: each subcell once.
+// ---------------------------------------------------------------------------
+__device__ inline int wrap_index(int i, int n, int periodic) {
+  if (periodic) {
+    i %= n;
+    return i < 0 ? i + n : i;
+  }
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+__global__ __launch_bounds__(kWave * kFvRows) void fv_godunov_kernel(FvGridArgs a) {
+  __shared__ double sQ[kFvRows][11][kWave];  // state (5), y flux (5), y eigenvalue
+  __shared__ double sFy[kFvRows][5][kWave];  // face above the row
+  // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a
+  // contiguous run of tiles (rows fastest), so tiles that share border rows
+  // share that XCD's L2
+  const int per = (int)(gridDim.x / 8);
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= a.ntiles) return;  // the whole workgroup
+  const int ry = t % a.nry, r2 = t / a.nry, sx = r2 % a.nsx, cz = r2 / a.nsx;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int xp = sx * kFvOutX - 1 + lane, yp = ry * kFvOutRows - 1 + w;
+  const int xs = wrap_index(xp, a.X, a.periodic), ys = wrap_index(yp, a.Y, a.periodic);
+  const bool out_row = w >= 1 && w <= kFvOutRows && yp < a.Y;  // uniform per wavefront
+  const bool out = out_row && lane >= 1 && lane <= kFvOutX && xp < a.X;
+  const bool face_above = out_row || w == 0;
+  const size_t plane = (size_t)a.X * a.Y, vol = plane * (size_t)a.Z;
+  const size_t col = (size_t)ys * a.X + xs;
+  const int z0 = cz * a.zlen, z1 = min(a.Z, z0 + a.zlen);
+  const double dt = a.state->dt;
+  const double cdt[3] = {dt / a.h[0], dt / a.h[1], dt / a.h[2]};
+  const double ch[3] = {a.cfl / 3 * a.h[0], a.cfl / 3 * a.h[1], a.cfl / 3 * a.h[2]};
+  auto load = [&](int z, double* q) __device__ {
+    const size_t o = (size_t)wrap_index(z, a.Z, a.periodic) * plane + col;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = a.u[v * vol + o];
+  };
+
+  double q[5], qn[5], qnn[5], Fz[5], fzlo[5];
+  double lz = 0.0;
+  CellBase b{};
+  if (out_row) {
+    double qm[5], Fm[5];
+    load(z0 - 1, qm);
+    load(z0, q);
+    load(z0 + 1, qn);
+    const CellBase bm = cell_base(qm);
+    b = cell_base(q);
+    cell_flux(qm, bm, 2, Fm);
+    cell_flux(q, b, 2, Fz);
+    lz = cell_lambda(q, b, 2);
+    rusanov_from_parts(qm, Fm, cell_lambda(qm, bm, 2), q, Fz, lz, fzlo);
+  }
+  double dtmin = DBL_MAX;
+  for (int z = z0; z < z1; ++z) {
+    if (out_row) {
+      load(z + 2, qnn);  // in flight during this plane
+    } else {
+      load(z, q);
+      b = cell_base(q);
+    }
+    // y data of this row
+    double Fy[5];
+    cell_flux(q, b, 1, Fy);
+    const double ly = cell_lambda(q, b, 1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      sQ[w][v][lane] = q[v];
+      sQ[w][5 + v][lane] = Fy[v];
+    }
+    sQ[w][10][lane] = ly;
+    double un[5];
+    if (out_row) {
+      // x faces
+      double Fx[5], qr[5], Fr[5], fxhi[5];
+      cell_flux(q, b, 0, Fx);
+      const double lx = cell_lambda(q, b, 0);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        qr[v] = __shfl_down(q[v], 1, kWave);
+        Fr[v] = __shfl_down(Fx[v], 1, kWave);
+      }
+      const double lr = __shfl_down(lx, 1, kWave);
+      rusanov_from_parts(q, Fx, lx, qr, Fr, lr, fxhi);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double fxlo = __shfl_up(fxhi[v], 1, kWave);
+        un[v] = q[v];
+        un[v] += cdt[0] * fxlo;
+        un[v] -= cdt[0] * fxhi[v];
+      }
+    }
+    __syncthreads();
+    double fyhi[5];
+    if (face_above) {
+      double qa[5], Fa[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        qa[v] = sQ[w + 1][v][lane];
+        Fa[v] = sQ[w + 1][5 + v][lane];
+      }
+      rusanov_from_parts(q, Fy, ly, qa, Fa, sQ[w + 1][10][lane], fyhi);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sFy[w][v][lane] = fyhi[v];
+    }
+    __syncthreads();
+    if (out_row) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        un[v] += cdt[1] * sFy[w - 1][v][lane];
+        un[v] -= cdt[1] * fyhi[v];
+      }
+      // z faces
+      const CellBase bn = cell_base(qn);
+      double Fzn[5], fzhi[5];
+      cell_flux(qn, bn, 2, Fzn);
+      const double lzn = cell_lambda(qn, bn, 2);
+      rusanov_from_parts(q, Fz, lz, qn, Fzn, lzn, fzhi);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        un[v] += cdt[2] * fzlo[v];
+        un[v] -= cdt[2] * fzhi[v];
+      }
+      if (out) {
+        const size_t o = (size_t)z * plane + col;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a.un[v * vol + o] = un[v];
+        dtmin = dmin(dtmin, cell_stable_dt(un, ch));
+      }
+      // walk one plane up
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        q[v] = qn[v];
+        qn[v] = qnn[v];
+        Fz[v] = Fzn[v];
+        fzlo[v] = fzhi[v];
+      }
+      b = bn;
+      lz = lzn;
+    }
+  }
+  const double m = wave_min(dtmin);
+  if (lane == 0 && m < DBL_MAX) atomic_min_positive(&a.state->dt_next, m);
+}
+
+__global__ void fv_finalize_kernel(FvState* st) {
+  if (threadIdx.x != 0) return;
+  const double nd = __longlong_as_double((long long)st->dt_next);
+  st->t += st->dt;
+  st->dt = nd;
+  st->dt_next = (unsigned long long)__double_as_longlong(DBL_MAX);
+  st->steps += 1;
+  if (!(nd > 0.0) || !(nd < DBL_MAX)) st->bad = 1;
+}
+
+__global__ __launch_bounds__(256) void fv_stable_dt_kernel(FvGridArgs a) {
+  const size_t vol = (size_t)a.X * a.Y * a.Z;
+  const double ch[3] = {a.cfl / 3 * a.h[0], a.cfl / 3 * a.h[1], a.cfl / 3 * a.h[2]};
+  double dtmin = DBL_MAX;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < vol; i += (size_t)gridDim.x * blockDim.x) {
+    double q[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = a.u[v * vol + i];
+    dtmin = dmin(dtmin, cell_stable_dt(q, ch));
+  }
+  const double m = wave_min(dtmin);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m < DBL_MAX) atomic_min_positive(&a.state->dt_next, m);
+}
+
+__global__ void fv_begin_kernel(FvState* st) {
+  if (threadIdx.x != 0) return;
+  const double nd = __longlong_as_double((long long)st->dt_next);
+  st->t = 0.0;
+  st->dt = nd;
+  st->dt_next = (unsigned long long)__double_as_longlong(DBL_MAX);
+  st->steps = 0;
+  st->bad = (!(nd > 0.0) || !(nd < DBL_MAX)) ? 1 : 0;
+}
+
+__global__ void fv_reset_next_kernel(FvState* st) {
+  if (threadIdx.x == 0) st->dt_next = (unsigned long long)__double_as_longlong(DBL_MAX);
+}
+
+// planes[v][Z][Y][X] <-> cells[cell][z][y][x][v]
+template <bool PACK>
+__global__ __launch_bounds__(256) void fv_pack_kernel(const double* src, double* dst, int nx, int ny, int nz, int L) {
+  const int X = nx * L, Y = ny * L;
+  const size_t vol = (size_t)X * Y * (size_t)(nz * L);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < vol; i += (size_t)gridDim.x * blockDim.x) {
+    const int xg = (int)(i % X), yg = (int)((i / X) % Y), zg = (int)(i / ((size_t)X * Y));
+    const size_t cell = ((size_t)(zg / L) * ny + yg / L) * nx + xg / L;
+    const size_t h = (((cell * L + zg % L) * L + yg % L) * L + xg % L) * 5;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      if constexpr (PACK)
+        dst[v * vol + i] = src[h + v];
+      else
+        dst[h + v] = src[v * vol + i];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void fv_euler_ic_kernel(ICArgs a, double* planes, int X, int Y, int Z, double h0,
+                                                         double h1, double h2) {
+  const size_t vol = (size_t)X * Y * (size_t)Z;
+  const double twopi = 6.283185307179586476925286766559;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < vol; i += (size_t)gridDim.x * blockDim.x) {
+    const int xg = (int)(i % X), yg = (int)((i / X) % Y), zg = (int)(i / ((size_t)X * Y));
+    const double P[3] = {a.lo[0] + (xg + 0.5) * h0, a.lo[1] + (yg + 0.5) * h1, a.lo[2] + (zg + 0.5) * h2};
+    double rho = a.rho0 + a.amp * sin(twopi * (a.k[0] * P[0] + a.k[1] * P[1] + a.k[2] * P[2]));
+    double u[3] = {a.vel[0], a.vel[1], a.vel[2]};
+    for (int mo = 0; mo < a.nmodes; ++mo) {
+      const double arg = twopi * (a.mode_k[mo][0] * P[0] + a.mode_k[mo][1] * P[1] + a.mode_k[mo][2] * P[2]);
+      rho += a.mode_amp * sin(arg + a.mode_phase[mo][0]);
+      u[0] += a.mode_amp * sin(arg + a.mode_phase[mo][1]);
+      u[1] += a.mode_amp * sin(arg + a.mode_phase[mo][2]);
+      u[2] += a.mode_amp * sin(arg + a.mode_phase[mo][3]);
+    }
+    planes[i] = rho;
+    planes[vol + i] = rho * u[0];
+    planes[2 * vol + i] = rho * u[1];
+    planes[3 * vol + i] = rho * u[2];
+    planes[4 * vol + i] = a.p / (kGamma - 1.0) + 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  }
+}
+
+unsigned stride_grid(size_t n) {
+  const size_t b = (n + 255) / 256;
+  return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
+}
+
+}  // namespace
+
+hipError_t launch_godunov_patch(const GodunovPatchArgs& a, hipStream_t s) {
+  const int S = a.L + 2 * a.g;
+  const int n = S * S * S;
+  hipLaunchKernelGGL(godunov_patch_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_godunov_step(const FvGridArgs& a, hipStream_t s) {
+  const int grid = (a.ntiles + 7) / 8 * 8;
+  hipLaunchKernelGGL(fv_godunov_kernel, dim3(grid), dim3(kWave * kFvRows), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_finalize(FvState* st, hipStream_t s) {
+  hipLaunchKernelGGL(fv_finalize_kernel, dim3(1), dim3(64), 0, s, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_initial_dt(const FvGridArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(fv_reset_next_kernel, dim3(1), dim3(64), 0, s, a.state);
+  hipLaunchKernelGGL(fv_stable_dt_kernel, dim3(stride_grid((size_t)a.X * a.Y * a.Z)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(fv_begin_kernel, dim3(1), dim3(64), 0, s, a.state);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_pack(const double* cells, double* planes, int nx, int ny, int nz, int L, hipStream_t s) {
+  const size_t vol = (size_t)nx * ny * nz * L * L * L;
+  hipLaunchKernelGGL(fv_pack_kernel<true>, dim3(stride_grid(vol)), dim3(256), 0, s, cells, planes, nx, ny, nz, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_unpack(const double* planes, double* cells, int nx, int ny, int nz, int L, hipStream_t s) {
+  const size_t vol = (size_t)nx * ny * nz * L * L * L;
+  hipLaunchKernelGGL(fv_pack_kernel<false>, dim3(stride_grid(vol)), dim3(256), 0, s, planes, cells, nx, ny, nz, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_euler_ic(const ICArgs& ic, double* planes, int X, int Y, int Z, const double* h,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(fv_euler_ic_kernel, dim3(stride_grid((size_t)X * Y * Z)), dim3(256), 0, s, ic, planes, X, Y, Z,
+                     h[0], h[1], h[2]);
+  return hipGetLastError();
+}
+
+}  // namespace exa

@@ -60,6 +60,9 @@ struct LimArgs {
   // optional: min over every FV face of 0.99/3 h_d / s_max (the admissible dt
   // musclhancock's solutionUpdate returns), as the bits of a positive double
   unsigned long long* fv_dt_max;
+  // limiter-type godunov: the first-order update (godunov.cpph:35-143) in
+  // place of MUSCL-Hancock; it reads only the first ghost layer
+  int fv_godunov;
 };
 
 enum : int { kLimFlagIrregular = 0, kLimFlagFvCount = 1, kLimFlagRcCount = 2 };
@@ -75,7 +78,8 @@ hipError_t launch_lim_minmax_merge(const SolverDesc& sd, const LimArgs& a, const
 // ghost layers of every listed FV cell (neighbour copies / transmissive);
 // only_if_irregular: do nothing unless flags[irregular] is set
 hipError_t launch_lim_ghosts(const SolverDesc& sd, const LimArgs& a, bool only_if_irregular, hipStream_t s);
-// MUSCL-Hancock update of the listed FV cells + projection to DG (+ patch min/max)
+// MUSCL-Hancock (or, with fv_godunov, Godunov) update of the listed FV cells
+// + projection to DG (+ patch min/max)
 hipError_t launch_lim_fv_update(const SolverDesc& sd, const LimArgs& a, bool only_if_irregular, hipStream_t s);
 // DG update of cells with status < A, then dt, DMP, admissibility and the
 // new limiter status of every cell

@@ -75,7 +75,7 @@ struct exa_lgrid {
   long ncells;
   double lo[3], hi[3], dx[3];
   int periodic;
-  int h, cure;
+  int h, cure, fv_godunov;
   double relax, scaling;
   hipStream_t stream = nullptr;
   double *luh = nullptr, *luh_prev = nullptr, *lduh = nullptr, *lQhbnd = nullptr, *lFhbnd = nullptr;
@@ -124,6 +124,7 @@ exa::LimArgs largs(exa_lgrid* g) {
   a.cfl_pnpm = g->sd.cfl * kPNPM[g->sd.order];
   a.relax = g->relax;
   a.scaling = g->scaling;
+  a.fv_godunov = g->fv_godunov;
   a.T = 2 * g->h + 1;
   a.A = g->h + 1;
   a.cure = g->cure;
@@ -247,13 +248,14 @@ hipError_t lstep(exa_lgrid* g) {
 // ---------------------------------------------------------------------------
 extern "C" int exa_lgrid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo, const double* hi,
                                 int boundary, int helper_layers, int steps_till_cured, double dmp_relaxation,
-                                double dmp_difference_scaling, exa_lgrid** out) {
+                                double dmp_difference_scaling, int fv_scheme, exa_lgrid** out) {
   if (!out) return EXA_E_INVALID;
   *out = nullptr;
   SolverDesc sd;
   if (!to_desc(s, &sd) || nx < 1 || ny < 1 || nz < 1 || !lo || !hi || helper_layers < 1 || steps_till_cured < 0)
     return EXA_E_INVALID;
   if (boundary != EXA_BC_PERIODIC && boundary != EXA_BC_TRANSMISSIVE) return EXA_E_INVALID;
+  if (fv_scheme != EXA_FV_GODUNOV && fv_scheme != EXA_FV_MUSCLHANCOCK) return EXA_E_INVALID;
   if (!exa::limiter_supported(sd) || !exa::supported(sd)) return EXA_E_UNSUPPORTED;
   const long ncells = (long)nx * ny * nz;
   if (ncells > 0x7fffffffL) return EXA_E_INVALID;
@@ -269,6 +271,7 @@ extern "C" int exa_lgrid_create(const exa_solver* s, int nx, int ny, int nz, con
   g->cure = steps_till_cured;
   g->relax = dmp_relaxation;
   g->scaling = dmp_difference_scaling;
+  g->fv_godunov = fv_scheme == EXA_FV_GODUNOV;
   for (int d = 0; d < 3; ++d) {
     g->lo[d] = lo[d];
     g->hi[d] = hi[d];

@@ -28,6 +28,7 @@
 #include <mutex>
 
 #include "exa_device.hpp"
+#include "fv_device.hpp"
 #include "limiter.hpp"
 
 namespace exa {
@@ -48,26 +49,6 @@ struct LimShape {
   static constexpr int TT = V * B * L * L;  // tensor3 temporaries, either direction
 };
 
-__device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min(a, b)
-__device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max(a, b)
-
-// Euler flux in direction d only (the d-th row of the Euler functor's flux,
-// same expression tree).
-__device__ inline void euler_flux_dir(const double* Q, int d, double* F) {
-  const double GAMMA = 1.4;
-  const double irho = 1.0 / Q[0];
-  const double p = (GAMMA - 1.0) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
-  const double qn = Q[d + 1];
-  F[0] = qn;
-#pragma unroll
-  for (int m = 1; m < 4; ++m) F[m] = (m == d + 1) ? irho * qn * Q[m] + p : irho * qn * Q[m];
-  F[4] = irho * qn * (Q[4] + p);
-}
-
-__device__ inline double minmod(double a, double b) {  // MH:27-40
-  if (a * b < 0.0) return 0.0;
-  return (fabs(a) < fabs(b)) ? a : b;
-}
 
 // out[v][Z][Y][X] = sum in[v][z][y][x] T[z][Z] T[y][Y] T[x][X]; in/out/t1/t2
 // in LDS, T (P x M, row = input index) in LDS.  Ends with a barrier.
@@ -327,8 +308,11 @@ __global__ __launch_bounds__(kLimNT) void lim_ghost_kernel(LimArgs a, int only_i
 // a Riemann problem are evolved: the interior and the first ghost layer of
 // the six faces (the reference also evolves other ghost cells whose states
 // no Riemann problem reads).
+// GODUNOV: the first-order update (GOD:35-143) -- the face states are the
+// cell values themselves, the face fluxes and their accumulation order are
+// those of MUSCL-Hancock (the two reference kernels share both).
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, bool GODUNOV>
 __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int only_if_irregular) {
   using Sh = LimShape<N>;
   constexpr int L = Sh::L, G = Sh::G, S = Sh::S, V = Sh::V, PATCH = Sh::PATCH, L3 = Sh::L3, B = Sh::B;
@@ -383,33 +367,39 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
         }
       }
       const int p0 = (z * S + y) * S + x;
-      const int step[3] = {1, S, S * S};
-      double u[V], slope[3][V];
+      double u[V], slope[3][V], Qt[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) u[v] = sP[p0 * V + v];
+      if constexpr (!GODUNOV) {
+        const int step[3] = {1, S, S * S};
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
+        for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int v = 0; v < V; ++v)
-          slope[d][v] = minmod(sP[(p0 + step[d]) * V + v] - u[v], u[v] - sP[(p0 - step[d]) * V + v]) * ih[d];
-      double Qt[V];
+          for (int v = 0; v < V; ++v)
+            slope[d][v] = minmod(sP[(p0 + step[d]) * V + v] - u[v], u[v] - sP[(p0 - step[d]) * V + v]) * ih[d];
 #pragma unroll
-      for (int v = 0; v < V; ++v) Qt[v] = 0.0;
+        for (int v = 0; v < V; ++v) Qt[v] = 0.0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double w[V], F[V];
+        for (int d = 0; d < 3; ++d) {
+          double w[V], F[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) w[v] = u[v] - 0.5 * h[d] * slope[d][v];
-        euler_flux_dir(w, d, F);
+          for (int v = 0; v < V; ++v) w[v] = u[v] - 0.5 * h[d] * slope[d][v];
+          euler_flux_dir(w, d, F);
 #pragma unroll
-        for (int v = 0; v < V; ++v) Qt[v] += F[v] * ih[d];
+          for (int v = 0; v < V; ++v) Qt[v] += F[v] * ih[d];
 #pragma unroll
-        for (int v = 0; v < V; ++v) w[v] = u[v] + 0.5 * h[d] * slope[d][v];
-        euler_flux_dir(w, d, F);
+          for (int v = 0; v < V; ++v) w[v] = u[v] + 0.5 * h[d] * slope[d][v];
+          euler_flux_dir(w, d, F);
 #pragma unroll
-        for (int v = 0; v < V; ++v) Qt[v] -= F[v] * ih[d];
+          for (int v = 0; v < V; ++v) Qt[v] -= F[v] * ih[d];
+        }
       }
-      // evolved face states (wL/wR += dt/2 Qt) into the face table
+      // face states into the face table: the evolved wL/wR (+= dt/2 Qt), or
+      // the cell value itself for the first-order scheme
+      auto face_state = [&](int d, int v, double sign) -> double {
+        if constexpr (GODUNOV) return u[v];
+        return (u[v] + sign * 0.5 * h[d] * slope[d][v]) + 0.5 * dt * Qt[v];
+      };
       const int co[3] = {x - G, y - G, z - G};
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -419,13 +409,11 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
         const bool needR = (sheet < 0) || (sheet == 2 * d);      // wR feeds the face on its right
         if (needL) {
 #pragma unroll
-          for (int v = 0; v < V; ++v)
-            sF[fidx(d, 1, v, fa, fb, co[d])] = (u[v] - 0.5 * h[d] * slope[d][v]) + 0.5 * dt * Qt[v];
+          for (int v = 0; v < V; ++v) sF[fidx(d, 1, v, fa, fb, co[d])] = face_state(d, v, -1.0);
         }
         if (needR) {
 #pragma unroll
-          for (int v = 0; v < V; ++v)
-            sF[fidx(d, 0, v, fa, fb, co[d] + 1)] = (u[v] + 0.5 * h[d] * slope[d][v]) + 0.5 * dt * Qt[v];
+          for (int v = 0; v < V; ++v) sF[fidx(d, 0, v, fa, fb, co[d] + 1)] = face_state(d, v, 1.0);
         }
       }
     }
@@ -443,7 +431,7 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
         qR[v] = sF[fidx(d, 1, v, fa, fb, f)];
       }
       const double smax = dmax(Euler<0>::max_abs_eigenvalue(qR, d), Euler<0>::max_abs_eigenvalue(qL, d));
-      const double dta = 0.99 / 3 * h[d] / smax;
+      const double dta = (GODUNOV ? 0.9 : 0.99) / 3 * h[d] / smax;  // FV solver CFL / MH:29
       dt_allowed = dta < dt_allowed ? dta : dt_allowed;
       euler_flux_dir(qL, d, FL);
       euler_flux_dir(qR, d, FR);
@@ -1278,8 +1266,12 @@ hipError_t launch_lim_ghosts(const SolverDesc& sd, const LimArgs& a, bool only_i
 
 hipError_t launch_lim_fv_update(const SolverDesc& sd, const LimArgs& a, bool only_if_irregular, hipStream_t s) {
   return dispatch_lim(sd.order, [&]<int N>() -> hipError_t {
-    hipLaunchKernelGGL(lim_fv_update_kernel<N>, dim3(persistent_grid(a.ncells)), dim3(kMusclNT), 0, s, a,
-                       only_if_irregular ? 1 : 0);
+    if (a.fv_godunov)
+      hipLaunchKernelGGL((lim_fv_update_kernel<N, true>), dim3(persistent_grid(a.ncells)), dim3(kMusclNT), 0, s, a,
+                         only_if_irregular ? 1 : 0);
+    else
+      hipLaunchKernelGGL((lim_fv_update_kernel<N, false>), dim3(persistent_grid(a.ncells)), dim3(kMusclNT), 0, s, a,
+                         only_if_irregular ? 1 : 0);
     return hipGetLastError();
   });
 }

@@ -201,6 +201,9 @@ int exa_discrete_maximum_principle_and_min_and_max_search(const double* luh, int
                                                           int basis_size, double relaxation,
                                                           double difference_scaling, double* bmin,
                                                           double* bmax, int* dmp_ok);
+/* Finite-volume schemes (the spec file's FV `type` / limiter-type). */
+enum { EXA_FV_GODUNOV = 0, EXA_FV_MUSCLHANCOCK = 1 };
+
 /* musclhancock::c::solutionUpdate (ExaHyPE/kernels/finitevolumes/musclhancock/
  * c/3d/musclhancock.cpph:49) for Euler: luh_new = update(luh) on one patch
  * with filled ghost layers (patch_size L, ghost_layer_width 2); dx is the
@@ -211,6 +214,59 @@ int exa_musclhancock_solution_update(const exa_solver* s, double* luh_new, const
                                      const double* dx, double dt, int patch_size,
                                      int ghost_layer_width, double* dt_max_allowed);
 
+/* godunov::c::solutionUpdate (ExaHyPE/kernels/finitevolumes/godunov/c/3d/
+ * godunov.cpph:35) for Euler (flux only): first-order update of one patch
+ * (L+2g)^3 with filled ghost layers, any patch_size L >= 1 and
+ * ghost_layer_width g >= 1 (the reference's Godunov solvers use g = 1).
+ * luh_new must hold a copy of luh on entry (FiniteVolumesSolver.cpp:669-671)
+ * and receives the reference's writes: the interior update and the
+ * boundary-face fluxes added to the first ghost layer.  dx is the patch
+ * (cell) size.  *dt_max_allowed (may be NULL) receives the function's return
+ * value: the minimum over the patch's faces of s->cfl/3 * h_d / s_max. */
+int exa_godunov_solution_update(const exa_solver* s, double* luh_new, const double* luh,
+                                const double* dx, double dt, int patch_size,
+                                int ghost_layer_width, double* dt_max_allowed);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident finite-volumes grid: one uniform level of
+ * FiniteVolumesSolver cells (ExaHyPE/exahype/solvers/FiniteVolumesSolver.cpp),
+ * each a patch of L^3 subcells, under global time stepping.  One step:
+ * neighbour merging (ghostLayerFilling, FVS:767-803; transmissive boundary
+ * FVS:805-846), updateSolution with the scheme's solutionUpdate (FVS:662-715),
+ * startNewTimeStep's stableTimeStepSize of the new solution with the solver's
+ * CFL (FVS:553-574), dt = minimum over cells.  Euler (flux only), FP64,
+ * scheme EXA_FV_GODUNOV.  Subcells live in dense planes without ghost layers:
+ * neighbour reads wrap (periodic) or clamp to the boundary subcell
+ * (transmissive, the reference's mirror for ghost width 1), which gives the
+ * values the reference's ghost layers hold.
+ * ------------------------------------------------------------------------- */
+typedef struct exa_fvgrid exa_fvgrid;
+/* nx*ny*nz cells on [lo, hi], cell c = (k*ny + j)*nx + i, patch_size L;
+ * boundary EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE (boundaryValues returning
+ * the inside state).  EXA_FV_MUSCLHANCOCK returns EXA_E_UNSUPPORTED. */
+int exa_fvgrid_create(const exa_solver* s, int scheme, int patch_size, int nx, int ny, int nz,
+                      const double* lo, const double* hi, int boundary, exa_fvgrid** out);
+int exa_fvgrid_destroy(exa_fvgrid* g);
+long exa_fvgrid_num_cells(const exa_fvgrid* g);
+/* Solution [cells][L^3][5] (subcells z,y,x, variable fastest).  Upload and
+ * init_euler restart the clock: t = 0 and dt = the new solution's stable dt. */
+int exa_fvgrid_upload(exa_fvgrid* g, const double* interior);
+int exa_fvgrid_download(exa_fvgrid* g, double* interior);
+/* The exa_grid_init_euler state evaluated at the subcell centres. */
+int exa_fvgrid_init_euler(exa_fvgrid* g, const exa_euler_ic* ic);
+/* Advance nsteps time steps (asynchronous on the grid's stream). */
+int exa_fvgrid_step(exa_fvgrid* g, int nsteps);
+int exa_fvgrid_synchronize(exa_fvgrid* g);
+/* Time stamp, time step size of the next step and completed steps;
+ * EXA_E_NONFINITE once a step produced a non-finite or non-positive dt. */
+int exa_fvgrid_time(exa_fvgrid* g, double* t, double* dt, long* steps);
+/* HIP-event timing of the update kernel (Godunov + stable dt of the new
+ * solution) and of the time-stamp kernel per step while profiling is on;
+ * stage_times synchronises, returns the sums and launch counts, and resets. */
+enum { EXA_FV_STAGE_UPDATE = 0, EXA_FV_STAGE_FINALIZE = 1, EXA_FV_NUM_STAGES = 2 };
+int exa_fvgrid_set_profiling(exa_fvgrid* g, int on);
+int exa_fvgrid_stage_times(exa_fvgrid* g, double* ms, long* launches);
+
 /* Device-resident limiting grid: LimitingADERDGSolver's non-fused time step
  * (ExaHyPE/exahype/solvers/LimitingADERDGSolver.cpp:806-1742, Runner.cpp:866-946,
  * 1116-1162) -- DG / FV neighbour merging by limiter status, status-dependent
@@ -218,10 +274,13 @@ int exa_musclhancock_solution_update(const exa_solver* s, double* luh_new, const
  * recomputation -- on one uniform level. */
 typedef struct exa_lgrid exa_lgrid;
 /* boundary: EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE; helper_layers h gives
- * troubled status 2h+1 and FV-active status >= h+1 (ADERDGSolver.cpp:461-462). */
+ * troubled status 2h+1 and FV-active status >= h+1 (ADERDGSolver.cpp:461-462).
+ * fv_scheme: the spec file's limiter-type, EXA_FV_MUSCLHANCOCK or
+ * EXA_FV_GODUNOV (the generated FV solver's solutionUpdate,
+ * Toolkit/src/eu/exahype/solvers/GenericFiniteVolumes{MUSCLHancock,Godunov}InC.java). */
 int exa_lgrid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo, const double* hi,
                      int boundary, int helper_layers, int steps_till_cured, double dmp_relaxation,
-                     double dmp_difference_scaling, exa_lgrid** out);
+                     double dmp_difference_scaling, int fv_scheme, exa_lgrid** out);
 int exa_lgrid_destroy(exa_lgrid* g);
 long exa_lgrid_num_cells(const exa_lgrid* g);
 /* Initial state: DG solution luh [cells][B^3][V] and the FV initial condition

@@ -570,7 +570,7 @@ struct ora_lgrid {
   long ncells;
   double dx[3], lo[3];
   double t, dt;
-  int h, T, A, cure, nobs, L, g, S;
+  int h, T, A, cure, nobs, L, g, S, fv_scheme;
   double relax, scaling;
   long n_luh, n_lim, n_qb, n_fb, n_du;
   double *luh, *luh_prev, *lduh, *lQhbnd, *lFhbnd, *lim, *lim_prev, *mn, *mx, *dtc;
@@ -596,7 +596,7 @@ static long lg_cell(const ora_lgrid* g, int i, int j, int k) {
 
 ora_lgrid* ora_lgrid_create(const ora_solver* s, int nx, int ny, int nz, const double* lo, const double* hi,
                             int periodic, int helper_layers, int steps_till_cured, double relax,
-                            double scaling) {
+                            double scaling, int fv_scheme) {
   ora_lgrid* g = (ora_lgrid*)calloc(1, sizeof(ora_lgrid));
   g->s = *s;
   g->nx = nx;
@@ -616,7 +616,8 @@ ora_lgrid* ora_lgrid_create(const ora_solver* s, int nx, int ny, int nz, const d
   g->scaling = scaling;
   g->nobs = s->nvar;
   g->L = 2 * s->order + 1;
-  g->g = 2;
+  g->fv_scheme = fv_scheme;
+  g->g = fv_scheme == ORA_FV_GODUNOV ? 1 : 2;
   g->S = g->L + 2 * g->g;
   const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
   g->n_luh = B * B * B * D;
@@ -746,7 +747,12 @@ static void lg_dg_update(ora_lgrid* g, long c) {
 /* FiniteVolumesSolver::updateSolution (FVS:662-700) + projectOnDGSpace */
 static void lg_fv_update(ora_lgrid* g, long c) {
   memcpy(g->lim_prev + c * g->n_lim, LG_LIM(g, c), sizeof(double) * g->n_lim);
-  if (g->dt > 0) ora_muscl_update(&g->s, g->L, g->g, LG_LIM(g, c), g->lim_prev + c * g->n_lim, g->dx, g->dt);
+  if (g->dt > 0) {
+    if (g->fv_scheme == ORA_FV_GODUNOV) /* the FV solver's CFL only scales the unused dt_max_allowed */
+      ora_godunov_update(&g->s, g->L, g->g, LG_LIM(g, c), g->lim_prev + c * g->n_lim, g->dx, g->dt, 0.9);
+    else
+      ora_muscl_update(&g->s, g->L, g->g, LG_LIM(g, c), g->lim_prev + c * g->n_lim, g->dx, g->dt);
+  }
   lg_project_dg(g, c);
 }
 

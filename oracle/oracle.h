@@ -114,11 +114,33 @@ void ora_fv_ghost_fill(int L, int g, int D, double* luh, const double* nb, int d
 void ora_fv_boundary_transmissive(int L, int g, int D, double* luh, int face);
 double ora_fv_stable_dt(const ora_solver* s, int L, int g, const double* luh, const double* dx, double cfl);
 
-/* Limiting grid (uniform, periodic or transmissive), restated driver. */
+/* ---- finite volumes (finitevolumes.c) ---- */
+enum { ORA_FV_GODUNOV = 0, ORA_FV_MUSCLHANCOCK = 1 };
+/* godunov::c::solutionUpdate for a flux-only PDE; returns dt_max_allowed. */
+double ora_godunov_update(const ora_solver* s, int L, int g, double* luh_new, const double* luh,
+                          const double* dx, double dt, double cfl);
+/* Stand-alone finite-volumes grid: nx*ny*nz cells of L^3 subcells, ghost
+ * width 1 (Godunov) or 2 (MUSCL-Hancock), cell c = (k*ny + j)*nx + i;
+ * periodic, or transmissive boundaries.  s->cfl is the solver's CFL. */
+typedef struct ora_fvgrid ora_fvgrid;
+ora_fvgrid* ora_fvgrid_create(const ora_solver* s, int scheme, int L, int nx, int ny, int nz, const double* lo,
+                              const double* hi, int periodic);
+void ora_fvgrid_destroy(ora_fvgrid* g);
+long ora_fvgrid_num_cells(const ora_fvgrid* g);
+/* interior [ncells][L^3][V+P] (z,y,x,var) */
+void ora_fvgrid_set_solution(ora_fvgrid* g, const double* interior);
+void ora_fvgrid_get_solution(const ora_fvgrid* g, double* interior);
+void ora_fvgrid_get_patch(const ora_fvgrid* g, long c, double* patch);
+void ora_fvgrid_step(ora_fvgrid* g, int nsteps, double* dt_out);
+double ora_fvgrid_time(const ora_fvgrid* g);
+double ora_fvgrid_dt(const ora_fvgrid* g);
+
+/* Limiting grid (uniform, periodic or transmissive), restated driver;
+ * fv_scheme ORA_FV_MUSCLHANCOCK (ghost width 2) or ORA_FV_GODUNOV (1). */
 typedef struct ora_lgrid ora_lgrid;
 ora_lgrid* ora_lgrid_create(const ora_solver* s, int nx, int ny, int nz, const double* lo, const double* hi,
                             int periodic, int helper_layers, int steps_till_cured, double relax,
-                            double scaling);
+                            double scaling, int fv_scheme);
 void ora_lgrid_destroy(ora_lgrid* g);
 long ora_lgrid_num_cells(const ora_lgrid* g);
 /* lim_ic: FV initial condition at subcell centres [ncells][L^3][V+P] or NULL */

@@ -92,7 +92,23 @@ def lib():
         L.ora_fv_boundary_transmissive.argtypes = [I, I, I, P, I]
         L.ora_fv_stable_dt.argtypes = [S, I, I, P, P, D_]
         L.ora_fv_stable_dt.restype = D_
-        L.ora_lgrid_create.argtypes = [S, I, I, I, P, P, I, I, I, D_, D_]
+        L.ora_godunov_update.argtypes = [S, I, I, P, P, P, D_, D_]
+        L.ora_godunov_update.restype = D_
+        VP = ctypes.c_void_p
+        L.ora_fvgrid_create.argtypes = [S, I, I, I, I, I, P, P, I]
+        L.ora_fvgrid_create.restype = VP
+        L.ora_fvgrid_destroy.argtypes = [VP]
+        L.ora_fvgrid_num_cells.argtypes = [VP]
+        L.ora_fvgrid_num_cells.restype = ctypes.c_long
+        L.ora_fvgrid_set_solution.argtypes = [VP, P]
+        L.ora_fvgrid_get_solution.argtypes = [VP, P]
+        L.ora_fvgrid_get_patch.argtypes = [VP, ctypes.c_long, P]
+        L.ora_fvgrid_step.argtypes = [VP, I, P]
+        L.ora_fvgrid_time.argtypes = [VP]
+        L.ora_fvgrid_time.restype = D_
+        L.ora_fvgrid_dt.argtypes = [VP]
+        L.ora_fvgrid_dt.restype = D_
+        L.ora_lgrid_create.argtypes = [S, I, I, I, P, P, I, I, I, D_, D_, I]
         L.ora_lgrid_create.restype = ctypes.c_void_p
         L.ora_lgrid_destroy.argtypes = [ctypes.c_void_p]
         L.ora_lgrid_num_cells.argtypes = [ctypes.c_void_p]
@@ -326,6 +342,69 @@ def muscl_update(s, L, g, patch, dx, dt):
     return new, dtm
 
 
+FV_GODUNOV, FV_MUSCLHANCOCK = 0, 1
+
+
+def godunov_update(s, L, g, patch, dx, dt, cfl=0.9, luh_new=None):
+    """One Godunov update of a patch with filled ghost layers; returns (new
+    patch, dt_max).  luh_new (default: a copy of the patch) is the array the
+    update accumulates onto."""
+    old = np.ascontiguousarray(patch, dtype=np.float64).ravel()
+    new = old.copy() if luh_new is None else np.array(luh_new, dtype=np.float64).ravel()
+    dtm = lib().ora_godunov_update(ctypes.byref(s), L, g, _p(new), _p(old), _p(dx3(dx)), dt, cfl)
+    return new, dtm
+
+
+class FVGrid:
+    """Stand-alone finite-volumes grid (restated FiniteVolumesSolver time step)."""
+
+    def __init__(self, s, scheme, L, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), periodic=True):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.s, self.L, self.n = s, L, tuple(n)
+        self.g = 1 if scheme == FV_GODUNOV else 2
+        self._h = lib().ora_fvgrid_create(ctypes.byref(s), scheme, L, n[0], n[1], n[2], _p(dx3(lo)), _p(dx3(hi)),
+                                          int(periodic))
+        assert self._h
+        self.ncells = lib().ora_fvgrid_num_cells(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ora_fvgrid_destroy(self._h)
+            self._h = None
+
+    def set_solution(self, interior):
+        lib().ora_fvgrid_set_solution(self._h, _p(np.ascontiguousarray(interior, dtype=np.float64).ravel()))
+
+    def solution(self):
+        out = np.zeros(self.ncells * self.L ** 3 * (self.s.nvar + self.s.npar))
+        lib().ora_fvgrid_get_solution(self._h, _p(out))
+        return out
+
+    def patch(self, c):
+        S = self.L + 2 * self.g
+        out = np.zeros(S ** 3 * (self.s.nvar + self.s.npar))
+        lib().ora_fvgrid_get_patch(self._h, c, _p(out))
+        return out
+
+    def step(self, nsteps):
+        dts = np.zeros(nsteps)
+        lib().ora_fvgrid_step(self._h, nsteps, _p(dts))
+        return dts
+
+    def time(self):
+        return lib().ora_fvgrid_time(self._h)
+
+    def dt(self):
+        return lib().ora_fvgrid_dt(self._h)
+
+
+def fv_stable_dt(s, L, g, patch, dx, cfl):
+    """FV stableTimeStepSize of a patch (interior subcells)."""
+    return lib().ora_fv_stable_dt(ctypes.byref(s), L, g, _p(np.ascontiguousarray(patch, dtype=np.float64).ravel()),
+                                  _p(dx3(dx)), cfl)
+
+
 def fv_ghost_fill(L, g, D, patch, neighbour, direction, side):
     out = np.array(patch, dtype=np.float64).ravel()
     lib().ora_fv_ghost_fill(L, g, D, _p(out), _p(np.ascontiguousarray(neighbour, dtype=np.float64).ravel()),
@@ -343,16 +422,16 @@ class LimitingGrid:
     """Uniform grid with the restated limiting driver (non-fused time stepping)."""
 
     def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), periodic=False, helper_layers=1,
-                 steps_till_cured=0, relax=1e-4, scaling=1e-3):
+                 steps_till_cured=0, relax=1e-4, scaling=1e-3, fv_scheme=FV_MUSCLHANCOCK):
         if isinstance(n, int):
             n = (n, n, n)
         self.s = s
         self.n = tuple(n)
         self._h = lib().ora_lgrid_create(ctypes.byref(s), n[0], n[1], n[2], _p(dx3(lo)), _p(dx3(hi)), int(periodic),
-                                         helper_layers, steps_till_cured, relax, scaling)
+                                         helper_layers, steps_till_cured, relax, scaling, fv_scheme)
         self.ncells = lib().ora_lgrid_num_cells(self._h)
         self.L = 2 * s.order + 1
-        self.S = self.L + 4
+        self.S = self.L + (2 if fv_scheme == FV_GODUNOV else 4)
 
     def __del__(self):
         if getattr(self, "_h", None):

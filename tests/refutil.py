@@ -223,3 +223,36 @@ def elastic_matrices(lam, mu, rho):
     Bx[7, 3], By[7, 1], Bz[7, 4] = -ir, -ir, -ir
     Bx[8, 5], By[8, 4], Bz[8, 2] = -ir, -ir, -ir
     return Bx, By, Bz
+
+
+def fv_state(rho, u, v, w, p):
+    """Conserved Euler variables from primitive arrays (any shape), stacked last."""
+    return np.stack([rho, rho * u, rho * v, rho * w, p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w)], -1)
+
+
+def fv_smooth(Z, Y, X):
+    """A smooth periodic Euler state on [0,1]^3 with all velocity components."""
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * (X + 2 * Y)) * np.cos(2 * np.pi * Z)
+    u = 0.3 + 0.1 * np.cos(2 * np.pi * Y)
+    v = -0.2 + 0.1 * np.sin(2 * np.pi * Z)
+    w = 0.1 * np.cos(2 * np.pi * X)
+    p = 1.0 + 0.1 * np.sin(2 * np.pi * (X - Z))
+    return fv_state(rho, u, v, w, p)
+
+
+def fv_grid_state(n, L, fn, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    """fn(Z, Y, X) at the subcell centres of an n = (nx, ny, nz) grid of L^3
+    patches: returns (global array [Z][Y][X][5], cell-major array
+    [cells][L][L][L][5] with cell c = (k ny + j) nx + i)."""
+    nx, ny, nz = n
+    cs = [lo[d] + (np.arange(n[d] * L) + 0.5) * (hi[d] - lo[d]) / (n[d] * L) for d in range(3)]
+    Z, Y, X = np.meshgrid(cs[2], cs[1], cs[0], indexing="ij")
+    G = fn(Z, Y, X)
+    cells = G.reshape(nz, L, ny, L, nx, L, 5).transpose(0, 2, 4, 1, 3, 5, 6).reshape(nx * ny * nz, L, L, L, 5)
+    return G, cells
+
+
+def fv_cells_to_global(cells, n, L):
+    nx, ny, nz = n
+    return np.asarray(cells).reshape(nz, ny, nx, L, L, L, 5).transpose(0, 3, 1, 4, 2, 5, 6).reshape(
+        nz * L, ny * L, nx * L, 5)

@@ -770,11 +770,6 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
 // Surface integral + solution update + the cell's stable time step, in double
 // on the double solution; TF is the storage type of lduh and the face fluxes.
 // ---------------------------------------------------------------------------
-__device__ inline void atomic_min_positive(unsigned long long* slot, double v) {
-  // For non-negative doubles the IEEE bit patterns order like the values.
-  atomicMin(slot, (unsigned long long)__double_as_longlong(v));
-}
-
 template <int B, class PDE, bool SURFACE, bool UPDATE, bool WRITE_LDUH, typename TF>
 __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
@@ -858,12 +853,10 @@ __global__ __launch_bounds__(256) void dt_finalize_kernel(StepState* st, int adv
   __shared__ double sRed[256 / kWave];
   double m = 1.0e308;
   for (int i = threadIdx.x; i < kDtSlots; i += 256) {
-    const double v = __longlong_as_double((long long)st->dt_slots[i]);
+    const double v = __longlong_as_double((long long)atomic_take(&st->dt_slots[i], kDtSlotEmpty));
     m = v < m ? v : m;
   }
   m = block_min<256>(m, sRed);
-  __syncthreads();
-  for (int i = threadIdx.x; i < kDtSlots; i += 256) st->dt_slots[i] = kDtSlotEmpty;
   if (threadIdx.x == 0) {
     if (advance_time) {
       st->t += st->dt;
@@ -1171,9 +1164,8 @@ __global__ __launch_bounds__(256) void patch_dt_finalize_kernel(double* dt, doub
   if (p >= npatches) return;
   double m = DBL_MAX;
   for (int i = 0; i < sp; ++i) {
-    const double v = __longlong_as_double((long long)slots[(size_t)p * sp + i]);
+    const double v = __longlong_as_double((long long)atomic_take(&slots[(size_t)p * sp + i], kDtSlotEmpty));
     m = v < m ? v : m;
-    slots[(size_t)p * sp + i] = kDtSlotEmpty;
   }
   if (advance) t[p] += dt[p];
   dt[p] = m;

@@ -57,6 +57,23 @@ __device__ inline T segment_sum(T v) {
   return v;
 }
 
+// Atomic minimum of a non-negative double kept as its bit pattern (for
+// non-negative doubles the IEEE bit patterns order like the values).  The
+// atomic returns and its result is consumed, so the wavefront waits until
+// the atomic has been performed: the slot is read by the next kernel on the
+// stream, and a small kernel launched right behind a producer has been seen
+// to read a slot before the producer's last no-return atomic landed.
+__device__ inline void atomic_min_positive(unsigned long long* slot, double v) {
+  const unsigned long long old = atomicMin(slot, (unsigned long long)__double_as_longlong(v));
+  asm volatile("" : : "v"(old));
+}
+
+// Reads a slot written by atomics of an earlier kernel and resets it, at the
+// atomics' coherence point (not through a cache that may hold an old copy).
+__device__ inline unsigned long long atomic_take(unsigned long long* slot, unsigned long long reset) {
+  return atomicExch(slot, reset);
+}
+
 template <typename T>
 __device__ inline T wave_min(T v) {
 #pragma unroll

@@ -26,13 +26,19 @@ struct GodunovPatchArgs {
 };
 hipError_t launch_godunov_patch(const GodunovPatchArgs& a, hipStream_t s);
 
-// Time-step state of a finite-volumes grid (device resident).
+// Time-step state of a finite-volumes grid (device resident).  The new
+// solution's stable dt, min over subcells and directions of
+// (cfl/3 h_d) / max|lambda_d| (FVC:409-460), is formed as
+// min over d of ch[d] / (max over subcells of max|lambda_d|): a correctly
+// rounded division by a positive number is monotone, so both give the same
+// double.  The update kernel only takes maxima.
 struct FvState {
-  double t;                   // time stamp
-  double dt;                  // time step size of the next update
-  unsigned long long dt_next; // running minimum of the new solution's cell dts (double bits)
+  double t;                       // time stamp
+  double dt;                      // time step size of the next update
+  unsigned long long lam_max[3];  // running maxima of max|lambda_d| (double bits; 0 = none)
+  double ch[3];                   // cfl / 3 * h_d
   long long steps;
-  int bad;                    // a non-finite or non-positive dt was produced
+  int bad;                        // a non-finite or non-positive dt was produced
 };
 
 // The grid's solution is one plane set per variable, u[v][Z][Y][X] with
@@ -46,16 +52,15 @@ struct FvGridArgs {
   int X, Y, Z;
   int periodic;
   double h[3];   // subcell sizes dx_d / L
-  double cfl;    // the solver's CFL (stableTimeStepSize's factor is cfl/3)
   FvState* state;
   int nsx, nry, nzc, zlen, ntiles;  // tiling of the update kernel
 };
 // One Godunov update u -> un with dt = state->dt, fused with the new
-// solution's stable dt (atomic minimum into state->dt_next).
+// solution's eigenvalue maxima (atomic maxima into state->lam_max).
 hipError_t launch_fv_godunov_step(const FvGridArgs& a, hipStream_t s);
-// t += dt; dt = dt_next; dt_next = max; steps += 1.
+// t += dt; dt = min_d ch[d] / lam_max[d]; lam_max = 0; steps += 1.
 hipError_t launch_fv_finalize(FvState* st, hipStream_t s);
-// Stable dt of u into state->dt_next, then t = 0, dt = that minimum.
+// Eigenvalue maxima of u, then t = 0 and dt as above.
 hipError_t launch_fv_initial_dt(const FvGridArgs& a, hipStream_t s);
 // Host layout [cells][L^3][5] (cell-major, subcells z,y,x) <-> planes.
 hipError_t launch_fv_pack(const double* cells, double* planes, int nx, int ny, int nz, int L, hipStream_t s);
@@ -64,11 +69,10 @@ hipError_t launch_fv_unpack(const double* planes, double* cells, int nx, int ny,
 hipError_t launch_fv_euler_ic(const ICArgs& ic, double* planes, int X, int Y, int Z, const double* h,
                               hipStream_t s);
 
-// Workgroup shape of the update kernel: 8 wavefronts, each a row of 64
-// consecutive subcells along x; rows 1..6 and lanes 1..62 produce output,
-// the outer rows and lanes supply neighbour values.
+// Workgroup shape of the update kernel: kFvRows wavefronts, each a row of 64
+// consecutive subcells along x; rows 1..kFvRows-2 and lanes 1..62 produce
+// output, the outer rows and lanes supply neighbour values.
 constexpr int kFvRows = 8;
-constexpr int kFvOutRows = kFvRows - 2;
 constexpr int kFvOutX = 64 - 2;
 
 }  // namespace exa

@@ -85,16 +85,17 @@ void fv_free(exa_fvgrid* g) {
 }
 
 // Tiling of the update kernel: 62 x 6 columns per workgroup, the z range cut
-// into chunks so that the launch has about 16 workgroups per CU in total
-// (eight rounds of two resident workgroups), chunks of at least 16 planes.
+// into chunks so that the launch has about 8 workgroups per CU in total,
+// chunks of at least 16 planes.
 void fv_tiling(exa_fvgrid* g) {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   exa::FvGridArgs& a = g->args;
+  constexpr int out_rows = exa::kFvRows - 2;
   a.nsx = (g->X + exa::kFvOutX - 1) / exa::kFvOutX;
-  a.nry = (g->Y + exa::kFvOutRows - 1) / exa::kFvOutRows;
+  a.nry = (g->Y + out_rows - 1) / out_rows;
   const long per_chunk = (long)a.nsx * a.nry;
-  long nzc = (16L * cus + per_chunk - 1) / per_chunk;
+  long nzc = (8L * cus + per_chunk - 1) / per_chunk;
   const long max_chunks = g->Z / 16 > 1 ? g->Z / 16 : 1;
   if (nzc > max_chunks) nzc = max_chunks;
   if (nzc < 1) nzc = 1;
@@ -147,7 +148,9 @@ extern "C" int exa_fvgrid_create(const exa_solver* s, int scheme, int patch_size
   if (e == hipSuccess) e = hipMalloc((void**)&g->u, sizeof(double) * 5 * g->vol);
   if (e == hipSuccess) e = hipMalloc((void**)&g->un, sizeof(double) * 5 * g->vol);
   if (e == hipSuccess) e = hipMalloc((void**)&g->state, sizeof(exa::FvState));
-  if (e == hipSuccess) e = hipMemsetAsync(g->state, 0, sizeof(exa::FvState), g->stream);
+  exa::FvState st0{};
+  for (int d = 0; d < 3; ++d) st0.ch[d] = sd.cfl / 3 * g->h[d];  // FVC:409-460's cfl/3 * h_d
+  if (e == hipSuccess) e = hipMemcpyAsync(g->state, &st0, sizeof(st0), hipMemcpyHostToDevice, g->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
   if (e != hipSuccess) {
     fv_free(g);
@@ -159,7 +162,6 @@ extern "C" int exa_fvgrid_create(const exa_solver* s, int scheme, int patch_size
   a.Z = g->Z;
   a.periodic = g->periodic;
   for (int d = 0; d < 3; ++d) a.h[d] = g->h[d];
-  a.cfl = sd.cfl;
   a.state = g->state;
   fv_tiling(g);
   *out = g;

@@ -73,20 +73,6 @@ __device__ inline double face_flux(const double* qa, const double* qb, int d, do
   return dmax(lb, la);
 }
 
-// FV stableTimeStepSize of one subcell (FVC:409-460): min over d of
-// cfl/3 * h_d / max|lambda_d|, with ch[d] = cfl/3 * h_d.
-__device__ inline double cell_stable_dt(const double* q, const double* ch) {
-  const CellBase b = cell_base(q);
-  double dt = DBL_MAX;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) dt = dmin(dt, ch[d] / cell_lambda(q, b, d));
-  return dt;
-}
-
-__device__ inline void atomic_min_positive(unsigned long long* slot, double v) {
-  atomicMin(slot, (unsigned long long)__double_as_longlong(v));
-}
-
 // ---------------------------------------------------------------------------
 // Single patch (GOD:35-143).  One thread per subcell of the (S,S,S) patch:
 // interior subcells take their six faces in the reference's order; a
@@ -163,23 +149,26 @@ __global__ __launch_bounds__(256) void godunov_patch_kernel(GodunovPatchArgs a) 
 }
 
 // ---------------------------------------------------------------------------
-// Grid update.  A workgroup owns a tile of 62 x 6 subcell columns and walks a
-// range of z planes.  Wavefront w holds row y0 - 1 + w, lane l column
-// x0 - 1 + l, so the outer rows and lanes carry the neighbours of the tile.
-// Per plane and column the subcell's pressure and sound speed are formed
-// once, each face flux once:
+// Grid update.  A workgroup owns a tile of 62 x 6 subcell columns and
+// walks a range of z planes.  Wavefront w holds row y0 - 1 + w, lane l
+// column x0 - 1 + l, so the outer rows and lanes carry the neighbours of the
+// tile.  Per plane and column the subcell's pressure and sound speed are
+// formed once, each face flux once:
 //   x faces: lane l forms the face to its right from lane l+1's state,
-//            directional flux and eigenvalue (cross-lane shuffles), and
-//            passes it to lane l+1 as that lane's left face;
+//            directional flux and eigenvalue (DPP lane shifts), and passes
+//            it to lane l+1 as that lane's left face;
 //   y faces: rows exchange state, y flux and eigenvalue through LDS; row w
 //            forms the face above it and leaves it in LDS for row w+1;
 //   z faces: carried along the walk (the face above plane z is the face
 //            below plane z+1).
-// The update is followed by the new subcell's stable dt (startNewTimeStep,
-// FVS:553-574), reduced over the wavefront and into state->dt_next.
-// Reads per step: each subcell once from the tile, plus the tile borders
-// (1/31 along x, 1/3 along y, read again by the neighbouring tile, mostly
-// from L2); writes: each subcell once.
+// The update is followed by the new subcell's eigenvalue maxima for the next
+// stable dt (startNewTimeStep, FVS:553-574), reduced over the wavefront and
+// into state->lam_max.  Reads per step: each subcell once from the tile, plus
+// the tile borders (1/31 along x, 1/3 along y, read again by the
+// neighbouring tile, partly from L2); writes: each subcell once.  Eight
+// wavefronts per workgroup and two workgroups per CU (114 VGPRs, 64 KiB of
+// LDS each) measured faster than one workgroup of sixteen, whose barriers
+// hold all sixteen wavefronts of the CU.
 // ---------------------------------------------------------------------------
 __device__ inline int wrap_index(int i, int n, int periodic) {
   if (periodic) {
@@ -189,20 +178,51 @@ __device__ inline int wrap_index(int i, int n, int periodic) {
   return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
+// The value of lane l+1 (wave_shl:1) / lane l-1 (wave_shr:1) of a double;
+// the last / first lane receives 0 (not used).
+__device__ inline double from_next_lane(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double from_prev_lane(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// The state's maxima are read by the next kernel on the stream.  A returning
+// atomic whose result is consumed makes the wavefront wait until the atomic
+// has been performed (a one-wavefront kernel launched right behind has been
+// seen to read a maximum before the producer's last no-return atomic landed).
+__device__ inline void atomic_max_positive(unsigned long long* slot, double v) {
+  const unsigned long long old = atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+  asm volatile("" : : "v"(old));
+}
+
+// Wavefront maximum of one direction's eigenvalue maxima into the state.
+__device__ inline void reduce_lambda(FvState* st, int d, double lm) {
+  const double m = -wave_min(-lm);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m > 0.0) atomic_max_positive(&st->lam_max[d], m);
+}
+
 __global__ __launch_bounds__(kWave * kFvRows) void fv_godunov_kernel(FvGridArgs a) {
-  __shared__ double sQ[kFvRows][11][kWave];  // state (5), y flux (5), y eigenvalue
-  __shared__ double sFy[kFvRows][5][kWave];  // face above the row
+  constexpr int ROWS = kFvRows, OUT_ROWS = ROWS - 2;
+  __shared__ double sQ[ROWS][11][kWave];  // state (5), y flux (5), y eigenvalue
+  __shared__ double sFy[ROWS][5][kWave];  // face above the row
   // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a
-  // contiguous run of tiles (rows fastest), so tiles that share border rows
-  // share that XCD's L2
+  // contiguous run of tiles with the x strip fastest, so that the tiles that
+  // share border columns (x neighbours, adjacent in the run) and border rows
+  // (y neighbours, nsx apart) are resident on that XCD at the same time and
+  // walk their planes together: the shared cache lines come from its L2
   const int per = (int)(gridDim.x / 8);
   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (t >= a.ntiles) return;  // the whole workgroup
-  const int ry = t % a.nry, r2 = t / a.nry, sx = r2 % a.nsx, cz = r2 / a.nsx;
+  const int sx = t % a.nsx, r2 = t / a.nsx, ry = r2 % a.nry, cz = r2 / a.nry;
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int xp = sx * kFvOutX - 1 + lane, yp = ry * kFvOutRows - 1 + w;
+  const int xp = sx * kFvOutX - 1 + lane, yp = ry * OUT_ROWS - 1 + w;
   const int xs = wrap_index(xp, a.X, a.periodic), ys = wrap_index(yp, a.Y, a.periodic);
-  const bool out_row = w >= 1 && w <= kFvOutRows && yp < a.Y;  // uniform per wavefront
+  const bool out_row = w >= 1 && w <= OUT_ROWS && yp < a.Y;  // uniform per wavefront
   const bool out = out_row && lane >= 1 && lane <= kFvOutX && xp < a.X;
   const bool face_above = out_row || w == 0;
   const size_t plane = (size_t)a.X * a.Y, vol = plane * (size_t)a.Z;
@@ -210,93 +230,88 @@ __global__ __launch_bounds__(kWave * kFvRows) void fv_godunov_kernel(FvGridArgs 
   const int z0 = cz * a.zlen, z1 = min(a.Z, z0 + a.zlen);
   const double dt = a.state->dt;
   const double cdt[3] = {dt / a.h[0], dt / a.h[1], dt / a.h[2]};
-  const double ch[3] = {a.cfl / 3 * a.h[0], a.cfl / 3 * a.h[1], a.cfl / 3 * a.h[2]};
   auto load = [&](int z, double* q) __device__ {
     const size_t o = (size_t)wrap_index(z, a.Z, a.periodic) * plane + col;
 #pragma unroll
     for (int v = 0; v < 5; ++v) q[v] = a.u[v * vol + o];
   };
 
-  double q[5], qn[5], qnn[5], Fz[5], fzlo[5];
-  double lz = 0.0;
+  // every row walks with the next plane's state already loaded and the one
+  // after in flight (the rows without output only supply their state)
+  double q[5], qn[5], qnn[5], fzlo[5];
   CellBase b{};
+  load(z0, q);
+  load(z0 + 1, qn);
   if (out_row) {
-    double qm[5], Fm[5];
+    double qm[5], Fm[5], Fz[5];
     load(z0 - 1, qm);
-    load(z0, q);
-    load(z0 + 1, qn);
     const CellBase bm = cell_base(qm);
     b = cell_base(q);
     cell_flux(qm, bm, 2, Fm);
     cell_flux(q, b, 2, Fz);
-    lz = cell_lambda(q, b, 2);
-    rusanov_from_parts(qm, Fm, cell_lambda(qm, bm, 2), q, Fz, lz, fzlo);
+    rusanov_from_parts(qm, Fm, cell_lambda(qm, bm, 2), q, Fz, cell_lambda(q, b, 2), fzlo);
   }
-  double dtmin = DBL_MAX;
+  double lmx = 0.0, lmy = 0.0, lmz = 0.0;
   for (int z = z0; z < z1; ++z) {
-    if (out_row) {
-      load(z + 2, qnn);  // in flight during this plane
-    } else {
-      load(z, q);
-      b = cell_base(q);
-    }
-    // y data of this row
-    double Fy[5];
-    cell_flux(q, b, 1, Fy);
-    const double ly = cell_lambda(q, b, 1);
+    // phase A: this row's state, y flux and y eigenvalue into LDS
+    load(z + 2, qnn);  // in flight during this plane
+    if (!out_row) b = cell_base(q);
+    {
+      double Fy[5];
+      cell_flux(q, b, 1, Fy);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      sQ[w][v][lane] = q[v];
-      sQ[w][5 + v][lane] = Fy[v];
+      for (int v = 0; v < 5; ++v) {
+        sQ[w][v][lane] = q[v];
+        sQ[w][5 + v][lane] = Fy[v];
+      }
+      sQ[w][10][lane] = cell_lambda(q, b, 1);
     }
-    sQ[w][10][lane] = ly;
+    __syncthreads();
+    // phase B: the y face above the row into LDS; the x faces
+    if (face_above) {
+      double Fy[5], qa[5], Fa[5], fyhi[5];
+      cell_flux(q, b, 1, Fy);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        qa[v] = sQ[w + 1][v][lane];
+        Fa[v] = sQ[w + 1][5 + v][lane];
+      }
+      rusanov_from_parts(q, Fy, cell_lambda(q, b, 1), qa, Fa, sQ[w + 1][10][lane], fyhi);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sFy[w][v][lane] = fyhi[v];
+    }
     double un[5];
     if (out_row) {
-      // x faces
       double Fx[5], qr[5], Fr[5], fxhi[5];
       cell_flux(q, b, 0, Fx);
       const double lx = cell_lambda(q, b, 0);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        qr[v] = __shfl_down(q[v], 1, kWave);
-        Fr[v] = __shfl_down(Fx[v], 1, kWave);
+        qr[v] = from_next_lane(q[v]);
+        Fr[v] = from_next_lane(Fx[v]);
       }
-      const double lr = __shfl_down(lx, 1, kWave);
-      rusanov_from_parts(q, Fx, lx, qr, Fr, lr, fxhi);
+      rusanov_from_parts(q, Fx, lx, qr, Fr, from_next_lane(lx), fxhi);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double fxlo = __shfl_up(fxhi[v], 1, kWave);
+        const double fxlo = from_prev_lane(fxhi[v]);
         un[v] = q[v];
         un[v] += cdt[0] * fxlo;
         un[v] -= cdt[0] * fxhi[v];
       }
     }
     __syncthreads();
-    double fyhi[5];
-    if (face_above) {
-      double qa[5], Fa[5];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        qa[v] = sQ[w + 1][v][lane];
-        Fa[v] = sQ[w + 1][5 + v][lane];
-      }
-      rusanov_from_parts(q, Fy, ly, qa, Fa, sQ[w + 1][10][lane], fyhi);
-#pragma unroll
-      for (int v = 0; v < 5; ++v) sFy[w][v][lane] = fyhi[v];
-    }
-    __syncthreads();
+    // phase C: y and z terms, store, eigenvalue maxima of the new state
     if (out_row) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         un[v] += cdt[1] * sFy[w - 1][v][lane];
-        un[v] -= cdt[1] * fyhi[v];
+        un[v] -= cdt[1] * sFy[w][v][lane];
       }
-      // z faces
       const CellBase bn = cell_base(qn);
-      double Fzn[5], fzhi[5];
+      double Fz[5], Fzn[5], fzhi[5];
+      cell_flux(q, b, 2, Fz);
       cell_flux(qn, bn, 2, Fzn);
-      const double lzn = cell_lambda(qn, bn, 2);
-      rusanov_from_parts(q, Fz, lz, qn, Fzn, lzn, fzhi);
+      rusanov_from_parts(q, Fz, cell_lambda(q, b, 2), qn, Fzn, cell_lambda(qn, bn, 2), fzhi);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         un[v] += cdt[2] * fzlo[v];
@@ -306,60 +321,77 @@ __global__ __launch_bounds__(kWave * kFvRows) void fv_godunov_kernel(FvGridArgs 
         const size_t o = (size_t)z * plane + col;
 #pragma unroll
         for (int v = 0; v < 5; ++v) a.un[v * vol + o] = un[v];
-        dtmin = dmin(dtmin, cell_stable_dt(un, ch));
+        const CellBase bu = cell_base(un);
+        lmx = dmax(lmx, cell_lambda(un, bu, 0));
+        lmy = dmax(lmy, cell_lambda(un, bu, 1));
+        lmz = dmax(lmz, cell_lambda(un, bu, 2));
       }
       // walk one plane up
 #pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        q[v] = qn[v];
-        qn[v] = qnn[v];
-        Fz[v] = Fzn[v];
-        fzlo[v] = fzhi[v];
-      }
+      for (int v = 0; v < 5; ++v) fzlo[v] = fzhi[v];
       b = bn;
-      lz = lzn;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      q[v] = qn[v];
+      qn[v] = qnn[v];
     }
   }
-  const double m = wave_min(dtmin);
-  if (lane == 0 && m < DBL_MAX) atomic_min_positive(&a.state->dt_next, m);
+  reduce_lambda(a.state, 0, lmx);
+  reduce_lambda(a.state, 1, lmy);
+  reduce_lambda(a.state, 2, lmz);
+}
+
+// dt = min over d of ch[d] / lam_max[d] (the minimum over subcells and
+// directions of FVC:409-460's candidates, see FvState); 0 / non-finite -> bad.
+__device__ inline double dt_from_lambda(FvState* st) {
+  double dt = DBL_MAX;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    // read and clear at L2 (not through the scalar cache)
+    const double lm = __longlong_as_double((long long)atomic_take(&st->lam_max[d], 0ull));
+    dt = dmin(dt, st->ch[d] / lm);
+  }
+  return dt;
 }
 
 __global__ void fv_finalize_kernel(FvState* st) {
   if (threadIdx.x != 0) return;
-  const double nd = __longlong_as_double((long long)st->dt_next);
+  const double nd = dt_from_lambda(st);
   st->t += st->dt;
   st->dt = nd;
-  st->dt_next = (unsigned long long)__double_as_longlong(DBL_MAX);
   st->steps += 1;
   if (!(nd > 0.0) || !(nd < DBL_MAX)) st->bad = 1;
 }
 
-__global__ __launch_bounds__(256) void fv_stable_dt_kernel(FvGridArgs a) {
+__global__ __launch_bounds__(256) void fv_lambda_kernel(FvGridArgs a) {
   const size_t vol = (size_t)a.X * a.Y * a.Z;
-  const double ch[3] = {a.cfl / 3 * a.h[0], a.cfl / 3 * a.h[1], a.cfl / 3 * a.h[2]};
-  double dtmin = DBL_MAX;
+  double lmx = 0.0, lmy = 0.0, lmz = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < vol; i += (size_t)gridDim.x * blockDim.x) {
     double q[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) q[v] = a.u[v * vol + i];
-    dtmin = dmin(dtmin, cell_stable_dt(q, ch));
+    const CellBase b = cell_base(q);
+    lmx = dmax(lmx, cell_lambda(q, b, 0));
+    lmy = dmax(lmy, cell_lambda(q, b, 1));
+    lmz = dmax(lmz, cell_lambda(q, b, 2));
   }
-  const double m = wave_min(dtmin);
-  if ((threadIdx.x & (kWave - 1)) == 0 && m < DBL_MAX) atomic_min_positive(&a.state->dt_next, m);
+  reduce_lambda(a.state, 0, lmx);
+  reduce_lambda(a.state, 1, lmy);
+  reduce_lambda(a.state, 2, lmz);
 }
 
 __global__ void fv_begin_kernel(FvState* st) {
   if (threadIdx.x != 0) return;
-  const double nd = __longlong_as_double((long long)st->dt_next);
+  const double nd = dt_from_lambda(st);
   st->t = 0.0;
   st->dt = nd;
-  st->dt_next = (unsigned long long)__double_as_longlong(DBL_MAX);
   st->steps = 0;
   st->bad = (!(nd > 0.0) || !(nd < DBL_MAX)) ? 1 : 0;
 }
 
-__global__ void fv_reset_next_kernel(FvState* st) {
-  if (threadIdx.x == 0) st->dt_next = (unsigned long long)__double_as_longlong(DBL_MAX);
+__global__ void fv_reset_lambda_kernel(FvState* st) {
+  if (threadIdx.x < 3) st->lam_max[threadIdx.x] = 0ull;
 }
 
 // planes[v][Z][Y][X] <-> cells[cell][z][y][x][v]
@@ -431,8 +463,8 @@ hipError_t launch_fv_finalize(FvState* st, hipStream_t s) {
 }
 
 hipError_t launch_fv_initial_dt(const FvGridArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(fv_reset_next_kernel, dim3(1), dim3(64), 0, s, a.state);
-  hipLaunchKernelGGL(fv_stable_dt_kernel, dim3(stride_grid((size_t)a.X * a.Y * a.Z)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(fv_reset_lambda_kernel, dim3(1), dim3(64), 0, s, a.state);
+  hipLaunchKernelGGL(fv_lambda_kernel, dim3(stride_grid((size_t)a.X * a.Y * a.Z)), dim3(256), 0, s, a);
   hipLaunchKernelGGL(fv_begin_kernel, dim3(1), dim3(64), 0, s, a.state);
   return hipGetLastError();
 }

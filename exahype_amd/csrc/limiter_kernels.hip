@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
     }
     if (a.fv_dt_max) {
       const double m = wave_min(dt_allowed);
-      if ((threadIdx.x & (kWave - 1)) == 0) atomicMin(a.fv_dt_max, (unsigned long long)__double_as_longlong(m));
+      if ((threadIdx.x & (kWave - 1)) == 0) atomic_min_positive(a.fv_dt_max, m);
     }
     __syncthreads();
     // flux differences in the reference's accumulation order (MH:267-329)
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
     dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
     dtn = wave_min(dtn);
     if (lane == 0) {
-      atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
+      atomic_min_positive(&a.st_state->dt_slots[c % kDtSlots], dtn);
       a.dt_cell[c] = dtn;
     }
   }
@@ -887,7 +887,7 @@ __global__ __launch_bounds__(kLimNT) void lim_recompute_kernel(LimArgs a) {
       double dtn = active ? node_stable_dt<Euler<0>>(q, invDx, a.cfl_pnpm) : DBL_MAX;
       dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
       dtn = block_min<kLimNT>(dtn, red);
-      if (tid == 0) atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
+      if (tid == 0) atomic_min_positive(&a.st_state->dt_slots[c % kDtSlots], dtn);
     }
     __syncthreads();
     double mn[V], mx[V];
@@ -918,7 +918,7 @@ __global__ __launch_bounds__(256) void lim_recompute_light_kernel(LimArgs a) {
   if (a.st[c] != 0) return;
   a.mn[i] = a.loc_mm[(size_t)c * 2 * V + v];
   a.mx[i] = a.loc_mm[(size_t)c * 2 * V + V + v];
-  if (r == 0) atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(a.dt_cell[c]));
+  if (r == 0) atomic_min_positive(&a.st_state->dt_slots[c % kDtSlots], a.dt_cell[c]);
 }
 
 // Initial state (LS:1031-1043 and the limiter patches' own initial condition,

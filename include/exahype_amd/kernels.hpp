@@ -1,8 +1,9 @@
 // exahype_amd/kernels.hpp -- the kernels::aderdg::generic::c templates of the
 // reference (ExaHyPE/kernels/aderdg/generic/Kernels.h:85-213), the limiter
 // kernels (ExaHyPE/kernels/limiter/generic/Limiter.h:48-141) and the
-// MUSCL-Hancock update (ExaHyPE/kernels/finitevolumes/musclhancock/c/
-// musclhancock.h:38-41) re-declared on top of the C-ABI, so that generated
+// MUSCL-Hancock and Godunov updates (ExaHyPE/kernels/finitevolumes/
+// musclhancock/c/musclhancock.h:38-41, godunov/c/godunov.h) re-declared on
+// top of the C-ABI, so that generated
 // solver glue (Toolkit/src/eu/exahype/solvers/templates/
 // AbstractGenericADERDGSolverImplementation.template:46-161,
 // AbstractGenericFiniteVolumesSolverInCImplementation.template:48-49) and the
@@ -226,5 +227,34 @@ double solutionUpdate(SolverType& /*solver*/, double* luh_new, const double* luh
 
 }  // namespace c
 }  // namespace musclhancock
+
+namespace godunov {
+namespace c {
+
+// godunov::c::solutionUpdate (ExaHyPE/kernels/finitevolumes/godunov/c/godunov.h,
+// c/3d/godunov.cpph:35): the first-order update of one patch with filled ghost
+// layers; luh_new holds a copy of luh on entry.  Returns the admissible time
+// step size (min over faces of CFL/3 h / s_max).
+template <bool useSource, bool useNCP, bool useFlux, class SolverType, typename DxVector>
+double solutionUpdate(SolverType& /*solver*/, double* luh_new, const double* luh, const DxVector& dx, double dt) {
+  exa_solver s;
+  s.pde = SolverType::ExaDevicePde;
+  s.order = 0;
+  s.num_variables = SolverType::NumberOfVariables;
+  s.num_parameters = SolverType::NumberOfParameters;
+  s.terms = exahype_amd::terms(useSource, useFlux, useNCP);
+  s.no_time_averaging = 0;
+  s.cfl = SolverType::CFL;
+  s.precision = EXA_PRECISION_FP64;
+  double d[3], dt_max = 0.0;
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(exa_godunov_solution_update(&s, luh_new, luh, d, dt, SolverType::PatchSize,
+                                                 SolverType::GhostLayerWidth, &dt_max),
+                     "godunov::solutionUpdate");
+  return dt_max;
+}
+
+}  // namespace c
+}  // namespace godunov
 }  // namespace finitevolumes
 }  // namespace kernels

@@ -119,11 +119,26 @@ double limiter_glue(const MyADERDGSolver* solver, MyFVSolver& fv, double* luh, d
       fv, limNew, lim, dx, 1e-3);
   return ok ? maxAdmissibleDt : 0.0;
 }
+// The EulerFV demonstrator's Godunov solver (patch-size 10, ghost width 1).
+struct MyGodunovSolver {
+  static constexpr int NumberOfVariables = 5;
+  static constexpr int NumberOfParameters = 0;
+  static constexpr int PatchSize = 10;
+  static constexpr int GhostLayerWidth = 1;
+  static constexpr double CFL = 0.9;
+  static constexpr int ExaDevicePde = EXA_PDE_EULER;
+};
+double godunov_glue(MyGodunovSolver& fv, double* luhNew, const double* luh) {
+  const Vec3 dx{{0.1, 0.1, 0.1}};
+  return kernels::finitevolumes::godunov::c::solutionUpdate<false, false, true, MyGodunovSolver>(fv, luhNew, luh,
+                                                                                               dx, 1e-3);
+}
 int main(int argc, char**) {
   if (argc > 99) {
     glue(*(MyEulerSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
     MyFVSolver fv;
-    return (int)limiter_glue(nullptr, fv, 0, 0, 0, 0, 0);
+    MyGodunovSolver gv;
+    return (int)(limiter_glue(nullptr, fv, 0, 0, 0, 0, 0) + godunov_glue(gv, 0, 0));
   }
   return 0;
 }

@@ -63,6 +63,18 @@ void ora_riemann_solver_nonlinear(const ora_solver* s, double* FL, double* FR, c
 void ora_solution_update(const ora_solver* s, double* luh, const double* lduh, double dt);
 double ora_stable_time_step_size(const ora_solver* s, const double* luh, const double* dx);
 
+/* ---- linear kernels (linear.c) ---- */
+void ora_coefficient_matrix(const ora_solver* s, const double* Q, int d, double* Bn);
+void ora_ncp_directional(const ora_solver* s, const double* Q, const double* gradQ, double* BgradQ);
+void ora_volume_integral_linear(const ora_solver* s, double* lduh, const double* lFhi);
+void ora_surface_integral_linear(const ora_solver* s, double* lduh, const double* lFhbnd, const double* dx);
+void ora_riemann_solver_linear(const ora_solver* s, double* FL, double* FR, const double* QL, const double* QR,
+                               double dt, int d);
+/* lQi (B+1)B^3 D, lFi B^4 4V, gradQ B^4 3V, lQhi B^3 D, lFhi 4 B^3 V; PSi may be NULL */
+void ora_space_time_predictor_linear(const ora_solver* s, double* lQbnd, double* lFbnd, double* lQi, double* lFi,
+                                     double* gradQ, double* lQhi, double* lFhi, const double* luh, const double* dx,
+                                     double dt, const double* PSi);
+
 /* table access (the same reference tables the kernels use) */
 double ora_table(int which, int order, int i, int j);
 enum { ORA_T_WEIGHTS = 0, ORA_T_NODES, ORA_T_FLCOEFF, ORA_T_FRCOEFF, ORA_T_KXI, ORA_T_IK1,

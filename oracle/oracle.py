@@ -92,6 +92,12 @@ def lib():
         L.ora_fv_boundary_transmissive.argtypes = [I, I, I, P, I]
         L.ora_fv_stable_dt.argtypes = [S, I, I, P, P, D_]
         L.ora_fv_stable_dt.restype = D_
+        L.ora_coefficient_matrix.argtypes = [S, P, I, P]
+        L.ora_ncp_directional.argtypes = [S, P, P, P]
+        L.ora_volume_integral_linear.argtypes = [S, P, P]
+        L.ora_surface_integral_linear.argtypes = [S, P, P, P]
+        L.ora_riemann_solver_linear.argtypes = [S, P, P, P, P, D_, I]
+        L.ora_space_time_predictor_linear.argtypes = [S, P, P, P, P, P, P, P, P, P, D_, P]
         L.ora_godunov_update.argtypes = [S, I, I, P, P, P, D_, D_]
         L.ora_godunov_update.restype = D_
         VP = ctypes.c_void_p
@@ -340,6 +346,55 @@ def muscl_update(s, L, g, patch, dx, dt):
     new = old.copy()
     dtm = lib().ora_muscl_update(ctypes.byref(s), L, g, _p(new), _p(old), _p(dx3(dx)), dt)
     return new, dtm
+
+
+def coefficient_matrix(s, Q, d):
+    Bn = np.zeros(s.nvar * s.nvar)
+    lib().ora_coefficient_matrix(ctypes.byref(s), _p(np.ascontiguousarray(Q, dtype=np.float64)), d, _p(Bn))
+    return Bn.reshape(s.nvar, s.nvar)
+
+
+def ncp_directional(s, Q, gradQ):
+    out = np.zeros(3 * s.nvar)
+    lib().ora_ncp_directional(ctypes.byref(s), _p(np.ascontiguousarray(Q, dtype=np.float64)),
+                              _p(np.ascontiguousarray(gradQ, dtype=np.float64).ravel()), _p(out))
+    return out.reshape(3, s.nvar)
+
+
+def volume_integral_linear(s, lFhi):
+    B = s.order + 1
+    lduh = np.zeros(B ** 3 * s.nvar)
+    lib().ora_volume_integral_linear(ctypes.byref(s), _p(lduh), _p(np.ascontiguousarray(lFhi, dtype=np.float64).ravel()))
+    return lduh
+
+
+def surface_integral_linear(s, lduh, lFhbnd, dx):
+    out = np.array(lduh, dtype=np.float64).ravel()
+    lib().ora_surface_integral_linear(ctypes.byref(s), _p(out), _p(np.ascontiguousarray(lFhbnd, dtype=np.float64).ravel()),
+                                      _p(dx3(dx)))
+    return out
+
+
+def riemann_linear(s, QL, QR, dt, direction):
+    B = s.order + 1
+    FL, FR = np.zeros(B * B * s.nvar), np.zeros(B * B * s.nvar)
+    lib().ora_riemann_solver_linear(ctypes.byref(s), _p(FL), _p(FR), _p(np.ascontiguousarray(QL, dtype=np.float64).ravel()),
+                                    _p(np.ascontiguousarray(QR, dtype=np.float64).ravel()), dt, direction)
+    return FL, FR
+
+
+def space_time_predictor_linear(s, luh, dx, dt, PSi=None):
+    """Returns dict lQhbnd, lFhbnd, lQi, lFi, gradQ, lQhi, lFhi."""
+    B, V, D = s.order + 1, s.nvar, s.nvar + s.npar
+    out = dict(lQhbnd=np.zeros(6 * B * B * D), lFhbnd=np.zeros(6 * B * B * V), lQi=np.zeros((B + 1) * B ** 3 * D),
+               lFi=np.zeros(B ** 4 * 4 * V), gradQ=np.zeros(B ** 4 * 3 * V), lQhi=np.zeros(B ** 3 * D),
+               lFhi=np.zeros(4 * B ** 3 * V))
+    ps = None if PSi is None else np.ascontiguousarray(PSi, dtype=np.float64).ravel()
+    lib().ora_space_time_predictor_linear(ctypes.byref(s), _p(out["lQhbnd"]), _p(out["lFhbnd"]), _p(out["lQi"]),
+                                          _p(out["lFi"]), _p(out["gradQ"]), _p(out["lQhi"]), _p(out["lFhi"]),
+                                          _p(np.ascontiguousarray(luh, dtype=np.float64).ravel()), _p(dx3(dx)), dt,
+                                          _p(ps))
+    return out
 
 
 FV_GODUNOV, FV_MUSCLHANCOCK = 0, 1

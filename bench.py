@@ -354,13 +354,15 @@ def bench_batch(capi, args):
 
 def bench_elastic(capi, args):
     """Elastic workload: velocity-stress NCP form, N=4, 81^3, Gaussian pulse
-    of width 0.05 in a seeded two-layer material; FP32-bulk vs FP64."""
+    of width 0.05 in a seeded two-layer material; FP32-bulk vs FP64, and the
+    same problem through the linear kernels (Cauchy-Kowalevski predictor,
+    linear Riemann solver) in FP64."""
     order, n = 4, args.cells
     B = order + 1
     res = {}
-    for form in ("fp64", "fp32"):
-        prec = capi.PRECISION_FP64 if form == "fp64" else capi.PRECISION_FP32_BULK
-        g = capi.Grid(capi.elastic_solver(order, precision=prec), n)
+    for form in ("fp64", "fp32", "linear"):
+        prec = capi.PRECISION_FP32_BULK if form == "fp32" else capi.PRECISION_FP64
+        g = capi.Grid(capi.elastic_solver(order, precision=prec, linear=form == "linear"), n)
         g.init_elastic(seed=args.seed, width=0.05)
         g.step(args.warmup)
         g.synchronize()
@@ -394,6 +396,15 @@ def bench_elastic(capi, args):
                  "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in m64["stages"].items()}},
         "fp32_vs_fp64": {"speedup": m64["wall"] / m["wall"], "l2_rel_err": l2,
                          "steps_compared": args.warmup + args.steps},
+        # the linear kernels on the same problem: a different scheme in its
+        # face states (face-averaged coefficient matrix), so the difference to
+        # the nonlinear FP64 run is a discretisation difference, not an error
+        "linear_fp64": {"value": dof * args.steps / res["linear"]["wall"],
+                        "ms_per_step": res["linear"]["wall"] / args.steps * 1e3,
+                        "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4)
+                                              for k, v in res["linear"]["stages"].items()},
+                        "l2_rel_diff_vs_nonlinear_fp64": float(np.linalg.norm(res["linear"]["sol"] - b)
+                                                               / np.linalg.norm(b))},
         "roofline": None,
         "cpu_baseline": None,
     }

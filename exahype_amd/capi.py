@@ -44,6 +44,8 @@ EXPORTED = (
     "exa_godunov_solution_update", "exa_fvgrid_create", "exa_fvgrid_destroy", "exa_fvgrid_num_cells",
     "exa_fvgrid_upload", "exa_fvgrid_download", "exa_fvgrid_init_euler", "exa_fvgrid_step",
     "exa_fvgrid_synchronize", "exa_fvgrid_time", "exa_fvgrid_set_profiling", "exa_fvgrid_stage_times",
+    "exa_space_time_predictor_linear", "exa_volume_integral_linear", "exa_surface_integral_linear",
+    "exa_riemann_solver_linear",
 )
 
 
@@ -55,7 +57,8 @@ class Solver(ctypes.Structure):
     """exa_solver: the reference SolverType's constants."""
     _fields_ = [("pde", ctypes.c_int), ("order", ctypes.c_int), ("num_variables", ctypes.c_int),
                 ("num_parameters", ctypes.c_int), ("terms", ctypes.c_int),
-                ("no_time_averaging", ctypes.c_int), ("cfl", ctypes.c_double), ("precision", ctypes.c_int)]
+                ("no_time_averaging", ctypes.c_int), ("cfl", ctypes.c_double), ("precision", ctypes.c_int),
+                ("linear", ctypes.c_int)]
 
     @property
     def B(self):
@@ -77,13 +80,15 @@ class PatchSpec(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("order", ctypes.c_int), ("k", ctypes.c_double * 3), ("amp", ctypes.c_double)]
 
 
-def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9, precision=PRECISION_FP64):
-    return Solver(PDE_EULER, order, 5, npar, terms, 0, cfl, precision)
+def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9, precision=PRECISION_FP64, linear=False):
+    return Solver(PDE_EULER, order, 5, npar, terms, 0, cfl, precision, int(linear))
 
 
-def elastic_solver(order, cfl=0.9, precision=PRECISION_FP64):
-    """Linear elasticity: 9 variables + (lambda, mu, rho), non-conservative product only."""
-    return Solver(PDE_ELASTIC, order, 9, 3, TERM_NCP, 0, cfl, precision)
+def elastic_solver(order, cfl=0.9, precision=PRECISION_FP64, linear=False):
+    """Linear elasticity: 9 variables + (lambda, mu, rho), non-conservative product only.
+    linear=True selects the linear kernels (Cauchy-Kowalevski predictor, linear
+    Riemann solver and surface integral) in grids."""
+    return Solver(PDE_ELASTIC, order, 9, 3, TERM_NCP, 0, cfl, precision, int(linear))
 
 
 _lib = None
@@ -110,6 +115,10 @@ def lib():
     L.exa_surface_integral_nonlinear.argtypes = [S, _D, _D, _D]
     L.exa_riemann_solver_nonlinear.argtypes = [S, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
     L.exa_solution_update.argtypes = [S, _D, _D, ctypes.c_double]
+    L.exa_space_time_predictor_linear.argtypes = [S, _D, _D, _D, _D, _D, _D, _D, _D, _D, ctypes.c_double, _D]
+    L.exa_volume_integral_linear.argtypes = [S, _D, _D, _D]
+    L.exa_surface_integral_linear.argtypes = [S, _D, _D, _D]
+    L.exa_riemann_solver_linear.argtypes = [S, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
     L.exa_stable_time_step_size.argtypes = [S, _D, _D, _D]
     L.exa_batch_predictor_volume.argtypes = [S, ctypes.c_long, _D, _D, _D, _D, _D, ctypes.c_double, _I]
     L.exa_grid_create.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int,
@@ -254,6 +263,45 @@ def stable_time_step_size(s, luh, dx):
     check(lib().exa_stable_time_step_size(ctypes.byref(s), _dp(_f64(luh)), _dp(_dx3(dx)), _dp(out)),
           "stableTimeStepSize")
     return float(out[0])
+
+
+def space_time_predictor_linear(s, luh, dx, dt, temps=False, PSi=None):
+    """spaceTimePredictorLinear: returns dict(lQhbnd, lFhbnd[, lQi, lFi, gradQ, lQhi, lFhi]).
+    PSi: point sources (B+1) B^3 V, or None."""
+    B, D, V = s.B, s.D, s.num_variables
+    out = dict(lQhbnd=np.zeros(6 * B * B * D), lFhbnd=np.zeros(6 * B * B * V))
+    if temps:
+        out.update(lQi=np.zeros((B + 1) * B ** 3 * D), lFi=np.zeros(4 * B ** 4 * V), gradQ=np.zeros(3 * B ** 4 * V),
+                   lQhi=np.zeros(B ** 3 * D), lFhi=np.zeros(4 * B ** 3 * V))
+    check(lib().exa_space_time_predictor_linear(
+        ctypes.byref(s), _dp(out["lQhbnd"]), _dp(out["lFhbnd"]), _dp(out.get("lQi")), _dp(out.get("lFi")),
+        _dp(out.get("gradQ")), _dp(out.get("lQhi")), _dp(out.get("lFhi")), _dp(_f64(luh)), _dp(_dx3(dx)), float(dt),
+        _dp(None if PSi is None else _f64(PSi))), "spaceTimePredictorLinear")
+    return out
+
+
+def volume_integral_linear(s, lFhi, dx=(1.0, 1.0, 1.0)):
+    """volumeIntegralLinear of lFhi (4 B^3 V; the source block is not read)."""
+    lduh = np.zeros(s.B ** 3 * s.num_variables)
+    check(lib().exa_volume_integral_linear(ctypes.byref(s), _dp(lduh), _dp(_f64(lFhi)), _dp(_dx3(dx))),
+          "volumeIntegralLinear")
+    return lduh
+
+
+def surface_integral_linear(s, lduh, lFhbnd, dx):
+    lduh = np.array(lduh, dtype=np.float64)
+    check(lib().exa_surface_integral_linear(ctypes.byref(s), _dp(lduh), _dp(_f64(lFhbnd)), _dp(_dx3(dx))),
+          "surfaceIntegralLinear")
+    return lduh
+
+
+def riemann_solver_linear(s, QL, QR, dt, direction):
+    """riemannSolverLinear: returns the fluctuations (FL, FR)."""
+    n = s.B * s.B * s.num_variables
+    FL, FR = np.zeros(n), np.zeros(n)
+    check(lib().exa_riemann_solver_linear(ctypes.byref(s), _dp(FL), _dp(FR), _dp(_f64(QL)), _dp(_f64(QR)),
+                                          float(dt), int(direction)), "riemannSolverLinear")
+    return FL, FR
 
 
 def batch_predictor_volume(s, luh, dx, dt):

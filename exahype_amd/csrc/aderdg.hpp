@@ -21,6 +21,7 @@ struct SolverDesc {
   int no_time_averaging;
   double cfl;
   int precision;  // kFP64 or kFP32Bulk
+  int linear;     // linear ADER-DG: the *Linear kernels (Cauchy-Kowalevski predictor)
 };
 
 // Precision forms.  kFP64: everything in double (the reference's arithmetic).
@@ -168,6 +169,8 @@ bool predictor_has_lds_variant(const SolverDesc& sd);
 constexpr long kPredictorWsCells = 4096;
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s);
 // update_kernel modes: surface integral, solution update, write lduh back.
+// With sd.linear the surface integral is surfaceIntegralLinear (fluctuations,
+// F_{2d+1} FRCoeff + F_{2d} FLCoeff, divided by dx).
 enum : int { kUpdSurface = 1, kUpdUpdate = 2, kUpdWriteLduh = 4 };
 hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, long ncells,
                          hipStream_t s);
@@ -180,6 +183,37 @@ hipError_t launch_volume_integral(const SolverDesc& sd, const VolumeArgs& a, lon
 // One cell: lFhi = the time average of the space-time fluxes lFi.
 hipError_t launch_time_average_lfi(int order, int V, const double* lFi, double* lFhi, hipStream_t s);
 hipError_t launch_euler_ic(const ICArgs& a, hipStream_t s);
+
+// ---- linear ADER-DG kernels (linear_kernels.hip), FP64 ----
+struct LinearPredictorArgs {
+  const double* luh;  // [cells][B^3][V+P]
+  double* lduh;       // [cells][B^3][V] volumeIntegralLinear of the prediction (may be null)
+  double* lQhbnd;     // [cells][6][B^2][V+P]
+  double* lFhbnd;     // [cells][6][B^2][V]
+  const double* dx;   // dx[cell * dx_stride + d]
+  long dx_stride;
+  const double* dt;   // device scalar
+  const double* PSi;  // point sources (t,z,y,x,var) (B+1) B^3 V of one cell, or null
+  long cell0;
+  // optional exports of the reference's temporaries (single-cell API only)
+  double* lQi;    // (t,z,y,x,var) (B+1) B^3 (V+P)
+  double* lFi;    // (t,z,y,x,dir,var) B^4 3V + a zeroed source block of B^4 V
+  double* gradQ;  // (t,z,y,x,dir,var) B^4 3V
+  double* lQhi;   // (z,y,x,var) B^3 (V+P)
+  double* lFhi;   // (dir,z,y,x,var) 3 B^3 V + a zeroed source block of B^3 V
+};
+bool linear_supported(const SolverDesc& sd);
+bool linear_sizes_supported(const SolverDesc& sd);  // (order, nvar) of the linear integrals
+hipError_t launch_linear_predictor(const SolverDesc& sd, bool export_temps, const LinearPredictorArgs& a,
+                                   long ncells, hipStream_t s);
+// riemannSolverLinear over a face list (fluctuations into both cells' blocks)
+hipError_t launch_linear_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s);
+// volumeIntegralLinear of one or more cells: lduh = -w^3 sum_d lFhi_d
+hipError_t launch_linear_volume(const SolverDesc& sd, double* lduh, const double* lFhi, long ncells,
+                                hipStream_t s);
+// surfaceIntegralLinear of one or more cells (lduh in/out, one dx for all)
+hipError_t launch_linear_surface(const SolverDesc& sd, double* lduh, const double* lFhbnd, const double* dx,
+                                 long ncells, hipStream_t s);
 // st->picard_total += sum(iters[0..ncells)), st->predictor_cells += ncells
 hipError_t launch_count_iterations(const int* iters, long ncells, StepState* st, hipStream_t s);
 

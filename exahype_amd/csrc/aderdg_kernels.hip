@@ -770,7 +770,7 @@ __global__ __launch_bounds__(256) void riemann_kernel(RiemannArgs a) {
 // Surface integral + solution update + the cell's stable time step, in double
 // on the double solution; TF is the storage type of lduh and the face fluxes.
 // ---------------------------------------------------------------------------
-template <int B, class PDE, bool SURFACE, bool UPDATE, bool WRITE_LDUH, typename TF>
+template <int B, class PDE, bool SURFACE, bool UPDATE, bool WRITE_LDUH, typename TF, bool LINEAR = false>
 __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
@@ -799,24 +799,36 @@ __global__ __launch_bounds__(node_threads(B)) void update_kernel(UpdateArgs a) {
       const TF* lF = static_cast<const TF*>(a.lFhbnd) + cell * (size_t)(6 * B2 * V);
       {  // x faces, (z,y)
         const double us = wz * wy * invDx[0];
+        const double usl = (wz * wy) / dxp[0];  // surfaceIntegralLinear.cpph:21-90
         const double fr = exa_fr_coeff[N][x], fl = exa_fl_coeff[N][x];
 #pragma unroll
         for (int m = 0; m < V; ++m)
-          du[m] -= ((double)lF[(1 * B2 + z * B + y) * V + m] * fr - (double)lF[(0 * B2 + z * B + y) * V + m] * fl) * us;
+          if constexpr (LINEAR)
+            du[m] -= ((double)lF[(1 * B2 + z * B + y) * V + m] * fr + (double)lF[(0 * B2 + z * B + y) * V + m] * fl) * usl;
+          else
+            du[m] -= ((double)lF[(1 * B2 + z * B + y) * V + m] * fr - (double)lF[(0 * B2 + z * B + y) * V + m] * fl) * us;
       }
       {  // y faces, (z,x)
         const double us = wz * wx * invDx[1];
+        const double usl = (wz * wx) / dxp[1];  // surfaceIntegralLinear.cpph:21-90
         const double fr = exa_fr_coeff[N][y], fl = exa_fl_coeff[N][y];
 #pragma unroll
         for (int m = 0; m < V; ++m)
-          du[m] -= ((double)lF[(3 * B2 + z * B + x) * V + m] * fr - (double)lF[(2 * B2 + z * B + x) * V + m] * fl) * us;
+          if constexpr (LINEAR)
+            du[m] -= ((double)lF[(3 * B2 + z * B + x) * V + m] * fr + (double)lF[(2 * B2 + z * B + x) * V + m] * fl) * usl;
+          else
+            du[m] -= ((double)lF[(3 * B2 + z * B + x) * V + m] * fr - (double)lF[(2 * B2 + z * B + x) * V + m] * fl) * us;
       }
       {  // z faces, (y,x)
         const double us = wy * wx * invDx[2];
+        const double usl = (wy * wx) / dxp[2];  // surfaceIntegralLinear.cpph:21-90
         const double fr = exa_fr_coeff[N][z], fl = exa_fl_coeff[N][z];
 #pragma unroll
         for (int m = 0; m < V; ++m)
-          du[m] -= ((double)lF[(5 * B2 + y * B + x) * V + m] * fr - (double)lF[(4 * B2 + y * B + x) * V + m] * fl) * us;
+          if constexpr (LINEAR)
+            du[m] -= ((double)lF[(5 * B2 + y * B + x) * V + m] * fr + (double)lF[(4 * B2 + y * B + x) * V + m] * fl) * usl;
+          else
+            du[m] -= ((double)lF[(5 * B2 + y * B + x) * V + m] * fr - (double)lF[(4 * B2 + y * B + x) * V + m] * fl) * us;
       }
     }
     if (UPDATE) {
@@ -1129,6 +1141,20 @@ hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, lo
   if (ncells <= 0) return hipSuccess;
   return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
     const dim3 grid((unsigned)ncells), block(node_threads(B));
+    if (sd.linear && (mode & kUpdSurface)) {  // the linear solvers' surface integral
+      if constexpr (std::is_same_v<PDE, Elastic> || std::is_same_v<PDE, Euler<1>>) {
+        if (sd.precision != kFP64) return hipErrorNotSupported;
+        if (mode == (kUpdSurface | kUpdUpdate))
+          hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false, double, true>), grid, block, 0, s, a);
+        else if (mode == (kUpdSurface | kUpdWriteLduh))
+          hipLaunchKernelGGL((update_kernel<B, PDE, true, false, true, double, true>), grid, block, 0, s, a);
+        else
+          return hipErrorInvalidValue;
+        return hipGetLastError();
+      } else {
+        return hipErrorNotSupported;
+      }
+    }
     if (sd.precision == kFP32Bulk) {
       if (mode == (kUpdSurface | kUpdUpdate))
         hipLaunchKernelGGL((update_kernel<B, PDE, true, true, false, float>), grid, block, 0, s, a);

@@ -38,7 +38,8 @@ extern "C" int exa_init(int device) {
 
 extern "C" int exa_solver_supported(const exa_solver* s) {
   SolverDesc sd;
-  return to_desc(s, &sd) && exa::supported(sd) ? 1 : 0;
+  if (!to_desc(s, &sd)) return 0;
+  return (sd.linear ? exa::linear_supported(sd) : exa::supported(sd)) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -49,6 +50,7 @@ static int predictor_impl(const exa_solver* s, long ncells, double* lQhbnd, doub
                           const double* luh, const double* dx, double dt, int* iters) {
   SolverDesc sd;
   if (!to_desc(s, &sd) || ncells < 1 || !lQhbnd || !lFhbnd || !luh || !dx) return EXA_E_INVALID;
+  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
   if (!exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   const bool exp = lQi || lFi || lQhi || lFhi;
@@ -123,6 +125,7 @@ extern "C" int exa_volume_integral_nonlinear(const exa_solver* s, double* lduh, 
                                              const double* dx) {
   SolverDesc sd;
   if (!to_desc(s, &sd) || !lduh || !lFhi || !dx) return EXA_E_INVALID;
+  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
   if (!exa::supported_sizes(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   DevBuf d_du, d_fhi, d_dx, d_fi;
@@ -149,6 +152,7 @@ static int update_impl(const exa_solver* s, int mode, double* luh, double* lduh,
                        const double* dx, double dt, double* dt_out) {
   SolverDesc sd;
   if (!to_desc(s, &sd)) return EXA_E_INVALID;
+  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
   if (sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const bool surface_only = mode == (exa::kUpdSurface | exa::kUpdWriteLduh);
   if (surface_only ? !exa::supported_sizes(sd) : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
@@ -215,6 +219,7 @@ extern "C" int exa_riemann_solver_nonlinear(const exa_solver* s, double* FL, dou
   (void)dt;  // unused by the nonlinear Rusanov solver, as in the reference
   SolverDesc sd;
   if (!to_desc(s, &sd) || !FL || !FR || !QL || !QR || direction < 0 || direction > 2) return EXA_E_INVALID;
+  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
   if (!exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   // Two cells: cell 0 owns face 2d+1 (left state), cell 1 owns face 2d (right state).
@@ -236,6 +241,132 @@ extern "C" int exa_riemann_solver_nonlinear(const exa_solver* s, double* FL, dou
   EXA_TRY(hipDeviceSynchronize());
   EXA_TRY(hipMemcpy(FL, d_fb.d() + ifL, sizeof(double) * ff, hipMemcpyDeviceToHost));
   EXA_TRY(hipMemcpy(FR, d_fb.d() + ifR, sizeof(double) * ff, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+// ---- linear kernels (the entry point names the kernel: the solver's linear flag is implied) ----
+
+extern "C" int exa_space_time_predictor_linear(const exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi,
+                                               double* lFi, double* gradQ, double* lQhi, double* lFhi,
+                                               const double* luh, const double* dx, double dt, const double* PSi) {
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || !lQhbnd || !lFhbnd || !luh || !dx) return EXA_E_INVALID;
+  sd.linear = 1;
+  const int ntemp = (lQi != nullptr) + (lFi != nullptr) + (gradQ != nullptr) + (lQhi != nullptr) + (lFhi != nullptr);
+  if (ntemp != 0 && ntemp != 5) return EXA_E_INVALID;  // the temporaries come all together or not at all
+  if (!exa::linear_supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  const bool exp = ntemp == 5;
+  const size_t nqi = (z.B + 1) * z.B3 * z.D, nfi = 4 * z.B4 * z.V, ngq = 3 * z.B4 * z.V, nqhi = z.B3 * z.D,
+               nfhi = 4 * z.B3 * z.V, nps = (z.B + 1) * z.B3 * z.V;
+  DevBuf d_luh, d_qb, d_fb, d_dx, d_dt, d_ps, d_qi, d_fi, d_gq, d_qhi, d_fhi;
+  EXA_TRY(d_luh.alloc(sizeof(double) * z.B3 * z.D));
+  EXA_TRY(d_qb.alloc(sizeof(double) * 6 * z.B2 * z.D));
+  EXA_TRY(d_fb.alloc(sizeof(double) * 6 * z.B2 * z.V));
+  EXA_TRY(d_dx.alloc(sizeof(double) * 3));
+  EXA_TRY(d_dt.alloc(sizeof(double)));
+  if (PSi) {
+    EXA_TRY(d_ps.alloc(sizeof(double) * nps));
+    EXA_TRY(hipMemcpy(d_ps.p, PSi, sizeof(double) * nps, hipMemcpyHostToDevice));
+  }
+  if (exp) {
+    EXA_TRY(d_qi.alloc(sizeof(double) * nqi));
+    EXA_TRY(d_fi.alloc(sizeof(double) * nfi));
+    EXA_TRY(d_gq.alloc(sizeof(double) * ngq));
+    EXA_TRY(d_qhi.alloc(sizeof(double) * nqhi));
+    EXA_TRY(d_fhi.alloc(sizeof(double) * nfhi));
+  }
+  EXA_TRY(hipMemcpy(d_luh.p, luh, sizeof(double) * z.B3 * z.D, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dx.p, dx, sizeof(double) * 3, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dt.p, &dt, sizeof(double), hipMemcpyHostToDevice));
+  exa::LinearPredictorArgs a{};
+  a.luh = d_luh.d();
+  a.lQhbnd = d_qb.d();
+  a.lFhbnd = d_fb.d();
+  a.dx = d_dx.d();
+  a.dt = d_dt.d();
+  a.PSi = PSi ? d_ps.d() : nullptr;
+  if (exp) {
+    a.lQi = d_qi.d();
+    a.lFi = d_fi.d();
+    a.gradQ = d_gq.d();
+    a.lQhi = d_qhi.d();
+    a.lFhi = d_fhi.d();
+  }
+  EXA_TRY(exa::launch_linear_predictor(sd, exp, a, 1, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(lQhbnd, d_qb.p, sizeof(double) * 6 * z.B2 * z.D, hipMemcpyDeviceToHost));
+  EXA_TRY(hipMemcpy(lFhbnd, d_fb.p, sizeof(double) * 6 * z.B2 * z.V, hipMemcpyDeviceToHost));
+  if (exp) {
+    EXA_TRY(hipMemcpy(lQi, d_qi.p, sizeof(double) * nqi, hipMemcpyDeviceToHost));
+    EXA_TRY(hipMemcpy(lFi, d_fi.p, sizeof(double) * nfi, hipMemcpyDeviceToHost));
+    EXA_TRY(hipMemcpy(gradQ, d_gq.p, sizeof(double) * ngq, hipMemcpyDeviceToHost));
+    EXA_TRY(hipMemcpy(lQhi, d_qhi.p, sizeof(double) * nqhi, hipMemcpyDeviceToHost));
+    EXA_TRY(hipMemcpy(lFhi, d_fhi.p, sizeof(double) * nfhi, hipMemcpyDeviceToHost));
+  }
+  return EXA_OK;
+}
+
+extern "C" int exa_volume_integral_linear(const exa_solver* s, double* lduh, const double* lFhi, const double* dx) {
+  (void)dx;  // unused by the linear volume integral, as in the reference
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || !lduh || !lFhi) return EXA_E_INVALID;
+  sd.linear = 1;
+  if (!exa::linear_sizes_supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  DevBuf d_du, d_fhi;
+  EXA_TRY(d_du.alloc(sizeof(double) * z.B3 * z.V));
+  EXA_TRY(d_fhi.alloc(sizeof(double) * 4 * z.B3 * z.V));
+  EXA_TRY(hipMemcpy(d_fhi.p, lFhi, sizeof(double) * 4 * z.B3 * z.V, hipMemcpyHostToDevice));
+  EXA_TRY(exa::launch_linear_volume(sd, d_du.d(), d_fhi.d(), 1, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(lduh, d_du.p, sizeof(double) * z.B3 * z.V, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+extern "C" int exa_surface_integral_linear(const exa_solver* s, double* lduh, const double* lFhbnd,
+                                           const double* dx) {
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || !lduh || !lFhbnd || !dx) return EXA_E_INVALID;
+  sd.linear = 1;
+  if (!exa::linear_sizes_supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  DevBuf d_du, d_fb, d_dx;
+  EXA_TRY(d_du.alloc(sizeof(double) * z.B3 * z.V));
+  EXA_TRY(d_fb.alloc(sizeof(double) * 6 * z.B2 * z.V));
+  EXA_TRY(d_dx.alloc(sizeof(double) * 3));
+  EXA_TRY(hipMemcpy(d_du.p, lduh, sizeof(double) * z.B3 * z.V, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_fb.p, lFhbnd, sizeof(double) * 6 * z.B2 * z.V, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_dx.p, dx, sizeof(double) * 3, hipMemcpyHostToDevice));
+  EXA_TRY(exa::launch_linear_surface(sd, d_du.d(), d_fb.d(), d_dx.d(), 1, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(lduh, d_du.p, sizeof(double) * z.B3 * z.V, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+extern "C" int exa_riemann_solver_linear(const exa_solver* s, double* FL, double* FR, const double* QL,
+                                         const double* QR, double dt, int direction) {
+  (void)dt;  // unused by the linear Riemann solver, as in the reference
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || !FL || !FR || !QL || !QR || direction < 0 || direction > 2) return EXA_E_INVALID;
+  sd.linear = 1;
+  if (!exa::linear_supported(sd)) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  // Two cells: cell 0 owns face 2d+1 (left state), cell 1 owns face 2d (right state).
+  DevBuf d_qb, d_fb, d_face;
+  const long qf = z.B2 * z.D, ff = z.B2 * z.V;
+  EXA_TRY(d_qb.alloc(sizeof(double) * 2 * 6 * qf));
+  EXA_TRY(d_fb.alloc(sizeof(double) * 2 * 6 * ff));
+  EXA_TRY(d_face.alloc(sizeof(FaceDesc)));
+  EXA_TRY(hipMemcpy(d_qb.d() + (2 * direction + 1) * qf, QL, sizeof(double) * qf, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_qb.d() + (6 + 2 * direction) * qf, QR, sizeof(double) * qf, hipMemcpyHostToDevice));
+  const FaceDesc f{0, 1, direction, 0};
+  EXA_TRY(hipMemcpy(d_face.p, &f, sizeof(FaceDesc), hipMemcpyHostToDevice));
+  exa::RiemannArgs a{static_cast<const FaceDesc*>(d_face.p), 1, d_qb.d(), d_fb.d()};
+  EXA_TRY(exa::launch_linear_riemann(sd, a, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(FL, d_fb.d() + (2 * direction + 1) * ff, sizeof(double) * ff, hipMemcpyDeviceToHost));
+  EXA_TRY(hipMemcpy(FR, d_fb.d() + (6 + 2 * direction) * ff, sizeof(double) * ff, hipMemcpyDeviceToHost));
   return EXA_OK;
 }
 
@@ -328,6 +459,19 @@ hipError_t launch_initial_dt(exa_grid* g) {
 }
 
 hipError_t launch_predict(exa_grid* g) {
+  if (g->sd.linear) {  // Cauchy-Kowalevski predictor + volumeIntegralLinear
+    exa::LinearPredictorArgs la{};
+    la.luh = g->luh;
+    la.lduh = static_cast<double*>(g->lduh);
+    la.lQhbnd = static_cast<double*>(g->lQhbnd);
+    la.lFhbnd = static_cast<double*>(g->lFhbnd);
+    la.dx = g->d_dx;
+    la.dx_stride = 0;
+    la.dt = &g->st->dt;
+    la.cell0 = 0;
+    return staged(g, EXA_STAGE_PREDICTOR,
+                  [&] { return exa::launch_linear_predictor(g->sd, false, la, g->ncells, g->stream); });
+  }
   exa::PredictorArgs a{};
   a.luh = g->luh;
   a.lduh = g->lduh;
@@ -347,7 +491,9 @@ hipError_t launch_predict(exa_grid* g) {
 
 hipError_t launch_one_step(exa_grid* g) {
   exa::RiemannArgs ra{g->faces, g->nfaces, g->lQhbnd, g->lFhbnd};
-  hipError_t e = staged(g, EXA_STAGE_RIEMANN, [&] { return exa::launch_riemann(g->sd, ra, g->stream); });
+  hipError_t e = staged(g, EXA_STAGE_RIEMANN, [&] {
+    return g->sd.linear ? exa::launch_linear_riemann(g->sd, ra, g->stream) : exa::launch_riemann(g->sd, ra, g->stream);
+  });
   if (e != hipSuccess) return e;
   exa::UpdateArgs ua{};
   ua.luh = g->luh;
@@ -378,8 +524,10 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   SolverDesc sd;
   if (!to_desc(s, &sd) || nx < 1 || ny < 1 || nz < 1 || !lo || !hi) return EXA_E_INVALID;
   if (boundary != EXA_BC_PERIODIC && boundary != EXA_BC_TRANSMISSIVE) return EXA_E_UNSUPPORTED;
-  if (!exa::supported(sd)) return EXA_E_UNSUPPORTED;
   const bool periodic = boundary == EXA_BC_PERIODIC;
+  // Linear solvers: periodic grids only (at a transmissive face the linear
+  // Riemann solver's fluctuations vanish, which this grid does not model).
+  if (sd.linear ? !exa::linear_supported(sd) || !periodic : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
   if ((double)nx * ny * nz > 2.0e9) return EXA_E_INVALID;
   exa_grid* g = new exa_grid();
   g->sd = sd;
@@ -406,7 +554,7 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   A((void**)&g->lFhbnd, g->bulk * g->ncells * 6 * z.B2 * z.V);
   A((void**)&g->d_dx, sizeof(double) * 3);
   A((void**)&g->iters, sizeof(int) * g->ncells);
-  if (const size_t wsb = exa::predictor_ws_bytes_per_cell(sd)) {
+  if (const size_t wsb = sd.linear ? 0 : exa::predictor_ws_bytes_per_cell(sd)) {
     g->ws_cells = std::min(g->ncells, exa::kPredictorWsCells);
     A(&g->ws, wsb * g->ws_cells);
   }

@@ -39,6 +39,7 @@ bool to_desc(const exa_solver* s, SolverDesc* d) {
   d->no_time_averaging = s->no_time_averaging;
   d->cfl = s->cfl;
   d->precision = s->precision;
+  d->linear = s->linear;
   return s->order >= 0 && s->order <= 9 && s->num_variables > 0 && s->num_parameters >= 0;
 }
 

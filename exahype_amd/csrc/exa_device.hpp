@@ -191,6 +191,22 @@ struct Euler {
 #pragma unroll
     for (int m = 0; m < V; ++m) BgradQ[m] = T(0);
   }
+
+  // Linear-kernel interface: B_d grad_d Q of one direction and the
+  // coefficient matrix; the test solver's are zero
+  // (GenericEulerKernelTest3D.cpp:108-132).
+  template <int DIR, typename T>
+  __device__ static inline void ncp_dir(const T*, const T*, T* out) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) out[m] = T(0);
+  }
+  template <typename T>
+  __device__ static inline void coefficient_matrix(const T*, int, T (*Bn)[V]) {
+#pragma unroll
+    for (int l = 0; l < V; ++l)
+#pragma unroll
+      for (int k = 0; k < V; ++k) Bn[l][k] = T(0);
+  }
 };
 
 // Linear elasticity, velocity-stress form: variables (sxx, syy, szz, sxy,
@@ -236,6 +252,87 @@ struct Elastic {
     BgradQ[6] = (-irho * g[0][0] + -irho * g[1][3]) + -irho * g[2][5];
     BgradQ[7] = (-irho * g[0][3] + -irho * g[1][1]) + -irho * g[2][4];
     BgradQ[8] = (-irho * g[0][5] + -irho * g[1][4]) + -irho * g[2][2];
+  }
+
+  // Linear-kernel interface (the linear kernels take B_d grad_d Q per
+  // direction, spaceTimePredictorLinear.cpph:32): one direction's block of
+  // the product above (ElasticityKernelTest2D.cpp:130-181, the z block being
+  // the comment there).
+  template <int DIR, typename T>
+  __device__ static inline void ncp_dir(const T* Q, const T* g, T* out) {
+    const T lam = Q[9], mu = Q[10], irho = T(1) / Q[11];
+    const T l2m = lam + T(2) * mu;
+    if constexpr (DIR == 0) {
+      out[0] = -l2m * g[6];
+      out[1] = -lam * g[6];
+      out[2] = -lam * g[6];
+      out[3] = -mu * g[7];
+      out[4] = T(0);
+      out[5] = -mu * g[8];
+      out[6] = -irho * g[0];
+      out[7] = -irho * g[3];
+      out[8] = -irho * g[5];
+    } else if constexpr (DIR == 1) {
+      out[0] = -lam * g[7];
+      out[1] = -l2m * g[7];
+      out[2] = -lam * g[7];
+      out[3] = -mu * g[6];
+      out[4] = -mu * g[8];
+      out[5] = T(0);
+      out[6] = -irho * g[3];
+      out[7] = -irho * g[1];
+      out[8] = -irho * g[4];
+    } else {
+      out[0] = -lam * g[8];
+      out[1] = -lam * g[8];
+      out[2] = -l2m * g[8];
+      out[3] = T(0);
+      out[4] = -mu * g[7];
+      out[5] = -mu * g[6];
+      out[6] = -irho * g[5];
+      out[7] = -irho * g[4];
+      out[8] = -irho * g[2];
+    }
+  }
+
+  // coefficientMatrix, Bn[l][k] = B_d(k, l) (ElasticityKernelTest2D.cpp:
+  // 183-228, the z case being the comment there).
+  template <typename T>
+  __device__ static inline void coefficient_matrix(const T* Q, int d, T (*Bn)[V]) {
+#pragma unroll
+    for (int l = 0; l < V; ++l)
+#pragma unroll
+      for (int k = 0; k < V; ++k) Bn[l][k] = T(0);
+    const T lam = Q[9], mu = Q[10], irho = T(1) / Q[11];
+    const T l2m = lam + T(2) * mu;
+    if (d == 0) {
+      Bn[6][0] = -l2m;
+      Bn[6][1] = -lam;
+      Bn[6][2] = -lam;
+      Bn[7][3] = -mu;
+      Bn[8][5] = -mu;
+      Bn[0][6] = -irho;
+      Bn[3][7] = -irho;
+      Bn[5][8] = -irho;
+    } else if (d == 1) {
+      Bn[7][0] = -lam;
+      Bn[7][1] = -l2m;
+      Bn[7][2] = -lam;
+      Bn[6][3] = -mu;
+      Bn[8][4] = -mu;
+      Bn[3][6] = -irho;
+      Bn[1][7] = -irho;
+      Bn[4][8] = -irho;
+    } else {
+      Bn[8][0] = -lam;
+      Bn[8][1] = -lam;
+      Bn[8][2] = -l2m;
+      Bn[7][4] = -mu;
+      Bn[6][5] = -mu;
+      Bn[5][6] = -irho;
+      Bn[4][7] = -irho;
+      Bn[2][8] = -irho;
+    }
   }
 };
 

@@ -1,0 +1,438 @@
+// linear_kernels.hip -- ADER-DG kernels for linear PDEs on gfx950, FP64.
+//
+// Reference semantics followed (K = ExaHyPE/kernels/aderdg/generic/c/3d):
+//   linear_predictor_kernel  K/spaceTimePredictorLinear.cpph:34-443 (Cauchy-Kowalevski
+//                            procedure, time averages, extrapolation)
+//                            + K/volumeIntegralLinear.cpph:21-53 (fused epilogue)
+//   linear_riemann_kernel    K/riemannSolverLinear.cpph:23-132
+//   linear_volume_kernel     K/volumeIntegralLinear.cpph:21-53
+//   linear_surface_kernel    K/surfaceIntegralLinear.cpph:21-90
+//
+// The predictor runs one workgroup per cell and one thread per node (z,y,x).
+// The time derivatives q_0 .. q_B of the Taylor expansion are formed one after
+// the other: each derivative's variables go through LDS (two alternating
+// buffers, one barrier per derivative) so that every thread can take its
+// node's three directional gradients, B_d grad_d q_i follows from the PDE's
+// directional non-conservative product, and q_{i+1} = -sum_d B_d grad_d q_i.
+// A thread keeps its node's running time averages of q and of the three
+// directional products in registers; nothing of the space-time expansion is
+// written to memory unless the caller asks for the reference's temporaries.
+#include "aderdg.hpp"
+#include "exa_device.hpp"
+
+#include <type_traits>
+
+namespace exa {
+
+namespace {
+
+// Node index of line position l through face node (a, b) of a face normal
+// to direction dir: x faces (a, b) = (z, y), y faces (z, x), z faces (y, x).
+__device__ inline int face_line_node(int dir, int a, int b, int l, int B) {
+  return dir == 0 ? (a * B + b) * B + l : (dir == 1 ? (a * B + l) * B + b : (l * B + a) * B + b);
+}
+
+}  // namespace
+
+template <int B, class PDE, bool EXPORT, bool PS>
+__global__ __launch_bounds__(node_threads(B)) void linear_predictor_kernel(LinearPredictorArgs a) {
+  constexpr int V = PDE::V, P = PDE::P, D = V + P;
+  constexpr int N = B - 1;
+  constexpr int B2 = B * B, B3 = B2 * B, B4 = B3 * B;
+  constexpr int NT = node_threads(B);
+  __shared__ double sQ[2][V * B3];  // time derivative i in buffer i & 1, [var][node]
+  __shared__ double sE[D * B3];     // extrapolation staging, [node][var]
+  __shared__ double sDu[B2];        // dudx[l][n]
+  const size_t cell = (size_t)blockIdx.x + a.cell0;
+  const int tid = threadIdx.x;
+  const bool active = tid < B3;
+  const int node = active ? tid : 0;
+  const int z = node / B2, y = (node / B) % B, x = node % B;
+  for (int e = tid; e < B2; e += NT) sDu[e] = exa_dudx[N][e / B][e % B];
+  const double* dxp = a.dx + cell * a.dx_stride;
+  const double rdx0 = 1.0 / dxp[0], rdx1 = 1.0 / dxp[1], rdx2 = 1.0 / dxp[2];
+  const double dt = *a.dt;
+  const double* luh = a.luh + cell * (size_t)(B3 * D) + node * D;
+  double Q[D];  // the current derivative's variables, then the (constant) parameters
+#pragma unroll
+  for (int m = 0; m < D; ++m) Q[m] = luh[m];
+  double psum[PS ? V : 1];
+  if constexpr (PS) {  // :79-98, sum_n w_n PSi[n+1]; lQi[0] += dt * sum
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      double s = 0.0;
+#pragma unroll
+      for (int n = 1; n < B + 1; ++n) s = s + exa_gl_weights[N][n - 1] * a.PSi[((size_t)n * B3 + node) * V + m];
+      psum[m] = s;
+      Q[m] += dt * s;
+    }
+  }
+  if (EXPORT && active) {
+#pragma unroll
+    for (int m = 0; m < D; ++m) a.lQi[(size_t)node * D + m] = Q[m];
+  }
+  double qh[V], fh[3][V];
+  double fac = 0.5 * dt;  // Taylor factor dt^i / (i+1)! of the time average, :235-250
+#pragma unroll 1
+  for (int i = 0; i < B; ++i) {
+    double* const sq = sQ[i & 1];
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) sq[m * B3 + node] = Q[m];
+    }
+    __syncthreads();
+    double qn[V];  // :99-105 (point source) and :205-219
+#pragma unroll
+    for (int m = 0; m < V; ++m) qn[m] = (PS && i == 0) ? a.PSi[(size_t)node * V + m] : 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      // gradient of q_i in direction d, :118-186: sum_n (1/dx_d q_i) dudx[l][n]
+      double g[V];
+      const double rdx = d == 0 ? rdx0 : (d == 1 ? rdx1 : rdx2);
+      const int l = d == 0 ? x : (d == 1 ? y : z);
+      const int base = d == 0 ? (z * B + y) * B : (d == 1 ? z * B2 + x : y * B + x);
+      const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        double acc = 0.0;
+#pragma unroll
+        for (int n = 0; n < B; ++n) acc += rdx * sq[m * B3 + base + n * stride] * sDu[l * B + n];
+        g[m] = acc;
+      }
+      double F[V];  // :188-203, B_d grad_d q_i
+      if (d == 0)
+        PDE::template ncp_dir<0>(Q, g, F);
+      else if (d == 1)
+        PDE::template ncp_dir<1>(Q, g, F);
+      else
+        PDE::template ncp_dir<2>(Q, g, F);
+      if (EXPORT && active) {
+        const size_t o = (((size_t)i * B3 + node) * 3 + d) * V;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+          a.gradQ[o + m] = g[m];
+          a.lFi[o + m] = F[m];
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        if (i == 0)
+          fh[d][m] = F[m];
+        else
+          fh[d][m] += fac * F[m];
+        qn[m] -= F[m];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      if (i == 0)
+        qh[m] = Q[m];
+      else
+        qh[m] += fac * Q[m];
+      Q[m] = qn[m];
+    }
+    if (i > 0) fac *= dt / (i + 2);
+    if (EXPORT && active) {  // lQi[i+1], parameters copied (:221-232)
+#pragma unroll
+      for (int m = 0; m < D; ++m) a.lQi[((size_t)(i + 1) * B3 + node) * D + m] = Q[m];
+    }
+  }
+  if constexpr (PS) {  // :292-309
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int m = 0; m < V; ++m) fh[d][m] += psum[m] / 3;
+  }
+  if (a.lduh && active) {  // volumeIntegralLinear, lduh = -w^3 sum_d lFhi_d
+    const double w = exa_gl_weights[N][z] * exa_gl_weights[N][y] * exa_gl_weights[N][x];
+    double* du = a.lduh + cell * (size_t)(B3 * V) + node * V;
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      double v = 0.0;
+      v -= w * fh[0][m];
+      v -= w * fh[1][m];
+      v -= w * fh[2][m];
+      du[m] = v;
+    }
+  }
+  if constexpr (EXPORT) {
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) a.lQhi[(size_t)node * D + m] = qh[m];
+#pragma unroll
+      for (int m = V; m < D; ++m) a.lQhi[(size_t)node * D + m] = Q[m];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < V; ++m) a.lFhi[((size_t)d * B3 + node) * V + m] = fh[d][m];
+#pragma unroll
+      for (int m = 0; m < V; ++m) a.lFhi[((size_t)3 * B3 + node) * V + m] = 0.0;
+    }
+    for (size_t e = tid; e < (size_t)B4 * V; e += NT) a.lFi[(size_t)B4 * 3 * V + e] = 0.0;
+  }
+  // Extrapolation to the six faces, :312-436: lQhi (D values), then the x, y
+  // and z time-averaged products onto their own two faces.
+  double* const qb = a.lQhbnd + cell * (size_t)(6 * B2 * D);
+  double* const fb = a.lFhbnd + cell * (size_t)(6 * B2 * V);
+#pragma unroll 1
+  for (int pass = 0; pass < 4; ++pass) {
+    const int W = pass == 0 ? D : V;
+    __syncthreads();
+    if (active) {
+      if (pass == 0) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) sE[node * D + m] = qh[m];
+#pragma unroll
+        for (int m = V; m < D; ++m) sE[node * D + m] = Q[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < V; ++m)
+          sE[node * V + m] = pass == 1 ? fh[0][m] : (pass == 2 ? fh[1][m] : fh[2][m]);
+      }
+    }
+    __syncthreads();
+    const int nfaces = pass == 0 ? 6 : 2;
+    const int face0 = pass == 0 ? 0 : 2 * (pass - 1);
+    double* const out = pass == 0 ? qb : fb + (size_t)face0 * B2 * V;
+    for (int e = tid; e < nfaces * B2 * W; e += NT) {
+      const int face = face0 + e / (B2 * W);
+      const int r = e % (B2 * W), ab = r / W, m = r % W;
+      const int dir = face >> 1;
+      const bool right = face & 1;
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < B; ++l)
+        acc += sE[face_line_node(dir, ab / B, ab % B, l, B) * W + m] * (right ? exa_fr_coeff[N][l] : exa_fl_coeff[N][l]);
+      out[e] = acc;
+    }
+  }
+}
+
+// One face per wavefront.  The face averages of both states give the
+// dissipation speed smax and the coefficient matrix of the averaged state;
+// the fluctuations are FL = 0.5 (Bn - smax I)^T (QR - QL) into the left
+// cell's face 2d+1 and FR = 0.5 (Bn + smax I)^T (QR - QL) into the right
+// cell's face 2d.
+template <int B, class PDE>
+__global__ __launch_bounds__(256) void linear_riemann_kernel(RiemannArgs a) {
+  constexpr int V = PDE::V, P = PDE::P, D = V + P;
+  constexpr int N = B - 1;
+  constexpr int B2 = B * B;
+  constexpr int WPB = 256 / kWave;
+  static_assert(B2 <= kWave, "one face node per lane");
+  static_assert(D <= 32, "two face averages per wavefront");
+  __shared__ double sQL[WPB][B2 * D], sQR[WPB][B2 * D];
+  __shared__ double sFL[WPB][B2 * V], sFR[WPB][B2 * V];
+  __shared__ double sAv[WPB][2 * D];
+  __shared__ double sBm[WPB][V * V], sBp[WPB][V * V];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const long f = (long)blockIdx.x * WPB + w;
+  if (f >= a.nfaces) return;  // whole wavefront exits together
+  const FaceDesc fd = a.faces[f];
+  const int d = fd.dir;
+  double* FL = static_cast<double*>(a.lFhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * V);
+  double* FR = static_cast<double*>(a.lFhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
+  const double* QL = static_cast<const double*>(a.lQhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * D);
+  const double* QR = static_cast<const double*>(a.lQhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * D);
+  for (int e = lane; e < B2 * D; e += kWave) {
+    sQL[w][e] = QL[e];
+    sQR[w][e] = QR[e];
+  }
+  wave_lds_sync();
+  // face averages, :40-66 (lanes 0..D-1 left, 32..32+D-1 right)
+  const int k = lane & 31;
+  if (k < D) {
+    const double* src = lane < 32 ? sQL[w] : sQR[w];
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) acc += exa_gl_weights[N][i] * exa_gl_weights[N][j] * src[(i * B + j) * D + k];
+    sAv[w][(lane < 32 ? 0 : D) + k] = acc;
+  }
+  wave_lds_sync();
+  double QavL[D], QavR[D], Qav[D];
+#pragma unroll
+  for (int m = 0; m < D; ++m) {
+    QavL[m] = sAv[w][m];
+    QavR[m] = sAv[w][D + m];
+    Qav[m] = 0.5 * (QavL[m] + QavR[m]);
+  }
+  const double sL = PDE::max_abs_eigenvalue(QavL, d);  // :68-78
+  const double sR = PDE::max_abs_eigenvalue(QavR, d);
+  const double smax = (sL < sR) ? sR : sL;
+  if (lane == 0) PDE::coefficient_matrix(Qav, d, reinterpret_cast<double(*)[V]>(sBm[w]));  // :80-99
+  wave_lds_sync();
+  for (int e = lane; e < V * V; e += kWave) {
+    const double bn = sBm[w][e];
+    const bool diag = e / V == e % V;
+    sBp[w][e] = diag ? bn + smax : bn;
+    sBm[w][e] = diag ? bn - smax : bn;
+  }
+  wave_lds_sync();
+  if (lane < B2) {  // :101-128
+    double q[V];
+#pragma unroll
+    for (int l = 0; l < V; ++l) q[l] = 0.5 * (sQR[w][lane * D + l] - sQL[w][lane * D + l]);
+#pragma unroll
+    for (int kk = 0; kk < V; ++kk) {
+      double fl = 0.0, fr = 0.0;
+#pragma unroll
+      for (int l = 0; l < V; ++l) {
+        fl += sBm[w][l * V + kk] * q[l];
+        fr += sBp[w][l * V + kk] * q[l];
+      }
+      sFL[w][lane * V + kk] = fl;
+      sFR[w][lane * V + kk] = fr;
+    }
+  }
+  wave_lds_sync();
+  for (int e = lane; e < B2 * V; e += kWave) {
+    FL[e] = sFL[w][e];
+    FR[e] = sFR[w][e];
+  }
+}
+
+template <int B, int V>
+__global__ __launch_bounds__(node_threads(B)) void linear_volume_kernel(double* lduh, const double* lFhi) {
+  constexpr int N = B - 1, B2 = B * B, B3 = B2 * B;
+  const int node = threadIdx.x;
+  if (node >= B3) return;
+  const size_t cell = blockIdx.x;
+  const int z = node / B2, y = (node / B) % B, x = node % B;
+  const double w = exa_gl_weights[N][z] * exa_gl_weights[N][y] * exa_gl_weights[N][x];
+  const double* F = lFhi + cell * (size_t)(4 * B3 * V);  // three directions + the source block
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    double v = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v -= w * F[((size_t)d * B3 + node) * V + m];
+    lduh[cell * (size_t)(B3 * V) + node * V + m] = v;
+  }
+}
+
+// lduh -= (F_{2d+1} FRCoeff + F_{2d} FLCoeff) w_a w_b / dx_d per direction.
+template <int B, int V>
+__global__ __launch_bounds__(node_threads(B)) void linear_surface_kernel(double* lduh, const double* lFhbnd,
+                                                                        const double* dx) {
+  constexpr int N = B - 1, B2 = B * B, B3 = B2 * B;
+  const int node = threadIdx.x;
+  if (node >= B3) return;
+  const size_t cell = blockIdx.x;
+  const int z = node / B2, y = (node / B) % B, x = node % B;
+  const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
+  const double* lF = lFhbnd + cell * (size_t)(6 * B2 * V);
+  double* du = lduh + cell * (size_t)(B3 * V) + node * V;
+  const int fn[3] = {z * B + y, z * B + x, y * B + x};
+  const int ln[3] = {x, y, z};
+  const double us[3] = {(wz * wy) / dx[0], (wz * wx) / dx[1], (wy * wx) / dx[2]};
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    double v = du[m];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      v -= (lF[((2 * d + 1) * B2 + fn[d]) * V + m] * exa_fr_coeff[N][ln[d]] +
+            lF[((2 * d) * B2 + fn[d]) * V + m] * exa_fl_coeff[N][ln[d]]) *
+           us[d];
+    du[m] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side dispatch.  Linear solvers: elasticity at orders 2..5, and the
+// five-variable Euler test solver with one parameter at order 3 (the sizes of
+// the reference's linear-kernel unit tests); the integrals at those sizes.
+// ---------------------------------------------------------------------------
+template <class F>
+static hipError_t dispatch_linear(const SolverDesc& sd, F&& f) {
+  if (sd.pde == kPdeEuler && sd.nvar == 5 && sd.npar == 1 && sd.order == 3) return f.template operator()<4, Euler<1>>();
+  if (sd.pde == kPdeElastic && sd.nvar == Elastic::V && sd.npar == Elastic::P) {
+    switch (sd.order) {
+      case 2: return f.template operator()<3, Elastic>();
+      case 3: return f.template operator()<4, Elastic>();
+      case 4: return f.template operator()<5, Elastic>();
+      case 5: return f.template operator()<6, Elastic>();
+      default: break;
+    }
+  }
+  return hipErrorNotSupported;
+}
+
+// The integrals depend on (basis size, variables) only.
+template <class F>
+static hipError_t dispatch_linear_sizes(const SolverDesc& sd, F&& f) {
+  if (sd.precision != kFP64) return hipErrorNotSupported;
+  if (sd.nvar == 5 && sd.order == 3) return f.template operator()<4, 5>();
+  if (sd.nvar == Elastic::V) {
+    switch (sd.order) {
+      case 2: return f.template operator()<3, 9>();
+      case 3: return f.template operator()<4, 9>();
+      case 4: return f.template operator()<5, 9>();
+      case 5: return f.template operator()<6, 9>();
+      default: break;
+    }
+  }
+  return hipErrorNotSupported;
+}
+
+bool linear_sizes_supported(const SolverDesc& sd) {
+  return dispatch_linear_sizes(sd, []<int B, int V>() -> hipError_t { return hipSuccess; }) == hipSuccess;
+}
+
+bool linear_supported(const SolverDesc& sd) {
+  if (!sd.linear || sd.precision != kFP64) return false;
+  return dispatch_linear(sd, []<int B, class PDE>() -> hipError_t { return hipSuccess; }) == hipSuccess;
+}
+
+hipError_t launch_linear_predictor(const SolverDesc& sd, bool export_temps, const LinearPredictorArgs& a,
+                                   long ncells, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  if (!linear_supported(sd)) return hipErrorNotSupported;
+  if ((export_temps || a.PSi) && ncells != 1) return hipErrorInvalidValue;  // one cell's temporaries / sources
+  if (export_temps && (!a.lQi || !a.lFi || !a.gradQ || !a.lQhi || !a.lFhi)) return hipErrorInvalidValue;
+  return dispatch_linear(sd, [&]<int B, class PDE>() -> hipError_t {
+    const dim3 grid((unsigned)ncells), block(node_threads(B));
+    if (export_temps) {
+      if (a.PSi)
+        hipLaunchKernelGGL((linear_predictor_kernel<B, PDE, true, true>), grid, block, 0, s, a);
+      else
+        hipLaunchKernelGGL((linear_predictor_kernel<B, PDE, true, false>), grid, block, 0, s, a);
+    } else if (a.PSi) {
+      hipLaunchKernelGGL((linear_predictor_kernel<B, PDE, false, true>), grid, block, 0, s, a);
+    } else {
+      hipLaunchKernelGGL((linear_predictor_kernel<B, PDE, false, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_linear_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s) {
+  if (a.nfaces <= 0) return hipSuccess;
+  if (!linear_supported(sd) || a.status) return hipErrorNotSupported;
+  return dispatch_linear(sd, [&]<int B, class PDE>() -> hipError_t {
+    const dim3 grid((unsigned)((a.nfaces + 3) / 4)), block(256);
+    hipLaunchKernelGGL((linear_riemann_kernel<B, PDE>), grid, block, 0, s, a);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_linear_volume(const SolverDesc& sd, double* lduh, const double* lFhi, long ncells,
+                                hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  return dispatch_linear_sizes(sd, [&]<int B, int V>() -> hipError_t {
+    hipLaunchKernelGGL((linear_volume_kernel<B, V>), dim3((unsigned)ncells), dim3(node_threads(B)), 0, s, lduh, lFhi);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_linear_surface(const SolverDesc& sd, double* lduh, const double* lFhbnd, const double* dx,
+                                 long ncells, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  return dispatch_linear_sizes(sd, [&]<int B, int V>() -> hipError_t {
+    hipLaunchKernelGGL((linear_surface_kernel<B, V>), dim3((unsigned)ncells), dim3(node_threads(B)), 0, s, lduh,
+                       lFhbnd, dx);
+    return hipGetLastError();
+  });
+}
+
+}  // namespace exa

@@ -60,6 +60,8 @@ typedef struct exa_solver {
   int no_time_averaging; /* template flag noTimeAveraging */
   double cfl;            /* SolverType::CFL */
   int precision;         /* EXA_PRECISION_* (0 = FP64) */
+  int linear;            /* 1: a linear ADER-DG solver -- the generated glue calls the *Linear
+                            kernels (AbstractGenericADERDGSolverImplementation.template:53-86) */
 } exa_solver;
 
 const char* exa_status_string(int status);
@@ -96,6 +98,35 @@ int exa_surface_integral_nonlinear(const exa_solver* s, double* lduh, const doub
 int exa_riemann_solver_nonlinear(const exa_solver* s, double* FL, double* FR, const double* QL,
                                  const double* QR, double dt, int direction);
 
+/* ---- linear ADER-DG kernels (FP64; the entry point implies the solver's linear flag).
+ * Supported: elasticity (9 variables, 3 parameters) at orders 2..5 and the
+ * five-variable Euler test solver with one parameter at order 3. */
+
+/* spaceTimePredictorLinear (K/c/3d/spaceTimePredictorLinear.cpph:34), the
+ * Cauchy-Kowalevski predictor.  lQi, lFi, gradQ, lQhi, lFhi are the reference's
+ * temporaries (tempSpaceTimeUnknowns[0], tempSpaceTimeFluxUnknowns[0..1],
+ * tempUnknowns, tempFluxUnknowns), written when all five are given (or all
+ * null): lQi (t,z,y,x,var) (B+1) B^3 (V+P); lFi (t,z,y,x,dir,var) 3 B^4 V plus
+ * a zeroed source block of B^4 V; gradQ (t,z,y,x,dir,var) 3 B^4 V; lQhi B^3 (V+P);
+ * lFhi (dir,z,y,x,var) 3 B^3 V plus a zeroed source block of B^3 V.  PSi:
+ * point sources (t,z,y,x,var) (B+1) B^3 V (tempPointForceSources), or null. */
+int exa_space_time_predictor_linear(const exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi,
+                                    double* lFi, double* gradQ, double* lQhi, double* lFhi,
+                                    const double* luh, const double* dx, double dt, const double* PSi);
+
+/* volumeIntegralLinear (K/c/3d/volumeIntegralLinear.cpph:22); lFhi 4 B^3 V
+ * (the source block is not read); dx is unused, as in the reference. */
+int exa_volume_integral_linear(const exa_solver* s, double* lduh, const double* lFhi, const double* dx);
+
+/* surfaceIntegralLinear (K/c/3d/surfaceIntegralLinear.cpph:27); lduh in/out. */
+int exa_surface_integral_linear(const exa_solver* s, double* lduh, const double* lFhbnd,
+                                const double* dx);
+
+/* riemannSolverLinear (K/c/3d/riemannSolverLinear.cpph:25); FL, FR out
+ * (fluctuations); dt is unused, as in the reference. */
+int exa_riemann_solver_linear(const exa_solver* s, double* FL, double* FR, const double* QL,
+                              const double* QR, double dt, int direction);
+
 /* solutionUpdate (K/c/3d/solutionUpdate.cpph:23); luh in/out. */
 int exa_solution_update(const exa_solver* s, double* luh, const double* lduh, double dt);
 
@@ -127,7 +158,10 @@ enum { EXA_BC_PERIODIC = 0, EXA_BC_TRANSMISSIVE = 1 };
 /* Uniform nx*ny*nz Cartesian grid on [lo, hi]; cell c = (k*ny + j)*nx + i;
  * boundary EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE (boundaryConditions,
  * ExaHyPE/kernels/aderdg/generic/c/3d/boundaryConditions.cpph:24, with
- * boundaryValues returning the inside state). */
+ * boundaryValues returning the inside state).  A solver with linear = 1 runs
+ * the linear kernels (Cauchy-Kowalevski predictor + volumeIntegralLinear,
+ * riemannSolverLinear, surfaceIntegralLinear + solutionUpdate) on periodic
+ * grids only; EXA_E_UNSUPPORTED otherwise. */
 int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo,
                     const double* hi, int boundary, exa_grid** out);
 int exa_grid_destroy(exa_grid* g);

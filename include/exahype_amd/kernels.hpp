@@ -50,6 +50,7 @@ inline exa_solver descriptor(int terms, bool noTimeAveraging) {
   s.no_time_averaging = noTimeAveraging ? 1 : 0;
   s.cfl = SolverType::CFL;
   s.precision = EXA_PRECISION_FP64;
+  s.linear = 0;
   return s;
 }
 
@@ -64,6 +65,7 @@ inline exa_solver size_descriptor(int numberOfVariables, int basisSize, int term
   s.no_time_averaging = noTimeAveraging ? 1 : 0;
   s.cfl = 0.9;
   s.precision = EXA_PRECISION_FP64;
+  s.linear = 0;
   return s;
 }
 
@@ -147,6 +149,54 @@ double stableTimeStepSize(SolverType& /*solver*/, const double* const luh, const
   return dt;
 }
 
+// ---- linear kernels (the glue's isLinear branch) ----
+
+template <bool usePointSource, bool useSource, bool useFlux, bool useNCP, typename SolverType, typename DxVector>
+void spaceTimePredictorLinear(SolverType& /*solver*/, double* lQbnd, double* lFbnd, double** tempSpaceTimeUnknowns,
+                              double** tempSpaceTimeFluxUnknowns, double* tempUnknowns, double* tempFluxUnknowns,
+                              const double* const luh, const DxVector& dx, const double dt,
+                              double* tempPointForceSources) {
+  exa_solver s = exahype_amd::descriptor<SolverType>(exahype_amd::terms(useSource, useFlux, useNCP), false);
+  s.linear = 1;
+  double d[3];
+  exahype_amd::dx3(dx, d);
+  const bool temps = tempSpaceTimeUnknowns && tempSpaceTimeFluxUnknowns && tempUnknowns && tempFluxUnknowns;
+  exahype_amd::check(
+      exa_space_time_predictor_linear(&s, lQbnd, lFbnd, temps ? tempSpaceTimeUnknowns[0] : nullptr,
+                                      temps ? tempSpaceTimeFluxUnknowns[0] : nullptr,
+                                      temps ? tempSpaceTimeFluxUnknowns[1] : nullptr, temps ? tempUnknowns : nullptr,
+                                      temps ? tempFluxUnknowns : nullptr, luh, d, dt,
+                                      usePointSource ? tempPointForceSources : nullptr),
+      "spaceTimePredictorLinear");
+}
+
+template <bool useSourceOrNCP, bool useFlux, int numberOfVariables, int basisSize, typename DxVector>
+void volumeIntegralLinear(double* lduh, const double* const lFhi, const DxVector& dx) {
+  exa_solver s = exahype_amd::size_descriptor(
+      numberOfVariables, basisSize, (useFlux ? EXA_TERM_FLUX : 0) | (useSourceOrNCP ? EXA_TERM_NCP : 0), false);
+  s.linear = 1;
+  double d[3];
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(exa_volume_integral_linear(&s, lduh, lFhi, d), "volumeIntegralLinear");
+}
+
+template <int numberOfVariables, int basisSize, typename DxVector>
+void surfaceIntegralLinear(double* lduh, const double* const lFbnd, const DxVector& dx) {
+  exa_solver s = exahype_amd::size_descriptor(numberOfVariables, basisSize, EXA_TERM_NCP, false);
+  s.linear = 1;
+  double d[3];
+  exahype_amd::dx3(dx, d);
+  exahype_amd::check(exa_surface_integral_linear(&s, lduh, lFbnd, d), "surfaceIntegralLinear");
+}
+
+template <bool useNCP, typename SolverType>
+void riemannSolverLinear(SolverType& /*solver*/, double* FL, double* FR, const double* const QL,
+                         const double* const QR, const double dt, const int normalNonZero) {
+  exa_solver s = exahype_amd::descriptor<SolverType>(useNCP ? EXA_TERM_NCP : EXA_TERM_FLUX, false);
+  s.linear = 1;
+  exahype_amd::check(exa_riemann_solver_linear(&s, FL, FR, QL, QR, dt, normalNonZero), "riemannSolverLinear");
+}
+
 }  // namespace c
 }  // namespace generic
 }  // namespace aderdg
@@ -217,6 +267,7 @@ double solutionUpdate(SolverType& /*solver*/, double* luh_new, const double* luh
   s.no_time_averaging = 0;
   s.cfl = SolverType::CFL;
   s.precision = EXA_PRECISION_FP64;
+  s.linear = 0;
   double d[3], dt_max = 0.0;
   exahype_amd::dx3(dx, d);
   exahype_amd::check(exa_musclhancock_solution_update(&s, luh_new, luh, d, dt, SolverType::PatchSize,
@@ -246,6 +297,7 @@ double solutionUpdate(SolverType& /*solver*/, double* luh_new, const double* luh
   s.no_time_averaging = 0;
   s.cfl = SolverType::CFL;
   s.precision = EXA_PRECISION_FP64;
+  s.linear = 0;
   double d[3], dt_max = 0.0;
   exahype_amd::dx3(dx, d);
   exahype_amd::check(exa_godunov_solution_update(&s, luh_new, luh, d, dt, SolverType::PatchSize,

@@ -882,7 +882,7 @@ static scratch_t scratch_alloc(const ora_solver* s) {
   const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
   const long B3 = B * B * B, B4 = B3 * B;
   scratch_t t;
-  t.lQi = (double*)malloc(sizeof(double) * B4 * D);
+  t.lQi = (double*)malloc(sizeof(double) * (B4 + B3) * D); /* (B+1) B^3 D for the linear predictor */
   t.rhs = (double*)malloc(sizeof(double) * B4 * D);
   t.lFi = (double*)malloc(sizeof(double) * B4 * 4 * V);
   t.gradQ = (double*)malloc(sizeof(double) * B4 * 3 * V);
@@ -910,6 +910,13 @@ static void predict_cells(ora_grid* g, long c0, long c1) {
     scratch_t t = scratch_alloc(s);
 #pragma omp for schedule(dynamic)
     for (long c = c0; c < c1; c++) {
+      if (s->linear) { /* the glue's isLinear branch (AbstractGenericADERDGSolverImplementation.template:53-86) */
+        ora_space_time_predictor_linear(s, g->lQhbnd + c * 6 * B2 * D, g->lFhbnd + c * 6 * B2 * V, t.lQi, t.lFi,
+                                        t.gradQ, t.lQhi, t.lFhi, g->luh + c * B3 * D, g->dx, g->dt, NULL);
+        ora_volume_integral_linear(s, g->lduh + c * B3 * V, t.lFhi);
+        g->iters[c] = 0;
+        continue;
+      }
       g->iters[c] = ora_space_time_predictor_nonlinear(
           s, g->lQhbnd + c * 6 * B2 * D, g->lFhbnd + c * 6 * B2 * V, t.lQi, t.rhs, t.lFi, t.gradQ,
           t.lQhi, t.lFhi, g->luh + c * B3 * D, g->dx, g->dt);
@@ -927,7 +934,10 @@ static void riemann_face(ora_grid* g, long cl, long cr, int d) {
   double* FR = g->lFhbnd + (cr * 6 + 2 * d) * B2 * V;
   const double* QL = g->lQhbnd + (cl * 6 + 2 * d + 1) * B2 * D;
   const double* QR = g->lQhbnd + (cr * 6 + 2 * d) * B2 * D;
-  ora_riemann_solver_nonlinear(s, FL, FR, QL, QR, g->dt, d);
+  if (s->linear)
+    ora_riemann_solver_linear(s, FL, FR, QL, QR, g->dt, d);
+  else
+    ora_riemann_solver_nonlinear(s, FL, FR, QL, QR, g->dt, d);
 }
 
 static void boundary_face(ora_grid* g, long c, int face) {
@@ -939,10 +949,16 @@ static void boundary_face(ora_grid* g, long c, int face) {
   double FOut[10 * 10 * 16], QOut[10 * 10 * 16];
   memcpy(FOut, FIn, sizeof(double) * B2 * V);
   memcpy(QOut, QIn, sizeof(double) * B2 * D);
-  if (face % 2 == 0)
+  if (s->linear) {
+    if (face % 2 == 0)
+      ora_riemann_solver_linear(s, FOut, FIn, QOut, QIn, g->dt, face / 2);
+    else
+      ora_riemann_solver_linear(s, FIn, FOut, QIn, QOut, g->dt, face / 2);
+  } else if (face % 2 == 0) {
     ora_riemann_solver_nonlinear(s, FOut, FIn, QOut, QIn, g->dt, face / 2);
-  else
+  } else {
     ora_riemann_solver_nonlinear(s, FIn, FOut, QIn, QOut, g->dt, face / 2);
+  }
 }
 
 static void update_cells(ora_grid* g, long c0, long c1) {
@@ -951,7 +967,10 @@ static void update_cells(ora_grid* g, long c0, long c1) {
   const long B2 = B * B, B3 = B2 * B;
 #pragma omp parallel for schedule(static)
   for (long c = c0; c < c1; c++) {
-    ora_surface_integral_nonlinear(s, g->lduh + c * B3 * V, g->lFhbnd + c * 6 * B2 * V, g->dx);
+    if (s->linear)
+      ora_surface_integral_linear(s, g->lduh + c * B3 * V, g->lFhbnd + c * 6 * B2 * V, g->dx);
+    else
+      ora_surface_integral_nonlinear(s, g->lduh + c * B3 * V, g->lFhbnd + c * 6 * B2 * V, g->dx);
     ora_solution_update(s, g->luh + c * B3 * D, g->lduh + c * B3 * V, g->dt);
   }
 }

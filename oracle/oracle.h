@@ -40,6 +40,7 @@ typedef struct ora_solver {
   int use_ncp;
   int no_time_averaging;
   double cfl;
+  int linear; /* 1: the grid driver runs the linear kernels (the glue's isLinear branch) */
 } ora_solver;
 
 /* ---- PDE callbacks (user solver interface restated) ---- */

@@ -23,7 +23,7 @@ class Solver(ctypes.Structure):
     _fields_ = [("pde", ctypes.c_int), ("order", ctypes.c_int), ("nvar", ctypes.c_int),
                 ("npar", ctypes.c_int), ("use_flux", ctypes.c_int), ("use_source", ctypes.c_int),
                 ("use_ncp", ctypes.c_int), ("no_time_averaging", ctypes.c_int),
-                ("cfl", ctypes.c_double)]
+                ("cfl", ctypes.c_double), ("linear", ctypes.c_int)]
 
 
 def build():
@@ -159,9 +159,9 @@ def eigenvalues(s, Q, d):
 
 
 def make_solver(order, nvar=5, npar=0, pde=PDE_EULER, use_flux=True, use_source=False,
-                use_ncp=False, no_time_averaging=False, cfl=0.9):
+                use_ncp=False, no_time_averaging=False, cfl=0.9, linear=False):
     return Solver(pde, order, nvar, npar, int(use_flux), int(use_source), int(use_ncp),
-                  int(no_time_averaging), cfl)
+                  int(no_time_averaging), cfl, int(linear))
 
 
 def table(which, order, i, j=0):

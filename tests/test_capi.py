@@ -133,9 +133,28 @@ double godunov_glue(MyGodunovSolver& fv, double* luhNew, const double* luh) {
   return kernels::finitevolumes::godunov::c::solutionUpdate<false, false, true, MyGodunovSolver>(fv, luhNew, luh,
                                                                                                dx, 1e-3);
 }
+// The glue's isLinear branch for a linear elasticity solver.
+struct MyElasticSolver {
+  static constexpr int NumberOfVariables = 9;
+  static constexpr int NumberOfParameters = 3;
+  static constexpr int Order = 3;
+  static constexpr double CFL = 0.9;
+  static constexpr int ExaDevicePde = EXA_PDE_ELASTIC;
+};
+void linear_glue(MyElasticSolver& solver, double* luh, double* lduh, double* lQhbnd, double* lFhbnd,
+                 double** stu, double** stfu, double* tu, double* tfu, double* FL, double* FR, double* ps) {
+  const Vec3 dx{{0.1, 0.1, 0.1}};
+  kernels::aderdg::generic::c::spaceTimePredictorLinear<true, false, false, true, MyElasticSolver>(
+      solver, lQhbnd, lFhbnd, stu, stfu, tu, tfu, luh, dx, 1e-3, ps);
+  kernels::aderdg::generic::c::volumeIntegralLinear<true, false, 9, 4>(lduh, tfu, dx);
+  kernels::aderdg::generic::c::riemannSolverLinear<true, MyElasticSolver>(solver, FL, FR, lQhbnd, lQhbnd, 1e-3, 0);
+  kernels::aderdg::generic::c::surfaceIntegralLinear<9, 4>(lduh, lFhbnd, dx);
+  kernels::aderdg::generic::c::solutionUpdate<MyElasticSolver>(solver, luh, lduh, 1e-3);
+}
 int main(int argc, char**) {
   if (argc > 99) {
     glue(*(MyEulerSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
+    linear_glue(*(MyElasticSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
     MyFVSolver fv;
     MyGodunovSolver gv;
     return (int)(limiter_glue(nullptr, fv, 0, 0, 0, 0, 0) + godunov_glue(gv, 0, 0));

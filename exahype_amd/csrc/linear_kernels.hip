@@ -8,15 +8,16 @@
 //   linear_volume_kernel     K/volumeIntegralLinear.cpph:21-53
 //   linear_surface_kernel    K/surfaceIntegralLinear.cpph:21-90
 //
-// The predictor runs one workgroup per cell and one thread per node (z,y,x).
-// The time derivatives q_0 .. q_B of the Taylor expansion are formed one after
-// the other: each derivative's variables go through LDS (two alternating
-// buffers, one barrier per derivative) so that every thread can take its
-// node's three directional gradients, B_d grad_d q_i follows from the PDE's
-// directional non-conservative product, and q_{i+1} = -sum_d B_d grad_d q_i.
-// A thread keeps its node's running time averages of q and of the three
-// directional products in registers; nothing of the space-time expansion is
+// The predictor runs one workgroup per cell and three threads per node
+// (z,y,x), one for each direction d.  The time derivatives q_0 .. q_B of the
+// Taylor expansion are formed one after the other: q_i sits in LDS, thread
+// (d, node) takes the node's gradient along d and the PDE's directional
+// non-conservative product B_d grad_d q_i, the three products meet in LDS and
+// the direction-0 thread forms q_{i+1} = -sum_d B_d grad_d q_i.  Thread (d,
+// node) keeps the running time average of its own product in registers (the
+// direction-0 thread also that of q); nothing of the space-time expansion is
 // written to memory unless the caller asks for the reference's temporaries.
+// The face extrapolation reads the time averages straight from LDS.
 #include "aderdg.hpp"
 #include "exa_device.hpp"
 
@@ -34,176 +35,214 @@ __device__ inline int face_line_node(int dir, int a, int b, int l, int B) {
 
 }  // namespace
 
+// Direction groups of the predictor: three groups of node threads (one per
+// direction) up to B = 5; at B = 6 the 768 threads of three groups leave too
+// few registers per thread, and one group handles all three directions.
+__host__ __device__ constexpr int linear_predictor_groups(int B) { return B <= 5 ? 3 : 1; }
+__host__ __device__ constexpr int linear_predictor_threads(int B) {
+  return linear_predictor_groups(B) * node_threads(B);
+}
+
 template <int B, class PDE, bool EXPORT, bool PS>
-__global__ __launch_bounds__(node_threads(B)) void linear_predictor_kernel(LinearPredictorArgs a) {
+__global__ __launch_bounds__(linear_predictor_threads(B)) void linear_predictor_kernel(LinearPredictorArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
   constexpr int B2 = B * B, B3 = B2 * B, B4 = B3 * B;
   constexpr int NT = node_threads(B);
-  __shared__ double sQ[2][V * B3];  // time derivative i in buffer i & 1, [var][node]
-  __shared__ double sE[D * B3];     // extrapolation staging, [node][var]
-  __shared__ double sDu[B2];        // dudx[l][n]
+  constexpr int G = linear_predictor_groups(B);
+  constexpr int NT3 = G * NT;  // threads of the workgroup
+  constexpr int ND = G == 3 ? 1 : 3;  // directions per thread
+  __shared__ double sQ[D * B3];      // q_i [var][node] (rows V.. the parameters); lQhi at the end
+  __shared__ double sF[3][V * B3];   // B_d grad_d q_i [d][var][node]; lFhi_d at the end
+  __shared__ double sDu[B2];         // dudx[l][n]
   const size_t cell = (size_t)blockIdx.x + a.cell0;
-  const int tid = threadIdx.x;
+  const int grp = threadIdx.x / NT;  // the direction this thread differentiates in (wave-uniform)
+  const int tid = threadIdx.x - grp * NT;
   const bool active = tid < B3;
   const int node = active ? tid : 0;
   const int z = node / B2, y = (node / B) % B, x = node % B;
-  for (int e = tid; e < B2; e += NT) sDu[e] = exa_dudx[N][e / B][e % B];
+  for (int e = threadIdx.x; e < B2; e += NT3) sDu[e] = exa_dudx[N][e / B][e % B];
   const double* dxp = a.dx + cell * a.dx_stride;
   const double rdx0 = 1.0 / dxp[0], rdx1 = 1.0 / dxp[1], rdx2 = 1.0 / dxp[2];
   const double dt = *a.dt;
-  const double* luh = a.luh + cell * (size_t)(B3 * D) + node * D;
-  double Q[D];  // the current derivative's variables, then the (constant) parameters
+  double Q[D];  // the node's current derivative, then the (constant) parameters
+  {
+    const double* luh = a.luh + cell * (size_t)(B3 * D) + node * D;
 #pragma unroll
-  for (int m = 0; m < D; ++m) Q[m] = luh[m];
-  double psum[PS ? V : 1];
-  if constexpr (PS) {  // :79-98, sum_n w_n PSi[n+1]; lQi[0] += dt * sum
+    for (int m = 0; m < D; ++m) Q[m] = luh[m];
+  }
+  // sum_n w_n PSi[n+1] of the node's variable m (:79-98, 292-309)
+  auto point_source_sum = [&](int m) {
+    double s = 0.0;
 #pragma unroll
-    for (int m = 0; m < V; ++m) {
-      double s = 0.0;
+    for (int n = 1; n < B + 1; ++n) s = s + exa_gl_weights[N][n - 1] * a.PSi[((size_t)n * B3 + node) * V + m];
+    return s;
+  };
+  if constexpr (PS) {  // lQi[0] += dt * sum
 #pragma unroll
-      for (int n = 1; n < B + 1; ++n) s = s + exa_gl_weights[N][n - 1] * a.PSi[((size_t)n * B3 + node) * V + m];
-      psum[m] = s;
-      Q[m] += dt * s;
+    for (int m = 0; m < V; ++m) Q[m] += dt * point_source_sum(m);
+  }
+  if (grp == 0 && active) {
+#pragma unroll
+    for (int m = 0; m < D; ++m) sQ[m * B3 + node] = Q[m];
+    if (EXPORT) {
+#pragma unroll
+      for (int m = 0; m < D; ++m) a.lQi[(size_t)node * D + m] = Q[m];
     }
   }
-  if (EXPORT && active) {
+  // Time averages lQhi = q_0 + sum_{i>0} fac_i q_i (group 0) and lFhi_d alike
+  // (group d), formed as 0 + 1 q_0 + ..., exact in the first term.
+  double qh[V], fh[ND][V];
 #pragma unroll
-    for (int m = 0; m < D; ++m) a.lQi[(size_t)node * D + m] = Q[m];
+  for (int m = 0; m < V; ++m) {
+    qh[m] = 0.0;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) fh[k][m] = 0.0;
   }
-  double qh[V], fh[3][V];
   double fac = 0.5 * dt;  // Taylor factor dt^i / (i+1)! of the time average, :235-250
 #pragma unroll 1
   for (int i = 0; i < B; ++i) {
-    double* const sq = sQ[i & 1];
-    if (active) {
+    const double c = i == 0 ? 1.0 : fac;
+    __syncthreads();  // q_i complete in sQ
 #pragma unroll
-      for (int m = 0; m < V; ++m) sq[m * B3 + node] = Q[m];
-    }
-    __syncthreads();
-    double qn[V];  // :99-105 (point source) and :205-219
+    for (int m = 0; m < V; ++m) Q[m] = sQ[m * B3 + node];
 #pragma unroll
-    for (int m = 0; m < V; ++m) qn[m] = (PS && i == 0) ? a.PSi[(size_t)node * V + m] : 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      // gradient of q_i in direction d, :118-186: sum_n (1/dx_d q_i) dudx[l][n]
-      double g[V];
+    for (int k = 0; k < ND; ++k) {
+      const int d = G == 3 ? grp : k;
+      // gradient of q_i in direction d, :118-186: sum_n (1/dx_d q_i) dudx[l][n],
+      // formed as 1/dx_d (sum_n q_i dudx[l][n]) (one rounding apart per term)
       const double rdx = d == 0 ? rdx0 : (d == 1 ? rdx1 : rdx2);
       const int l = d == 0 ? x : (d == 1 ? y : z);
       const int base = d == 0 ? (z * B + y) * B : (d == 1 ? z * B2 + x : y * B + x);
       const int stride = d == 0 ? 1 : (d == 1 ? B : B2);
+      double g[V], F[V];
 #pragma unroll
       for (int m = 0; m < V; ++m) {
         double acc = 0.0;
 #pragma unroll
-        for (int n = 0; n < B; ++n) acc += rdx * sq[m * B3 + base + n * stride] * sDu[l * B + n];
-        g[m] = acc;
+        for (int n = 0; n < B; ++n) acc += sQ[m * B3 + base + n * stride] * sDu[l * B + n];
+        g[m] = rdx * acc;
       }
-      double F[V];  // :188-203, B_d grad_d q_i
-      if (d == 0)
+      if (d == 0)  // :188-203, B_d grad_d q_i
         PDE::template ncp_dir<0>(Q, g, F);
       else if (d == 1)
         PDE::template ncp_dir<1>(Q, g, F);
       else
         PDE::template ncp_dir<2>(Q, g, F);
-      if (EXPORT && active) {
-        const size_t o = (((size_t)i * B3 + node) * 3 + d) * V;
+      if (active) {
 #pragma unroll
         for (int m = 0; m < V; ++m) {
-          a.gradQ[o + m] = g[m];
-          a.lFi[o + m] = F[m];
+          fh[k][m] += c * F[m];
+          sF[d][m * B3 + node] = F[m];
+        }
+        if (EXPORT) {
+          const size_t o = (((size_t)i * B3 + node) * 3 + d) * V;
+#pragma unroll
+          for (int m = 0; m < V; ++m) {
+            a.gradQ[o + m] = g[m];
+            a.lFi[o + m] = F[m];
+          }
         }
       }
+    }
+    __syncthreads();  // the three products complete in sF; sQ no longer read
+    if (grp == 0 && active) {
+      // q_{i+1} = init - F_x - F_y - F_z (:99-105 point source, :205-219)
 #pragma unroll
       for (int m = 0; m < V; ++m) {
-        if (i == 0)
-          fh[d][m] = F[m];
-        else
-          fh[d][m] += fac * F[m];
-        qn[m] -= F[m];
+        double qn = (PS && i == 0) ? a.PSi[(size_t)node * V + m] : 0.0;
+        qn -= sF[0][m * B3 + node];
+        qn -= sF[1][m * B3 + node];
+        qn -= sF[2][m * B3 + node];
+        qh[m] += c * Q[m];
+        sQ[m * B3 + node] = qn;
+        if (EXPORT) a.lQi[((size_t)(i + 1) * B3 + node) * D + m] = qn;
+      }
+      if (EXPORT) {  // parameters copied (:221-232)
+#pragma unroll
+        for (int m = V; m < D; ++m) a.lQi[((size_t)(i + 1) * B3 + node) * D + m] = Q[m];
       }
     }
-#pragma unroll
-    for (int m = 0; m < V; ++m) {
-      if (i == 0)
-        qh[m] = Q[m];
-      else
-        qh[m] += fac * Q[m];
-      Q[m] = qn[m];
-    }
     if (i > 0) fac *= dt / (i + 2);
-    if (EXPORT && active) {  // lQi[i+1], parameters copied (:221-232)
-#pragma unroll
-      for (int m = 0; m < D; ++m) a.lQi[((size_t)(i + 1) * B3 + node) * D + m] = Q[m];
-    }
   }
   if constexpr (PS) {  // :292-309
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+    for (int m = 0; m < V; ++m) {
+      const double s3 = point_source_sum(m) / 3;
 #pragma unroll
-      for (int m = 0; m < V; ++m) fh[d][m] += psum[m] / 3;
+      for (int k = 0; k < ND; ++k) fh[k][m] += s3;
+    }
   }
-  if (a.lduh && active) {  // volumeIntegralLinear, lduh = -w^3 sum_d lFhi_d
+  __syncthreads();  // the last q_B written; sF free
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      const int d = G == 3 ? grp : k;
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        sF[d][m * B3 + node] = fh[k][m];
+        if (EXPORT) a.lFhi[((size_t)d * B3 + node) * V + m] = fh[k][m];
+      }
+    }
+    if (grp == 0) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) sQ[m * B3 + node] = qh[m];  // parameter rows unchanged
+    }
+    if (EXPORT) {
+      if (grp == 0) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+          a.lQhi[(size_t)node * D + m] = qh[m];
+          a.lFhi[((size_t)3 * B3 + node) * V + m] = 0.0;
+        }
+#pragma unroll
+        for (int m = V; m < D; ++m) a.lQhi[(size_t)node * D + m] = Q[m];
+      }
+    }
+  }
+  if constexpr (EXPORT) {
+    for (size_t e = threadIdx.x; e < (size_t)B4 * V; e += NT3) a.lFi[(size_t)B4 * 3 * V + e] = 0.0;
+  }
+  __syncthreads();  // lQhi in sQ, lFhi_d in sF[d]
+  if (grp == 0 && active && a.lduh) {  // volumeIntegralLinear, lduh = -w^3 sum_d lFhi_d
     const double w = exa_gl_weights[N][z] * exa_gl_weights[N][y] * exa_gl_weights[N][x];
     double* du = a.lduh + cell * (size_t)(B3 * V) + node * V;
 #pragma unroll
     for (int m = 0; m < V; ++m) {
       double v = 0.0;
-      v -= w * fh[0][m];
-      v -= w * fh[1][m];
-      v -= w * fh[2][m];
+      v -= w * sF[0][m * B3 + node];
+      v -= w * sF[1][m * B3 + node];
+      v -= w * sF[2][m * B3 + node];
       du[m] = v;
     }
   }
-  if constexpr (EXPORT) {
-    if (active) {
+  // Extrapolation to the six faces, :312-436: lQhi (D values) onto every face,
+  // lFhi_d onto faces 2d and 2d+1; one thread per face node, all variables.
+  for (int e = threadIdx.x; e < 6 * B2; e += NT3) {
+    const int face = e / B2, ab = e % B2, dir = face >> 1;
+    const bool right = face & 1;
+    int nd[B];
 #pragma unroll
-      for (int m = 0; m < V; ++m) a.lQhi[(size_t)node * D + m] = qh[m];
+    for (int ll = 0; ll < B; ++ll) nd[ll] = face_line_node(dir, ab / B, ab % B, ll, B);
+    double cf[B];
 #pragma unroll
-      for (int m = V; m < D; ++m) a.lQhi[(size_t)node * D + m] = Q[m];
+    for (int ll = 0; ll < B; ++ll) cf[ll] = right ? exa_fr_coeff[N][ll] : exa_fl_coeff[N][ll];
+    double* const qb = a.lQhbnd + cell * (size_t)(6 * B2 * D) + (size_t)e * D;
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int m = 0; m < V; ++m) a.lFhi[((size_t)d * B3 + node) * V + m] = fh[d][m];
-#pragma unroll
-      for (int m = 0; m < V; ++m) a.lFhi[((size_t)3 * B3 + node) * V + m] = 0.0;
-    }
-    for (size_t e = tid; e < (size_t)B4 * V; e += NT) a.lFi[(size_t)B4 * 3 * V + e] = 0.0;
-  }
-  // Extrapolation to the six faces, :312-436: lQhi (D values), then the x, y
-  // and z time-averaged products onto their own two faces.
-  double* const qb = a.lQhbnd + cell * (size_t)(6 * B2 * D);
-  double* const fb = a.lFhbnd + cell * (size_t)(6 * B2 * V);
-#pragma unroll 1
-  for (int pass = 0; pass < 4; ++pass) {
-    const int W = pass == 0 ? D : V;
-    __syncthreads();
-    if (active) {
-      if (pass == 0) {
-#pragma unroll
-        for (int m = 0; m < V; ++m) sE[node * D + m] = qh[m];
-#pragma unroll
-        for (int m = V; m < D; ++m) sE[node * D + m] = Q[m];
-      } else {
-#pragma unroll
-        for (int m = 0; m < V; ++m)
-          sE[node * V + m] = pass == 1 ? fh[0][m] : (pass == 2 ? fh[1][m] : fh[2][m]);
-      }
-    }
-    __syncthreads();
-    const int nfaces = pass == 0 ? 6 : 2;
-    const int face0 = pass == 0 ? 0 : 2 * (pass - 1);
-    double* const out = pass == 0 ? qb : fb + (size_t)face0 * B2 * V;
-    for (int e = tid; e < nfaces * B2 * W; e += NT) {
-      const int face = face0 + e / (B2 * W);
-      const int r = e % (B2 * W), ab = r / W, m = r % W;
-      const int dir = face >> 1;
-      const bool right = face & 1;
+    for (int m = 0; m < D; ++m) {
       double acc = 0.0;
 #pragma unroll
-      for (int l = 0; l < B; ++l)
-        acc += sE[face_line_node(dir, ab / B, ab % B, l, B) * W + m] * (right ? exa_fr_coeff[N][l] : exa_fl_coeff[N][l]);
-      out[e] = acc;
+      for (int ll = 0; ll < B; ++ll) acc += sQ[m * B3 + nd[ll]] * cf[ll];
+      qb[m] = acc;
+    }
+    const double* const sf = sF[dir];
+    double* const fb = a.lFhbnd + cell * (size_t)(6 * B2 * V) + (size_t)e * V;
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      double acc = 0.0;
+#pragma unroll
+      for (int ll = 0; ll < B; ++ll) acc += sf[m * B3 + nd[ll]] * cf[ll];
+      fb[m] = acc;
     }
   }
 }
@@ -391,7 +430,7 @@ hipError_t launch_linear_predictor(const SolverDesc& sd, bool export_temps, cons
   if ((export_temps || a.PSi) && ncells != 1) return hipErrorInvalidValue;  // one cell's temporaries / sources
   if (export_temps && (!a.lQi || !a.lFi || !a.gradQ || !a.lQhi || !a.lFhi)) return hipErrorInvalidValue;
   return dispatch_linear(sd, [&]<int B, class PDE>() -> hipError_t {
-    const dim3 grid((unsigned)ncells), block(node_threads(B));
+    const dim3 grid((unsigned)ncells), block(linear_predictor_threads(B));
     if (export_temps) {
       if (a.PSi)
         hipLaunchKernelGGL((linear_predictor_kernel<B, PDE, true, true>), grid, block, 0, s, a);

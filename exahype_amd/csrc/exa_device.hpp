@@ -192,20 +192,13 @@ struct Euler {
     for (int m = 0; m < V; ++m) BgradQ[m] = T(0);
   }
 
-  // Linear-kernel interface: B_d grad_d Q of one direction and the
-  // coefficient matrix; the test solver's are zero
-  // (GenericEulerKernelTest3D.cpp:108-132).
+  // Linear-kernel interface: B_d grad_d Q of one direction (the linear
+  // kernels' NCP, and B_d q = Bn^T q for the Riemann solver's coefficient
+  // matrix); the test solver's are zero (GenericEulerKernelTest3D.cpp:108-132).
   template <int DIR, typename T>
   __device__ static inline void ncp_dir(const T*, const T*, T* out) {
 #pragma unroll
     for (int m = 0; m < V; ++m) out[m] = T(0);
-  }
-  template <typename T>
-  __device__ static inline void coefficient_matrix(const T*, int, T (*Bn)[V]) {
-#pragma unroll
-    for (int l = 0; l < V; ++l)
-#pragma unroll
-      for (int k = 0; k < V; ++k) Bn[l][k] = T(0);
   }
 };
 
@@ -257,7 +250,8 @@ struct Elastic {
   // Linear-kernel interface (the linear kernels take B_d grad_d Q per
   // direction, spaceTimePredictorLinear.cpph:32): one direction's block of
   // the product above (ElasticityKernelTest2D.cpp:130-181, the z block being
-  // the comment there).
+  // the comment there).  B_d q is also Bn^T q for the coefficient matrix
+  // Bn[l][k] = B_d(k, l) of ElasticityKernelTest2D.cpp:183-228.
   template <int DIR, typename T>
   __device__ static inline void ncp_dir(const T* Q, const T* g, T* out) {
     const T lam = Q[9], mu = Q[10], irho = T(1) / Q[11];
@@ -292,46 +286,6 @@ struct Elastic {
       out[6] = -irho * g[5];
       out[7] = -irho * g[4];
       out[8] = -irho * g[2];
-    }
-  }
-
-  // coefficientMatrix, Bn[l][k] = B_d(k, l) (ElasticityKernelTest2D.cpp:
-  // 183-228, the z case being the comment there).
-  template <typename T>
-  __device__ static inline void coefficient_matrix(const T* Q, int d, T (*Bn)[V]) {
-#pragma unroll
-    for (int l = 0; l < V; ++l)
-#pragma unroll
-      for (int k = 0; k < V; ++k) Bn[l][k] = T(0);
-    const T lam = Q[9], mu = Q[10], irho = T(1) / Q[11];
-    const T l2m = lam + T(2) * mu;
-    if (d == 0) {
-      Bn[6][0] = -l2m;
-      Bn[6][1] = -lam;
-      Bn[6][2] = -lam;
-      Bn[7][3] = -mu;
-      Bn[8][5] = -mu;
-      Bn[0][6] = -irho;
-      Bn[3][7] = -irho;
-      Bn[5][8] = -irho;
-    } else if (d == 1) {
-      Bn[7][0] = -lam;
-      Bn[7][1] = -l2m;
-      Bn[7][2] = -lam;
-      Bn[6][3] = -mu;
-      Bn[8][4] = -mu;
-      Bn[3][6] = -irho;
-      Bn[1][7] = -irho;
-      Bn[4][8] = -irho;
-    } else {
-      Bn[8][0] = -lam;
-      Bn[8][1] = -lam;
-      Bn[8][2] = -l2m;
-      Bn[7][4] = -mu;
-      Bn[6][5] = -mu;
-      Bn[5][6] = -irho;
-      Bn[4][7] = -irho;
-      Bn[2][8] = -irho;
     }
   }
 };

@@ -247,11 +247,15 @@ __global__ __launch_bounds__(linear_predictor_threads(B)) void linear_predictor_
   }
 }
 
-// One face per wavefront.  The face averages of both states give the
-// dissipation speed smax and the coefficient matrix of the averaged state;
-// the fluctuations are FL = 0.5 (Bn - smax I)^T (QR - QL) into the left
-// cell's face 2d+1 and FR = 0.5 (Bn + smax I)^T (QR - QL) into the right
-// cell's face 2d.
+// One face per wavefront, one face node per lane.  The face averages of both
+// states give the dissipation speed smax and the averaged state Qav; the
+// fluctuations are FL = 0.5 (Bn - smax I)^T (QR - QL) into the left cell's
+// face 2d+1 and FR = 0.5 (Bn + smax I)^T (QR - QL) into the right cell's face
+// 2d, Bn[l][k] = B_d(k, l) the coefficient matrix at Qav (:80-128).  Bn^T q is
+// B_d q, which is what the PDE's directional non-conservative product
+// computes, so each lane forms ncp_dir<d>(Qav, dq) -+ smax dq for its node
+// with dq = 0.5 (QR - QL); the zero entries of Bn the reference multiplies
+// through add nothing.
 template <int B, class PDE>
 __global__ __launch_bounds__(256) void linear_riemann_kernel(RiemannArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
@@ -259,11 +263,7 @@ __global__ __launch_bounds__(256) void linear_riemann_kernel(RiemannArgs a) {
   constexpr int B2 = B * B;
   constexpr int WPB = 256 / kWave;
   static_assert(B2 <= kWave, "one face node per lane");
-  static_assert(D <= 32, "two face averages per wavefront");
-  __shared__ double sQL[WPB][B2 * D], sQR[WPB][B2 * D];
-  __shared__ double sFL[WPB][B2 * V], sFR[WPB][B2 * V];
-  __shared__ double sAv[WPB][2 * D];
-  __shared__ double sBm[WPB][V * V], sBp[WPB][V * V];
+  __shared__ double sStage[WPB][2 * B2 * D];  // QL | QR, then FL | FR
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const long f = (long)blockIdx.x * WPB + w;
   if (f >= a.nfaces) return;  // whole wavefront exits together
@@ -273,62 +273,51 @@ __global__ __launch_bounds__(256) void linear_riemann_kernel(RiemannArgs a) {
   double* FR = static_cast<double*>(a.lFhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
   const double* QL = static_cast<const double*>(a.lQhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * D);
   const double* QR = static_cast<const double*>(a.lQhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * D);
-  for (int e = lane; e < B2 * D; e += kWave) {
-    sQL[w][e] = QL[e];
-    sQR[w][e] = QR[e];
+  double* const sQL = sStage[w];
+  double* const sQR = sQL + B2 * D;
+  for (int e = lane; e < B2 * D; e += kWave) {  // contiguous, coalesced loads
+    sQL[e] = QL[e];
+    sQR[e] = QR[e];
   }
   wave_lds_sync();
-  // face averages, :40-66 (lanes 0..D-1 left, 32..32+D-1 right)
-  const int k = lane & 31;
-  if (k < D) {
-    const double* src = lane < 32 ? sQL[w] : sQR[w];
-    double acc = 0.0;
+  const bool active = lane < B2;
+  const int node = active ? lane : 0;
+  const double weight = active ? exa_gl_weights[N][node / B] * exa_gl_weights[N][node % B] : 0.0;
+  double qL[D], qR[D], QavL[D], QavR[D], Qav[D];
 #pragma unroll
-    for (int i = 0; i < B; ++i)
-#pragma unroll
-      for (int j = 0; j < B; ++j) acc += exa_gl_weights[N][i] * exa_gl_weights[N][j] * src[(i * B + j) * D + k];
-    sAv[w][(lane < 32 ? 0 : D) + k] = acc;
-  }
-  wave_lds_sync();
-  double QavL[D], QavR[D], Qav[D];
-#pragma unroll
-  for (int m = 0; m < D; ++m) {
-    QavL[m] = sAv[w][m];
-    QavR[m] = sAv[w][D + m];
-    Qav[m] = 0.5 * (QavL[m] + QavR[m]);
+  for (int k = 0; k < D; ++k) {  // :40-66
+    qL[k] = sQL[node * D + k];
+    qR[k] = sQR[node * D + k];
+    QavL[k] = wave_sum(weight * qL[k]);
+    QavR[k] = wave_sum(weight * qR[k]);
+    Qav[k] = 0.5 * (QavL[k] + QavR[k]);
   }
   const double sL = PDE::max_abs_eigenvalue(QavL, d);  // :68-78
   const double sR = PDE::max_abs_eigenvalue(QavR, d);
   const double smax = (sL < sR) ? sR : sL;
-  if (lane == 0) PDE::coefficient_matrix(Qav, d, reinterpret_cast<double(*)[V]>(sBm[w]));  // :80-99
-  wave_lds_sync();
-  for (int e = lane; e < V * V; e += kWave) {
-    const double bn = sBm[w][e];
-    const bool diag = e / V == e % V;
-    sBp[w][e] = diag ? bn + smax : bn;
-    sBm[w][e] = diag ? bn - smax : bn;
-  }
-  wave_lds_sync();
-  if (lane < B2) {  // :101-128
-    double q[V];
+  double dq[V], F[V];
 #pragma unroll
-    for (int l = 0; l < V; ++l) q[l] = 0.5 * (sQR[w][lane * D + l] - sQL[w][lane * D + l]);
+  for (int k = 0; k < V; ++k) dq[k] = 0.5 * (qR[k] - qL[k]);
+  if (d == 0)
+    PDE::template ncp_dir<0>(Qav, dq, F);
+  else if (d == 1)
+    PDE::template ncp_dir<1>(Qav, dq, F);
+  else
+    PDE::template ncp_dir<2>(Qav, dq, F);
+  wave_lds_sync();  // every lane has read its states: the stage takes the fluctuations
+  double* const sFL = sStage[w];
+  double* const sFR = sFL + B2 * V;
+  if (active) {
 #pragma unroll
-    for (int kk = 0; kk < V; ++kk) {
-      double fl = 0.0, fr = 0.0;
-#pragma unroll
-      for (int l = 0; l < V; ++l) {
-        fl += sBm[w][l * V + kk] * q[l];
-        fr += sBp[w][l * V + kk] * q[l];
-      }
-      sFL[w][lane * V + kk] = fl;
-      sFR[w][lane * V + kk] = fr;
+    for (int k = 0; k < V; ++k) {
+      sFL[node * V + k] = F[k] - smax * dq[k];
+      sFR[node * V + k] = F[k] + smax * dq[k];
     }
   }
   wave_lds_sync();
   for (int e = lane; e < B2 * V; e += kWave) {
-    FL[e] = sFL[w][e];
-    FR[e] = sFR[w][e];
+    FL[e] = sFL[e];
+    FR[e] = sFR[e];
   }
 }
 

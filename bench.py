@@ -226,6 +226,116 @@ def cpu_baseline_fv(L, seconds):
                        f"Euler state, 1 thread, {el:.1f} s")
 
 
+AMR_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "amr_volume_n5_27to81_pmc_traffic.txt")
+
+
+def amr_measured_traffic(kind):
+    """HBM bytes per AMR transfer launch (rocprofv3 --pmc, copy-calibrated) as
+    recorded for the 27^3 -> 81^3 workload; None otherwise."""
+    if not os.path.exists(AMR_TRAFFIC_FILE):
+        return None
+    key = f"traffic_bytes_per_launch amr_{kind} 27^3->81^3 N=5"
+    for line in open(AMR_TRAFFIC_FILE):
+        if line.startswith(key + " "):
+            return int(line.split()[-1])
+    return None
+
+
+def amr_items(nc, r):
+    """Transfers of an nc^3 coarse grid refined r times per direction: rows
+    (fine block, coarse block, sx, sy, sz), coarse-major, and the CSR pointer
+    of each coarse block's children."""
+    nf = nc * r
+    c = np.arange(nc ** 3)
+    ci, cj, ck = c % nc, (c // nc) % nc, c // (nc * nc)
+    s = np.arange(r ** 3)
+    sx, sy, sz = s % r, (s // r) % r, s // (r * r)
+    fi = (ci[:, None] * r + sx[None, :])
+    fj = (cj[:, None] * r + sy[None, :])
+    fk = (ck[:, None] * r + sz[None, :])
+    items = np.stack([fi + nf * (fj + nf * fk), np.repeat(c[:, None], r ** 3, 1),
+                      np.broadcast_to(sx, fi.shape), np.broadcast_to(sy, fi.shape), np.broadcast_to(sz, fi.shape)],
+                     -1).reshape(-1, 5).astype(np.int32)
+    return items, np.arange(0, nc ** 3 * r ** 3 + 1, r ** 3, dtype=np.int32)
+
+
+def cpu_baseline_amr(seconds):
+    """The oracle's volume prolongation + restriction (single calls, as the
+    reference's grid traversal makes them), one thread, N=5, 5 variables."""
+    from oracle import oracle as O
+    s = O.make_solver(5)
+    rng = np.random.default_rng(1)
+    uc, uf = rng.standard_normal(216 * 5), rng.standard_normal(216 * 5)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        sub = (n % 3, (n // 3) % 3, (n // 9) % 3)
+        O.volume_unknowns_prolongation(s, uc, 0, 1, sub)
+        uc = O.volume_unknowns_restriction(s, uc, uf, 0, 1, sub)
+        n += 1
+    el = time.perf_counter() - t0
+    return dict(value=2 * n * 216 * 5 / el, unit="DOF-transfers/s", cores=1, kind="port",
+                sample=f"{n} volume prolongations + {n} restrictions (oracle single calls, N=5, 5 variables, one "
+                       f"level), 1 thread, {el:.1f} s")
+
+
+def bench_amr(capi, args):
+    """AMR workload: every cell of a 27^3 Euler grid at N=5 refined once (81^3
+    fine cells = 531,441 volume prolongations), then all fine cells restricted
+    back (27 children added per coarse cell), FP64, device-resident blocks."""
+    nc, r, B, D = 27, 3, 6, 5
+    s = capi.euler_solver(5)
+    rng = np.random.default_rng(args.seed)
+    items, ptr = amr_items(nc, r)
+    coarse = capi.DeviceArray(rng.standard_normal((nc ** 3, B ** 3 * D)))
+    fine = capi.DeviceArray(np.zeros((len(items), B ** 3 * D)))
+    d_items, d_ptr = capi.DeviceArray(items), capi.DeviceArray(ptr)
+    d_ritems = capi.DeviceArray(items[:, [1, 0, 2, 3, 4]])
+    acc = capi.DeviceArray(np.zeros((nc ** 3, B ** 3 * D)))
+    for _ in range(args.warmup):
+        capi.volume_prolongation_batch(s, fine, coarse, d_items, 1)
+        capi.volume_restriction_batch(s, acc, fine, d_ptr, d_ritems, 1)
+    tp, tr = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tp.append(capi.volume_prolongation_batch(s, fine, coarse, d_items, 1))
+        tr.append(capi.volume_restriction_batch(s, acc, fine, d_ptr, d_ritems, 1))
+    wall = time.perf_counter() - t0
+    # parity on a sample: restriction of a prolongated field returns the coarse data
+    chk = capi.DeviceArray(np.zeros((nc ** 3, B ** 3 * D)))
+    capi.volume_restriction_batch(s, chk, fine, d_ptr, d_ritems, 1)
+    err = float(np.abs(chk.get() - coarse.get()).max())
+    nf = len(items)
+    dof = nf * B ** 3 * D
+    p_ms, r_ms = float(np.mean(tp)), float(np.mean(tr))
+    blk = B ** 3 * D * 8
+    alg_p = nf * blk + nc ** 3 * blk          # write every fine block, read every coarse block
+    alg_r = nf * blk + 2 * nc ** 3 * blk      # read every fine block, read and write every coarse block
+    achieved = alg_p / (p_ms * 1e-3) / 1e9
+    out = {
+        "metric": "DOF-transfers/s, 3D ADER-DG AMR volume prolongation + restriction, 27^3 -> 81^3 cells, N=5, FP64",
+        "value": 2 * dof * args.steps / wall, "unit": "DOF-transfers/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded normal coarse data, generated on the host, resident on the device)",
+        "config": {"workload": "amr volume transfers: 27^3 coarse cells, one level (81^3 = 531441 fine cells), N=5, "
+                               "5 variables; a step = all prolongations + all restrictions",
+                   "transfers_per_step": 2 * nf, "seed": args.seed},
+        "kernel_ms": {"prolongation": round(p_ms, 4), "restriction": round(r_ms, 4)},
+        "restrict_after_prolong_max_err": err,
+        "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=amr_measured_traffic("prolongation"),
+                         kernel="amr_kernel<6,3,false> (volume prolongation, one level)",
+                         avg_launch_ms=round(p_ms, 4), algorithmic_bytes_per_launch=alg_p,
+                         restriction={"avg_launch_ms": round(r_ms, 4), "algorithmic_bytes_per_launch": alg_r,
+                                      "achieved_GBs": round(alg_r / (r_ms * 1e-3) / 1e9, 1),
+                                      "traffic": amr_measured_traffic("restriction")}),
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_amr(args.cpu_seconds),
+    }
+    print(json.dumps(out), flush=True)
+    for a in (coarse, fine, d_items, d_ptr, d_ritems, acc, chk):
+        a.close()
+
+
 def bench_fv(capi, args):
     """Finite-volumes workload: the EulerFV demonstrator's Godunov solver in
     3-D, 54^3 cells of 10^3 subcells (540^3), periodic entropy wave + 8
@@ -434,12 +544,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fv-cells", type=int, default=54, help="fv: cells per direction")
     ap.add_argument("--patch-size", type=int, default=10, help="fv: subcells per cell and direction")
-    ap.add_argument("--workload", choices=("precision", "limiter", "batch", "elastic", "fv"), default="precision",
+    ap.add_argument("--workload", choices=("precision", "limiter", "batch", "elastic", "fv", "amr"),
+                    default="precision",
                     help="precision: Euler N=5 81^3, FP32-bulk vs FP64 (the headline line); "
                          "limiter: Euler blast N=3 81^3 with the subcell limiter, FP64; "
                          "batch: 96 seeded uneven patches (orders 2..7), FP32-bulk vs FP64; "
                          "elastic: linear elasticity N=4 81^3, FP32-bulk vs FP64; "
-                         "fv: Godunov finite volumes, 54^3 cells of 10^3 subcells, FP64")
+                         "fv: Godunov finite volumes, 54^3 cells of 10^3 subcells, FP64; "
+                         "amr: volume prolongation + restriction, 27^3 -> 81^3 cells, N=5, FP64")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="batch: GB of patch state per wave")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times): the known-size "
@@ -464,6 +576,9 @@ def main():
         return
     if args.form == "linear" and args.workload != "elastic":
         sys.exit("--form linear applies to the elastic workload only")
+    if args.workload == "amr":
+        bench_amr(capi, args)
+        return
     if args.workload == "elastic":
         bench_elastic(capi, args)
         return

@@ -46,6 +46,10 @@ EXPORTED = (
     "exa_fvgrid_synchronize", "exa_fvgrid_time", "exa_fvgrid_set_profiling", "exa_fvgrid_stage_times",
     "exa_space_time_predictor_linear", "exa_volume_integral_linear", "exa_surface_integral_linear",
     "exa_riemann_solver_linear",
+    "exa_face_unknowns_prolongation", "exa_face_unknowns_restriction", "exa_volume_unknowns_prolongation",
+    "exa_volume_unknowns_restriction", "exa_device_alloc", "exa_device_free", "exa_copy_to_device",
+    "exa_copy_to_host", "exa_volume_unknowns_prolongation_batch", "exa_volume_unknowns_restriction_batch",
+    "exa_face_unknowns_prolongation_batch", "exa_face_unknowns_restriction_batch",
 )
 
 
@@ -119,6 +123,21 @@ def lib():
     L.exa_volume_integral_linear.argtypes = [S, _D, _D, _D]
     L.exa_surface_integral_linear.argtypes = [S, _D, _D, _D]
     L.exa_riemann_solver_linear.argtypes = [S, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
+    I, Lg, V = ctypes.c_int, ctypes.c_long, ctypes.c_void_p
+    F = ctypes.POINTER(ctypes.c_float)
+    L.exa_face_unknowns_prolongation.argtypes = [S, _D, _D, _D, _D, I, I, _I]
+    L.exa_face_unknowns_restriction.argtypes = [S, _D, _D, _D, _D, I, I, _I]
+    L.exa_volume_unknowns_prolongation.argtypes = [S, _D, _D, I, I, _I]
+    L.exa_volume_unknowns_restriction.argtypes = [S, _D, _D, I, I, _I]
+    L.exa_device_alloc.argtypes = [ctypes.c_size_t]
+    L.exa_device_alloc.restype = V
+    L.exa_device_free.argtypes = [V]
+    L.exa_copy_to_device.argtypes = [V, V, ctypes.c_size_t]
+    L.exa_copy_to_host.argtypes = [V, V, ctypes.c_size_t]
+    L.exa_volume_unknowns_prolongation_batch.argtypes = [S, V, Lg, V, Lg, V, Lg, I, F]
+    L.exa_volume_unknowns_restriction_batch.argtypes = [S, V, Lg, V, Lg, V, Lg, V, Lg, I, F]
+    L.exa_face_unknowns_prolongation_batch.argtypes = [S, V, V, Lg, V, V, Lg, V, Lg, I, F]
+    L.exa_face_unknowns_restriction_batch.argtypes = [S, V, V, Lg, V, V, Lg, V, Lg, V, Lg, I, F]
     L.exa_stable_time_step_size.argtypes = [S, _D, _D, _D]
     L.exa_batch_predictor_volume.argtypes = [S, ctypes.c_long, _D, _D, _D, _D, _D, ctypes.c_double, _I]
     L.exa_grid_create.argtypes = [S, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int,
@@ -302,6 +321,111 @@ def riemann_solver_linear(s, QL, QR, dt, direction):
     check(lib().exa_riemann_solver_linear(ctypes.byref(s), _dp(FL), _dp(FR), _dp(_f64(QL)), _dp(_f64(QR)),
                                           float(dt), int(direction)), "riemannSolverLinear")
     return FL, FR
+
+
+# ---------------------------------------------------------------------------
+# AMR transfer operators (amrRoutines.cpph)
+# ---------------------------------------------------------------------------
+def _sub(v, n):
+    return (ctypes.c_int * n)(*[int(x) for x in v])
+
+
+def face_unknowns_prolongation(s, QC, FC, coarse_level, fine_level, subface):
+    """faceUnknownsProlongation: returns (lQhbndFine, lFhbndFine)."""
+    B, D, V = s.B, s.D, s.num_variables
+    QF, FF = np.zeros(B * B * D), np.zeros(B * B * V)
+    check(lib().exa_face_unknowns_prolongation(ctypes.byref(s), _dp(QF), _dp(FF), _dp(_f64(QC)), _dp(_f64(FC)),
+                                               coarse_level, fine_level, _sub(subface, 2)),
+          "faceUnknownsProlongation")
+    return QF, FF
+
+
+def face_unknowns_restriction(s, QC, FC, QF, FF, coarse_level, fine_level, subface):
+    """faceUnknownsRestriction: returns the coarse arrays with the restricted fine values added."""
+    QC, FC = np.array(QC, dtype=np.float64), np.array(FC, dtype=np.float64)
+    check(lib().exa_face_unknowns_restriction(ctypes.byref(s), _dp(QC), _dp(FC), _dp(_f64(QF)), _dp(_f64(FF)),
+                                              coarse_level, fine_level, _sub(subface, 2)),
+          "faceUnknownsRestriction")
+    return QC, FC
+
+
+def volume_unknowns_prolongation(s, uC, coarse_level, fine_level, subcell, uF=None):
+    uF = np.zeros(s.B ** 3 * s.D) if uF is None else np.array(uF, dtype=np.float64)
+    check(lib().exa_volume_unknowns_prolongation(ctypes.byref(s), _dp(uF), _dp(_f64(uC)), coarse_level, fine_level,
+                                                 _sub(subcell, 3)), "volumeUnknownsProlongation")
+    return uF
+
+
+def volume_unknowns_restriction(s, uC, uF, coarse_level, fine_level, subcell):
+    uC = np.array(uC, dtype=np.float64)
+    check(lib().exa_volume_unknowns_restriction(ctypes.byref(s), _dp(uC), _dp(_f64(uF)), coarse_level, fine_level,
+                                                _sub(subcell, 3)), "volumeUnknownsRestriction")
+    return uC
+
+
+class DeviceArray:
+    """A device buffer (exa_device_alloc) holding a numpy array's bytes."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a)
+        self.dtype, self.shape, self.nbytes = a.dtype, a.shape, a.nbytes
+        self.ptr = lib().exa_device_alloc(max(a.nbytes, 8))
+        if not self.ptr:
+            raise ExaError("exa_device_alloc failed")
+        check(lib().exa_copy_to_device(self.ptr, a.ctypes.data, a.nbytes), "exa_copy_to_device")
+
+    def get(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().exa_copy_to_host(out.ctypes.data, self.ptr, self.nbytes), "exa_copy_to_host")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().exa_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ms(timed):
+    return ctypes.byref(timed) if timed is not None else None
+
+
+def volume_prolongation_batch(s, fine, coarse, items, level_delta):
+    """fine, coarse: DeviceArray of blocks; items: DeviceArray int32 [n, 5].  Returns the kernel time (ms)."""
+    t = ctypes.c_float(0.0)
+    check(lib().exa_volume_unknowns_prolongation_batch(
+        ctypes.byref(s), fine.ptr, fine.shape[0], coarse.ptr, coarse.shape[0], items.ptr, items.shape[0],
+        level_delta, _ms(t)), "volumeUnknownsProlongation (batch)")
+    return t.value
+
+
+def volume_restriction_batch(s, coarse, fine, ptr, items, level_delta):
+    t = ctypes.c_float(0.0)
+    check(lib().exa_volume_unknowns_restriction_batch(
+        ctypes.byref(s), coarse.ptr, coarse.shape[0], fine.ptr, fine.shape[0], ptr.ptr, ptr.shape[0] - 1, items.ptr,
+        items.shape[0], level_delta, _ms(t)), "volumeUnknownsRestriction (batch)")
+    return t.value
+
+
+def face_prolongation_batch(s, QF, FF, QC, FC, items, level_delta):
+    t = ctypes.c_float(0.0)
+    check(lib().exa_face_unknowns_prolongation_batch(
+        ctypes.byref(s), QF.ptr, FF.ptr, QF.shape[0], QC.ptr, FC.ptr, QC.shape[0], items.ptr, items.shape[0],
+        level_delta, _ms(t)), "faceUnknownsProlongation (batch)")
+    return t.value
+
+
+def face_restriction_batch(s, QC, FC, QF, FF, ptr, items, level_delta):
+    t = ctypes.c_float(0.0)
+    check(lib().exa_face_unknowns_restriction_batch(
+        ctypes.byref(s), QC.ptr, FC.ptr, QC.shape[0], QF.ptr, FF.ptr, QF.shape[0], ptr.ptr, ptr.shape[0] - 1,
+        items.ptr, items.shape[0], level_delta, _ms(t)), "faceUnknownsRestriction (batch)")
+    return t.value
 
 
 def batch_predictor_volume(s, luh, dx, dt):

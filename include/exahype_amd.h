@@ -22,6 +22,8 @@
 #ifndef EXAHYPE_AMD_H
 #define EXAHYPE_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -126,6 +128,63 @@ int exa_surface_integral_linear(const exa_solver* s, double* lduh, const double*
  * (fluctuations); dt is unused, as in the reference. */
 int exa_riemann_solver_linear(const exa_solver* s, double* FL, double* FR, const double* QL,
                               const double* QR, double dt, int direction);
+
+/* ---- AMR transfer operators (FP64; K/c/3d/amrRoutines.cpph, Kernels.h:163-199).
+ * Blocks: volumes luh (z,y,x,var) B^3 (V+P); faces lQhbnd (b,a,var) B^2 (V+P) and
+ * lFhbnd B^2 V of one face.  subfaceIndex[2] / subcellIndex[3] in [0, 3^(fine -
+ * coarse level)) per direction (faces: the face's two in-plane directions in
+ * storage order; volumes: x, y, z).  Prolongation overwrites the fine block
+ * (fine = coarse level: no change, as in the reference); restriction adds the
+ * restricted fine block to the coarse block and needs fine > coarse level.
+ * Orders 1..5; level differences up to 6. */
+
+/* faceUnknownsProlongation (amrRoutines.cpph:48). */
+int exa_face_unknowns_prolongation(const exa_solver* s, double* lQhbndFine, double* lFhbndFine,
+                                   const double* lQhbndCoarse, const double* lFhbndCoarse,
+                                   int coarseGridLevel, int fineGridLevel, const int* subfaceIndex);
+/* faceUnknownsRestriction (amrRoutines.cpph:183). */
+int exa_face_unknowns_restriction(const exa_solver* s, double* lQhbndCoarse, double* lFhbndCoarse,
+                                  const double* lQhbndFine, const double* lFhbndFine,
+                                  int coarseGridLevel, int fineGridLevel, const int* subfaceIndex);
+/* volumeUnknownsProlongation (amrRoutines.cpph:313). */
+int exa_volume_unknowns_prolongation(const exa_solver* s, double* luhFine, const double* luhCoarse,
+                                     int coarseGridLevel, int fineGridLevel, const int* subcellIndex);
+/* volumeUnknownsRestriction (amrRoutines.cpph:420). */
+int exa_volume_unknowns_restriction(const exa_solver* s, double* luhCoarse, const double* luhFine,
+                                    int coarseGridLevel, int fineGridLevel, const int* subcellIndex);
+
+/* Device buffers for the batched entry points (hipMalloc / hipFree / copies). */
+void* exa_device_alloc(size_t bytes);
+int exa_device_free(void* p);
+int exa_copy_to_device(void* dst, const void* src, size_t bytes);
+int exa_copy_to_host(void* dst, const void* src, size_t bytes);
+
+/* Batched transfers on device arrays of blocks (all pointers from
+ * exa_device_alloc; no counterpart in the reference, which calls the kernels
+ * above once per cell or face pair from its grid traversal).  items: device
+ * array of 5 ints per transfer: destination block, source block, subinterval
+ * index per direction (faces use two; face blocks are numbered cell * 6 + face
+ * in lQhbnd/lFhbnd layouts).  Prolongation: nitems transfers, each destination
+ * overwritten.  Restriction: ndst destinations; ptr (device, ndst + 1 ints)
+ * gives each destination's items [ptr[k], ptr[k+1]) of the nitems, all with
+ * that destination block; their restricted blocks are added to it in list
+ * order, as successive single calls would.  The calls block until done; items
+ * with an index out of range are skipped and the call returns EXA_E_INVALID.
+ * elapsed_ms (optional): the kernels' time from HIP events on their stream. */
+int exa_volume_unknowns_prolongation_batch(const exa_solver* s, double* luhFine, long nfine,
+                                           const double* luhCoarse, long ncoarse, const int* items,
+                                           long nitems, int levelDelta, float* elapsed_ms);
+int exa_volume_unknowns_restriction_batch(const exa_solver* s, double* luhCoarse, long ncoarse,
+                                          const double* luhFine, long nfine, const int* ptr, long ndst,
+                                          const int* items, long nitems, int levelDelta, float* elapsed_ms);
+int exa_face_unknowns_prolongation_batch(const exa_solver* s, double* lQhbndFine, double* lFhbndFine,
+                                         long nfine_faces, const double* lQhbndCoarse,
+                                         const double* lFhbndCoarse, long ncoarse_faces, const int* items,
+                                         long nitems, int levelDelta, float* elapsed_ms);
+int exa_face_unknowns_restriction_batch(const exa_solver* s, double* lQhbndCoarse, double* lFhbndCoarse,
+                                        long ncoarse_faces, const double* lQhbndFine, const double* lFhbndFine,
+                                        long nfine_faces, const int* ptr, long ndst, const int* items,
+                                        long nitems, int levelDelta, float* elapsed_ms);
 
 /* solutionUpdate (K/c/3d/solutionUpdate.cpph:23); luh in/out. */
 int exa_solution_update(const exa_solver* s, double* luh, const double* lduh, double dt);

@@ -197,6 +197,52 @@ void riemannSolverLinear(SolverType& /*solver*/, double* FL, double* FR, const d
   exahype_amd::check(exa_riemann_solver_linear(&s, FL, FR, QL, QR, dt, normalNonZero), "riemannSolverLinear");
 }
 
+// ---- AMR transfer operators (Kernels.h:163-199, amrRoutines.cpph) ----
+
+template <int numberOfVariables, int numberOfParameters, int basisSize, typename FaceIndex>
+void faceUnknownsProlongation(double* lQhbndFine, double* lFhbndFine, const double* lQhbndCoarse,
+                              const double* lFhbndCoarse, const int coarseGridLevel, const int fineGridLevel,
+                              const FaceIndex& subfaceIndex) {
+  exa_solver s = exahype_amd::size_descriptor(numberOfVariables, basisSize, EXA_TERM_FLUX, false);
+  s.num_parameters = numberOfParameters;
+  const int sub[2] = {subfaceIndex[0], subfaceIndex[1]};
+  exahype_amd::check(exa_face_unknowns_prolongation(&s, lQhbndFine, lFhbndFine, lQhbndCoarse, lFhbndCoarse,
+                                                    coarseGridLevel, fineGridLevel, sub),
+                     "faceUnknownsProlongation");
+}
+
+template <int numberOfVariables, int numberOfParameters, int basisSize, typename FaceIndex>
+void faceUnknownsRestriction(double* lQhbndCoarse, double* lFhbndCoarse, const double* lQhbndFine,
+                             const double* lFhbndFine, const int coarseGridLevel, const int fineGridLevel,
+                             const FaceIndex& subfaceIndex) {
+  exa_solver s = exahype_amd::size_descriptor(numberOfVariables, basisSize, EXA_TERM_FLUX, false);
+  s.num_parameters = numberOfParameters;
+  const int sub[2] = {subfaceIndex[0], subfaceIndex[1]};
+  exahype_amd::check(exa_face_unknowns_restriction(&s, lQhbndCoarse, lFhbndCoarse, lQhbndFine, lFhbndFine,
+                                                   coarseGridLevel, fineGridLevel, sub),
+                     "faceUnknownsRestriction");
+}
+
+template <int numberOfVariables, int numberOfParameters, int basisSize, typename CellIndex>
+void volumeUnknownsProlongation(double* luhFine, const double* luhCoarse, const int coarseGridLevel,
+                                const int fineGridLevel, const CellIndex& subcellIndex) {
+  exa_solver s = exahype_amd::size_descriptor(numberOfVariables, basisSize, EXA_TERM_FLUX, false);
+  s.num_parameters = numberOfParameters;
+  const int sub[3] = {subcellIndex[0], subcellIndex[1], subcellIndex[2]};
+  exahype_amd::check(exa_volume_unknowns_prolongation(&s, luhFine, luhCoarse, coarseGridLevel, fineGridLevel, sub),
+                     "volumeUnknownsProlongation");
+}
+
+template <int numberOfVariables, int numberOfParameters, int basisSize, typename CellIndex>
+void volumeUnknownsRestriction(double* luhCoarse, const double* luhFine, const int coarseGridLevel,
+                               const int fineGridLevel, const CellIndex& subcellIndex) {
+  exa_solver s = exahype_amd::size_descriptor(numberOfVariables, basisSize, EXA_TERM_FLUX, false);
+  s.num_parameters = numberOfParameters;
+  const int sub[3] = {subcellIndex[0], subcellIndex[1], subcellIndex[2]};
+  exahype_amd::check(exa_volume_unknowns_restriction(&s, luhCoarse, luhFine, coarseGridLevel, fineGridLevel, sub),
+                     "volumeUnknownsRestriction");
+}
+
 }  // namespace c
 }  // namespace generic
 }  // namespace aderdg

@@ -71,6 +71,19 @@ void ora_volume_integral_linear(const ora_solver* s, double* lduh, const double*
 void ora_surface_integral_linear(const ora_solver* s, double* lduh, const double* lFhbnd, const double* dx);
 void ora_riemann_solver_linear(const ora_solver* s, double* FL, double* FR, const double* QL, const double* QR,
                                double dt, int d);
+
+/* ---- AMR routines (amr.c, K/c/3d/amrRoutines.cpph) ---- */
+double ora_fine_proj(int order, int j, int n, int m); /* fineGridProjector1d[order][j][n][m] */
+void ora_face_unknowns_prolongation(const ora_solver* s, double* lQhbndFine, double* lFhbndFine,
+                                    const double* lQhbndCoarse, const double* lFhbndCoarse, int coarse_level,
+                                    int fine_level, const int* subface);
+void ora_face_unknowns_restriction(const ora_solver* s, double* lQhbndCoarse, double* lFhbndCoarse,
+                                   const double* lQhbndFine, const double* lFhbndFine, int coarse_level,
+                                   int fine_level, const int* subface);
+void ora_volume_unknowns_prolongation(const ora_solver* s, double* luhFine, const double* luhCoarse,
+                                      int coarse_level, int fine_level, const int* subcell);
+void ora_volume_unknowns_restriction(const ora_solver* s, double* luhCoarse, const double* luhFine,
+                                     int coarse_level, int fine_level, const int* subcell);
 /* lQi (B+1)B^3 D, lFi B^4 4V, gradQ B^4 3V, lQhi B^3 D, lFhi 4 B^3 V; PSi may be NULL */
 void ora_space_time_predictor_linear(const ora_solver* s, double* lQbnd, double* lFbnd, double* lQi, double* lFi,
                                      double* gradQ, double* lQhi, double* lFhi, const double* luh, const double* dx,

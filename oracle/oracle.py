@@ -98,6 +98,13 @@ def lib():
         L.ora_surface_integral_linear.argtypes = [S, P, P, P]
         L.ora_riemann_solver_linear.argtypes = [S, P, P, P, P, D_, I]
         L.ora_space_time_predictor_linear.argtypes = [S, P, P, P, P, P, P, P, P, P, D_, P]
+        IP = ctypes.POINTER(ctypes.c_int)
+        L.ora_fine_proj.argtypes = [I, I, I, I]
+        L.ora_fine_proj.restype = D_
+        L.ora_face_unknowns_prolongation.argtypes = [S, P, P, P, P, I, I, IP]
+        L.ora_face_unknowns_restriction.argtypes = [S, P, P, P, P, I, I, IP]
+        L.ora_volume_unknowns_prolongation.argtypes = [S, P, P, I, I, IP]
+        L.ora_volume_unknowns_restriction.argtypes = [S, P, P, I, I, IP]
         L.ora_godunov_update.argtypes = [S, I, I, P, P, P, D_, D_]
         L.ora_godunov_update.restype = D_
         VP = ctypes.c_void_p
@@ -532,3 +539,49 @@ class LimitingGrid:
 
     def dt(self):
         return lib().ora_lgrid_dt(self._h)
+
+
+# ---------------------------------------------------------------- AMR routines
+
+def fine_proj(order, j, n, m):
+    return lib().ora_fine_proj(order, j, n, m)
+
+
+def _idx(v):
+    return (ctypes.c_int * len(v))(*[int(x) for x in v])
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+
+def face_unknowns_prolongation(s, QC, FC, coarse_level, fine_level, subface):
+    """Returns (lQhbndFine, lFhbndFine) (overwritten by the reference's kernel)."""
+    B, D, V = s.order + 1, s.nvar + s.npar, s.nvar
+    QF, FF = np.zeros(B * B * D), np.zeros(B * B * V)
+    lib().ora_face_unknowns_prolongation(ctypes.byref(s), _p(QF), _p(FF), _p(_f(QC)), _p(_f(FC)), coarse_level,
+                                         fine_level, _idx(subface))
+    return QF, FF
+
+
+def face_unknowns_restriction(s, QC, FC, QF, FF, coarse_level, fine_level, subface):
+    """Returns the coarse arrays with the restricted fine values added."""
+    QC, FC = _f(QC).copy(), _f(FC).copy()
+    lib().ora_face_unknowns_restriction(ctypes.byref(s), _p(QC), _p(FC), _p(_f(QF)), _p(_f(FF)), coarse_level,
+                                        fine_level, _idx(subface))
+    return QC, FC
+
+
+def volume_unknowns_prolongation(s, uC, coarse_level, fine_level, subcell):
+    B, D = s.order + 1, s.nvar + s.npar
+    uF = np.zeros(B ** 3 * D)
+    lib().ora_volume_unknowns_prolongation(ctypes.byref(s), _p(uF), _p(_f(uC)), coarse_level, fine_level,
+                                           _idx(subcell))
+    return uF
+
+
+def volume_unknowns_restriction(s, uC, uF, coarse_level, fine_level, subcell):
+    uC = _f(uC).copy()
+    lib().ora_volume_unknowns_restriction(ctypes.byref(s), _p(uC), _p(_f(uF)), coarse_level, fine_level,
+                                          _idx(subcell))
+    return uC

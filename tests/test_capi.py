@@ -151,10 +151,23 @@ void linear_glue(MyElasticSolver& solver, double* luh, double* lduh, double* lQh
   kernels::aderdg::generic::c::surfaceIntegralLinear<9, 4>(lduh, lFhbnd, dx);
   kernels::aderdg::generic::c::solutionUpdate<MyElasticSolver>(solver, luh, lduh, 1e-3);
 }
+// The AMR calls of ADERDGSolver's prolongation/restriction (Kernels.h:163-199).
+struct Idx2 { int v[2]; int operator[](int i) const { return v[i]; } };
+struct Idx3 { int v[3]; int operator[](int i) const { return v[i]; } };
+void amr_glue(double* lQf, double* lFf, const double* lQc, const double* lFc, double* luhF, double* luhC) {
+  const Idx2 sf{{1, 2}};
+  const Idx3 sc{{0, 1, 2}};
+  kernels::aderdg::generic::c::faceUnknownsProlongation<5, 1, 4>(lQf, lFf, lQc, lFc, 1, 2, sf);
+  kernels::aderdg::generic::c::faceUnknownsRestriction<5, 1, 4>(const_cast<double*>(lQc), const_cast<double*>(lFc),
+                                                                  lQf, lFf, 1, 2, sf);
+  kernels::aderdg::generic::c::volumeUnknownsProlongation<5, 1, 4>(luhF, luhC, 0, 1, sc);
+  kernels::aderdg::generic::c::volumeUnknownsRestriction<5, 1, 4>(luhC, luhF, 0, 1, sc);
+}
 int main(int argc, char**) {
   if (argc > 99) {
     glue(*(MyEulerSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
     linear_glue(*(MyElasticSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
+    amr_glue(0, 0, 0, 0, 0, 0);
     MyFVSolver fv;
     MyGodunovSolver gv;
     return (int)(limiter_glue(nullptr, fv, 0, 0, 0, 0, 0) + godunov_glue(gv, 0, 0));

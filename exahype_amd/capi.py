@@ -49,7 +49,8 @@ EXPORTED = (
     "exa_face_unknowns_prolongation", "exa_face_unknowns_restriction", "exa_volume_unknowns_prolongation",
     "exa_volume_unknowns_restriction", "exa_device_alloc", "exa_device_free", "exa_copy_to_device",
     "exa_copy_to_host", "exa_volume_unknowns_prolongation_batch", "exa_volume_unknowns_restriction_batch",
-    "exa_face_unknowns_prolongation_batch", "exa_face_unknowns_restriction_batch",
+    "exa_face_unknowns_prolongation_batch", "exa_face_unknowns_restriction_batch", "exa_point_source_projection",
+    "exa_quadrature_node",
 )
 
 
@@ -125,6 +126,9 @@ def lib():
     L.exa_riemann_solver_linear.argtypes = [S, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int]
     I, Lg, V = ctypes.c_int, ctypes.c_long, ctypes.c_void_p
     F = ctypes.POINTER(ctypes.c_float)
+    L.exa_point_source_projection.argtypes = [S, _D, _D, _D, _D]
+    L.exa_quadrature_node.argtypes = [I, I]
+    L.exa_quadrature_node.restype = ctypes.c_double
     L.exa_face_unknowns_prolongation.argtypes = [S, _D, _D, _D, _D, I, I, _I]
     L.exa_face_unknowns_restriction.argtypes = [S, _D, _D, _D, _D, I, I, _I]
     L.exa_volume_unknowns_prolongation.argtypes = [S, _D, _D, I, I, _I]
@@ -321,6 +325,14 @@ def riemann_solver_linear(s, QL, QR, dt, direction):
     check(lib().exa_riemann_solver_linear(ctypes.byref(s), _dp(FL), _dp(FR), _dp(_f64(QL)), _dp(_f64(QR)),
                                           float(dt), int(direction)), "riemannSolverLinear")
     return FL, FR
+
+
+def point_source_projection(s, PSi, x0, center, dx):
+    """pointSource's projection: PSi (B+1) B^3 V forces, x0 [(B+1) B^3, 3] source positions."""
+    PSi = np.array(PSi, dtype=np.float64).ravel()
+    check(lib().exa_point_source_projection(ctypes.byref(s), _dp(PSi), _dp(_f64(np.ravel(x0))), _dp(_dx3(center)),
+                                            _dp(_dx3(dx))), "pointSource")
+    return PSi
 
 
 # ---------------------------------------------------------------------------

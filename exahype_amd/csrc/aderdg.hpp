@@ -204,6 +204,10 @@ struct LinearPredictorArgs {
 };
 bool linear_supported(const SolverDesc& sd);
 bool linear_sizes_supported(const SolverDesc& sd);  // (order, nvar) of the linear integrals
+// pointSource's projection of one cell's (B+1) B^3 V forces; x0 per space-time
+// node (3 doubles), geo = center[3], dx[3] (device); orders 1..7.
+hipError_t launch_point_source(const SolverDesc& sd, double* PSi, const double* x0, const double* geo,
+                               hipStream_t s);
 hipError_t launch_linear_predictor(const SolverDesc& sd, bool export_temps, const LinearPredictorArgs& a,
                                    long ncells, hipStream_t s);
 // riemannSolverLinear over a face list (fluctuations into both cells' blocks)

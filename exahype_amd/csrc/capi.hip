@@ -307,6 +307,36 @@ extern "C" int exa_space_time_predictor_linear(const exa_solver* s, double* lQhb
   return EXA_OK;
 }
 
+extern "C" int exa_point_source_projection(const exa_solver* s, double* pointForceSources, const double* x0,
+                                           const double* center, const double* dx) {
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || !pointForceSources || !x0 || !center || !dx) return EXA_E_INVALID;
+  if (sd.order < 1 || sd.order > 7) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  const size_t nodes = (z.B + 1) * z.B3;
+  DevBuf d_ps, d_x0, d_geo;
+  EXA_TRY(d_ps.alloc(sizeof(double) * nodes * z.V));
+  EXA_TRY(d_x0.alloc(sizeof(double) * nodes * 3));
+  EXA_TRY(d_geo.alloc(sizeof(double) * 6));
+  const double geo[6] = {center[0], center[1], center[2], dx[0], dx[1], dx[2]};
+  EXA_TRY(hipMemcpy(d_ps.p, pointForceSources, sizeof(double) * nodes * z.V, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_x0.p, x0, sizeof(double) * nodes * 3, hipMemcpyHostToDevice));
+  EXA_TRY(hipMemcpy(d_geo.p, geo, sizeof(geo), hipMemcpyHostToDevice));
+  EXA_TRY(exa::launch_point_source(sd, d_ps.d(), d_x0.d(), d_geo.d(), nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(pointForceSources, d_ps.p, sizeof(double) * nodes * z.V, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+extern "C" double exa_quadrature_node(int order, int i) {
+  // the tables as constant-initialised locals (host code; only the nodes are used)
+#define EXA_TABLE static const
+#include "dg_tables.inc"
+#undef EXA_TABLE
+  if (order < 0 || order > 9 || i < 0 || i > order) return 0.0;
+  return exa_gl_nodes[order][i];
+}
+
 extern "C" int exa_volume_integral_linear(const exa_solver* s, double* lduh, const double* lFhi, const double* dx) {
   (void)dx;  // unused by the linear volume integral, as in the reference
   SolverDesc sd;

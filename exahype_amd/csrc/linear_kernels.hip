@@ -321,6 +321,51 @@ __global__ __launch_bounds__(256) void linear_riemann_kernel(RiemannArgs a) {
   }
 }
 
+// The projection step of pointSource (K/pointSource.cpph:40-126): space-time
+// node (n, z, y, x) per thread; its force values become phi_z(xRef_z)
+// phi_y(xRef_y) phi_x(xRef_x) F / (w_z dz w_y dy w_x dx) for the source
+// position x0 the user's pointSource returned at that node, xRef = (x0 -
+// (center - dx/2)) / dx, or zero when x0 lies outside the cell.  phi: the
+// Lagrange basis on the Gauss-Legendre nodes in product form.
+template <int B>
+__global__ __launch_bounds__(256) void point_source_kernel(double* PSi, const double* x0, const double* geo, int V) {
+  constexpr int N = B - 1, B3 = B * B * B;
+  const int node = blockIdx.x * 256 + threadIdx.x;
+  if (node >= (B + 1) * B3) return;
+  const int k = node % B, j = (node / B) % B, i = (node / (B * B)) % B;
+  double xr[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) xr[d] = (x0[node * 3 + d] - (geo[d] - geo[3 + d] / 2.)) / geo[3 + d];
+  const bool inside = xr[0] >= 0. && xr[1] >= 0. && xr[2] >= 0. && xr[0] <= 1.0 && xr[1] <= 1.0 && xr[2] <= 1.0;
+  auto phi = [](int n, double x) {
+    double v = 1.0;
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+      if (q != n) v *= (x - exa_gl_nodes[N][q]) / (exa_gl_nodes[N][n] - exa_gl_nodes[N][q]);
+    return v;
+  };
+  const double b = phi(i, xr[2]) * phi(j, xr[1]) * phi(k, xr[0]);
+  const double den = exa_gl_weights[N][i] * geo[5] * exa_gl_weights[N][j] * geo[4] * exa_gl_weights[N][k] * geo[3];
+  for (int m = 0; m < V; ++m) {
+    double* p = PSi + (size_t)node * V + m;
+    *p = inside ? b * (*p) / den : 0.0;
+  }
+}
+
+hipError_t launch_point_source(const SolverDesc& sd, double* PSi, const double* x0, const double* geo,
+                               hipStream_t s) {
+  const int B = sd.order + 1;
+  const unsigned blocks = (unsigned)(((B + 1) * B * B * B + 255) / 256);
+  switch (B) {
+#define EXA_PS(BB) \
+  case BB: hipLaunchKernelGGL((point_source_kernel<BB>), dim3(blocks), dim3(256), 0, s, PSi, x0, geo, sd.nvar); break;
+    EXA_PS(2) EXA_PS(3) EXA_PS(4) EXA_PS(5) EXA_PS(6) EXA_PS(7) EXA_PS(8)
+#undef EXA_PS
+    default: return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
 template <int B, int V>
 __global__ __launch_bounds__(node_threads(B)) void linear_volume_kernel(double* lduh, const double* lFhi) {
   constexpr int N = B - 1, B2 = B * B, B3 = B2 * B;

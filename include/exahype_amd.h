@@ -116,6 +116,19 @@ int exa_space_time_predictor_linear(const exa_solver* s, double* lQhbnd, double*
                                     double* lFi, double* gradQ, double* lQhi, double* lFhi,
                                     const double* luh, const double* dx, double dt, const double* PSi);
 
+/* The projection step of pointSource (K/c/3d/pointSource.cpph:17-126): on entry
+ * pointForceSources holds the user's pointSource values at the (B+1) B^3
+ * space-time nodes ((t,z,y,x,var); time node 0 at t, node n >= 1 at t + dt
+ * x_{n-1}; space node at center + dx (x_node - 1/2)) and x0 the source position
+ * that call returned (3 doubles per node); on return each value is the point
+ * force projected onto the cell's nodal basis, or zero where x0 lies outside
+ * the cell.  Orders 1..7.  kernels.hpp's pointSource<SolverType> makes the
+ * user calls on the host and then calls this. */
+int exa_point_source_projection(const exa_solver* s, double* pointForceSources, const double* x0,
+                                const double* center, const double* dx);
+/* Gauss-Legendre node i of [0,1] at an order (the table the kernels use). */
+double exa_quadrature_node(int order, int i);
+
 /* volumeIntegralLinear (K/c/3d/volumeIntegralLinear.cpph:22); lFhi 4 B^3 V
  * (the source block is not read); dx is unused, as in the reference. */
 int exa_volume_integral_linear(const exa_solver* s, double* lduh, const double* lFhi, const double* dx);

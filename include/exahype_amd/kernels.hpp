@@ -197,6 +197,42 @@ void riemannSolverLinear(SolverType& /*solver*/, double* FL, double* FR, const d
   exahype_amd::check(exa_riemann_solver_linear(&s, FL, FR, QL, QR, dt, normalNonZero), "riemannSolverLinear");
 }
 
+// pointSource (Kernels.h:203, pointSource.cpph:17): the user's pointSource is
+// called on the host at every space-time node, as the reference calls it; the
+// projection onto the nodal basis runs on the device.
+template <typename SolverType, typename DxVector>
+void pointSource(SolverType& solver, const double t, const double dt, const DxVector& center, const DxVector& dx,
+                 const int numberOfVariables, const int numberOfParameters, const int basisSize,
+                 double* pointForceSources) {
+  const int order = basisSize - 1;
+  const int nodes = (basisSize + 1) * basisSize * basisSize * basisSize;
+  double* x0 = new double[3 * nodes];
+  double x[3], x0n[3];
+  for (int n = 0; n < basisSize + 1; ++n) {
+    const double tn = n == 0 ? t : dt * exa_quadrature_node(order, n - 1) + t;
+    for (int i = 0; i < basisSize; ++i)
+      for (int j = 0; j < basisSize; ++j)
+        for (int k = 0; k < basisSize; ++k) {
+          x[2] = center[2] + dx[2] * (exa_quadrature_node(order, i) - 0.5);
+          x[1] = center[1] + dx[1] * (exa_quadrature_node(order, j) - 0.5);
+          x[0] = center[0] + dx[0] * (exa_quadrature_node(order, k) - 0.5);
+          const int node = ((n * basisSize + i) * basisSize + j) * basisSize + k;
+          solver.pointSource(x, tn, dt, pointForceSources + node * numberOfVariables, x0n);
+          x0[3 * node] = x0n[0];
+          x0[3 * node + 1] = x0n[1];
+          x0[3 * node + 2] = x0n[2];
+        }
+  }
+  exa_solver s = exahype_amd::size_descriptor(numberOfVariables, basisSize, EXA_TERM_SOURCE, false);
+  s.num_parameters = numberOfParameters;
+  double c[3], d[3];
+  exahype_amd::dx3(center, c);
+  exahype_amd::dx3(dx, d);
+  const int st = exa_point_source_projection(&s, pointForceSources, x0, c, d);
+  delete[] x0;
+  exahype_amd::check(st, "pointSource");
+}
+
 // ---- AMR transfer operators (Kernels.h:163-199, amrRoutines.cpph) ----
 
 template <int numberOfVariables, int numberOfParameters, int basisSize, typename FaceIndex>

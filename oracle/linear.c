@@ -401,3 +401,44 @@ void ora_space_time_predictor_linear(const ora_solver* s, double* lQbnd, double*
 #undef FLC
 #undef FRC
 }
+
+/* Lagrange basis function n on the Gauss-Legendre nodes of [0,1], at x (the
+ * reference's basisFunctions[order][n], DGBasisFunctions.cpp, printed as
+ * 15-digit polynomials; the product form agrees with them to ~1e-14). */
+static double gl_basis(int o, int n, double x) {
+  double v = 1.0;
+  for (int k = 0; k <= o; k++)
+    if (k != n) v *= (x - ora_table(ORA_T_NODES, o, k, 0)) / (ora_table(ORA_T_NODES, o, n, 0) - ora_table(ORA_T_NODES, o, k, 0));
+  return v;
+}
+
+/* The projection step of pointSource, K/pointSource.cpph:40-126: the user's
+ * pointSource has filled PSi[(n,i,j,k)] (V values, time node n = 0 at t and
+ * n >= 1 at t + dt x_{n-1}; space node (z,y,x) = (i,j,k)) and returned the
+ * source position x0 for that node (x0[node*3 + {x,y,z}]); each value becomes
+ * phi_i(xRef_z) phi_j(xRef_y) phi_k(xRef_x) F / (w_i dz w_j dy w_k dx) with
+ * xRef = (x0 - (center - dx/2)) / dx, or zero when x0 lies outside the cell. */
+void ora_point_source_projection(const ora_solver* s, double* PSi, const double* x0, const double* center,
+                                 const double* dx) {
+  const int V = s->nvar, B = s->order + 1, o = s->order;
+  for (int n = 0; n < B + 1; n++)
+    for (int i = 0; i < B; i++)
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++) {
+          const long node = (((long)n * B + i) * B + j) * B + k;
+          double xr[3];
+          for (int d = 0; d < 3; d++) xr[d] = (x0[node * 3 + d] - (center[d] - dx[d] / 2.)) / dx[d];
+          const int inside = xr[0] >= 0. && xr[1] >= 0. && xr[2] >= 0. && xr[0] <= 1.0 && xr[1] <= 1.0 && xr[2] <= 1.0;
+          for (int m = 0; m < V; m++) {
+            double* p = PSi + node * V + m;
+            if (inside) {
+              double value = gl_basis(o, i, xr[2]) * gl_basis(o, j, xr[1]) * gl_basis(o, k, xr[0]) * (*p);
+              value /= (ora_table(ORA_T_WEIGHTS, o, i, 0) * dx[2] * ora_table(ORA_T_WEIGHTS, o, j, 0) * dx[1] *
+                        ora_table(ORA_T_WEIGHTS, o, k, 0) * dx[0]);
+              *p = value;
+            } else {
+              *p = 0.0;
+            }
+          }
+        }
+}

@@ -72,6 +72,10 @@ void ora_surface_integral_linear(const ora_solver* s, double* lduh, const double
 void ora_riemann_solver_linear(const ora_solver* s, double* FL, double* FR, const double* QL, const double* QR,
                                double dt, int d);
 
+/* pointSource's projection (linear.c, K/c/3d/pointSource.cpph:40-126). */
+void ora_point_source_projection(const ora_solver* s, double* PSi, const double* x0, const double* center,
+                                 const double* dx);
+
 /* ---- AMR routines (amr.c, K/c/3d/amrRoutines.cpph) ---- */
 double ora_fine_proj(int order, int j, int n, int m); /* fineGridProjector1d[order][j][n][m] */
 void ora_face_unknowns_prolongation(const ora_solver* s, double* lQhbndFine, double* lFhbndFine,

@@ -99,6 +99,7 @@ def lib():
         L.ora_riemann_solver_linear.argtypes = [S, P, P, P, P, D_, I]
         L.ora_space_time_predictor_linear.argtypes = [S, P, P, P, P, P, P, P, P, P, D_, P]
         IP = ctypes.POINTER(ctypes.c_int)
+        L.ora_point_source_projection.argtypes = [S, P, P, P, P]
         L.ora_fine_proj.argtypes = [I, I, I, I]
         L.ora_fine_proj.restype = D_
         L.ora_face_unknowns_prolongation.argtypes = [S, P, P, P, P, I, I, IP]
@@ -585,3 +586,10 @@ def volume_unknowns_restriction(s, uC, uF, coarse_level, fine_level, subcell):
     lib().ora_volume_unknowns_restriction(ctypes.byref(s), _p(uC), _p(_f(uF)), coarse_level, fine_level,
                                           _idx(subcell))
     return uC
+
+
+def point_source_projection(s, PSi, x0, center, dx):
+    """pointSource's projection: PSi (B+1) B^3 V forces, x0 [(B+1) B^3, 3] source positions."""
+    out = _f(PSi).copy()
+    lib().ora_point_source_projection(ctypes.byref(s), _p(out), _p(_f(x0)), _p(dx3(center)), _p(dx3(dx)))
+    return out

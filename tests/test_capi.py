@@ -151,6 +151,17 @@ void linear_glue(MyElasticSolver& solver, double* luh, double* lduh, double* lQh
   kernels::aderdg::generic::c::surfaceIntegralLinear<9, 4>(lduh, lFhbnd, dx);
   kernels::aderdg::generic::c::solutionUpdate<MyElasticSolver>(solver, luh, lduh, 1e-3);
 }
+// A solver with a point source (UserSolverInterface.h:75) and the pointSource kernel.
+struct MyPointSourceSolver {
+  void pointSource(const double* const x, const double t, const double dt, double* forceVector, double* x0) {
+    for (int m = 0; m < 9; ++m) forceVector[m] = x[0] + t + dt;
+    x0[0] = 0.5; x0[1] = 0.5; x0[2] = 0.5;
+  }
+};
+void point_source_glue(MyPointSourceSolver& solver, double* ps) {
+  const Vec3 c{{0.5, 0.5, 0.5}}, dx{{0.1, 0.1, 0.1}};
+  kernels::aderdg::generic::c::pointSource(solver, 0.0, 1e-3, c, dx, 9, 3, 4, ps);
+}
 // The AMR calls of ADERDGSolver's prolongation/restriction (Kernels.h:163-199).
 struct Idx2 { int v[2]; int operator[](int i) const { return v[i]; } };
 struct Idx3 { int v[3]; int operator[](int i) const { return v[i]; } };
@@ -168,6 +179,8 @@ int main(int argc, char**) {
     glue(*(MyEulerSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
     linear_glue(*(MyElasticSolver*)nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
     amr_glue(0, 0, 0, 0, 0, 0);
+    MyPointSourceSolver pss;
+    point_source_glue(pss, 0);
     MyFVSolver fv;
     MyGodunovSolver gv;
     return (int)(limiter_glue(nullptr, fv, 0, 0, 0, 0, 0) + godunov_glue(gv, 0, 0));

@@ -143,3 +143,20 @@ def test_linear_grid_vs_oracle(exa, oracle, order, n, steps):
     print(f"  max rel error per variable {err:.3e}", flush=True)
     assert err <= 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("order", [1, 3, 4, 5, 7])
+def test_point_source_projection_vs_oracle(exa, oracle, order):
+    print(f"case: pointSource projection N={order}", flush=True)
+    rng = np.random.default_rng(300 + order)
+    B = order + 1
+    n = (B + 1) * B ** 3
+    center, dx = np.array([0.3, -0.2, 1.1]), np.array([0.05, 0.07, 0.04])
+    PSi = rng.standard_normal(n * 9)
+    # per-node positions: most inside the cell, some outside
+    X0 = center - dx / 2 + rng.uniform(-0.2, 1.2, (n, 3)) * dx
+    dev = exa.point_source_projection(exa.elastic_solver(order), PSi, X0, center, dx)
+    ref = oracle.point_source_projection(oracle.elastic_solver(order), PSi, X0, center, dx)
+    assert _rel(dev, ref) <= 1e-12
+    assert np.array_equal(dev == 0.0, ref == 0.0)
+    assert all(exa.lib().exa_quadrature_node(order, i) == oracle.table(oracle.T_NODES, order, i) for i in range(B))

@@ -292,3 +292,38 @@ def test_linear_grid_matches_nonlinear_grid_homogeneous(oracle):
     assert np.array_equal(a[:, 9:], b[:, 9:])
     # the step moved the state by far more than that difference
     assert np.abs(a[:, :9] - luh[:, :9]).max() > 1e3 * err * np.abs(a[:, :9]).max()
+
+
+# ----------------------------------------------------------- pointSource projection
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_point_source_projection_interpolates_the_delta(oracle, order):
+    """pointSource.cpph:40-126 projects a point force at x0 onto the nodal
+    basis: quadrature of the projected field against any polynomial p of
+    degree <= N per direction returns F p(x0) (Lagrange interpolation on the
+    Gauss nodes), for every time node; sources outside the cell give zero."""
+    rng = np.random.default_rng(order)
+    s = oracle.elastic_solver(order)
+    B, V = order + 1, 9
+    nodes, w = _nodes(oracle, order), _w(oracle, order)
+    center, dx = np.array([0.3, -0.2, 1.1]), np.array([0.05, 0.07, 0.04])
+    xref0 = rng.random(3)
+    x0 = center - dx / 2 + xref0 * dx
+    F = rng.standard_normal((B + 1, V))
+    PSi = np.repeat(F[:, None, :], B ** 3, 1)                  # same force at every space node of a time node
+    X0 = np.tile(x0, ((B + 1) * B ** 3, 1))
+    out = oracle.point_source_projection(s, PSi, X0, center, dx).reshape(B + 1, B, B, B, V)
+    coef = rng.standard_normal((B, B, B))
+    def p(x, y, z):
+        return sum(coef[a, b, c] * x ** a * y ** b * z ** c for a in range(B) for b in range(B) for c in range(B))
+    quad = np.zeros((B + 1, V))
+    for i in range(B):
+        for j in range(B):
+            for k in range(B):
+                wt = (w[i] * dx[2]) * (w[j] * dx[1]) * (w[k] * dx[0])
+                quad += wt * out[:, i, j, k, :] * p(nodes[k], nodes[j], nodes[i])
+    exact = F * p(*xref0)
+    assert np.abs(quad - exact).max() <= 1e-10 * np.abs(exact).max()
+    # outside the cell: zero
+    X1 = np.tile(center + 0.6 * dx, ((B + 1) * B ** 3, 1))
+    assert not np.any(oracle.point_source_projection(s, PSi, X1, center, dx))

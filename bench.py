@@ -336,6 +336,58 @@ def bench_amr(capi, args):
         a.close()
 
 
+OUTPUT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "output_plot_vertices_n5_81cube_pmc_traffic.txt")
+
+
+def output_measured_traffic():
+    """HBM bytes per vertex-interpolation launch (rocprofv3 --pmc, copy-calibrated); None otherwise."""
+    if not os.path.exists(OUTPUT_TRAFFIC_FILE):
+        return None
+    for line in open(OUTPUT_TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch plot_vertices 81^3 N=5 "):
+            return int(line.split()[-1])
+    return None
+
+
+def bench_output(capi, args):
+    """Output workload: the Cartesian VTK plotter's interpolation of a 81^3
+    Euler grid at N=5 (every cell's 6^3 equidistant vertex values) on the
+    device, FP64; a step = one projection of the whole grid."""
+    n, order = 81, 5
+    B, D = order + 1, 5
+    g = capi.Grid(capi.euler_solver(order), n)
+    g.init_euler()
+    g.step(1)
+    g.synchronize()
+    for _ in range(args.warmup):
+        g.plot_vertices()
+    ts = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts.append(g.plot_vertices(timed=True)[1])
+    wall = time.perf_counter() - t0  # includes the copy of the 4.6 GB result to the host
+    k_ms = float(np.mean(ts))
+    vals = n ** 3 * B ** 3 * D
+    alg = 2 * vals * 8  # read luh, write the vertex values
+    achieved = alg / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": "vertex values/s, Cartesian VTK plotter interpolation on the device, 3D Euler N=5 81^3, FP64",
+        "value": vals / (k_ms * 1e-3), "unit": "vertex values/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": k_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (entropy wave initial condition generated on the device, one time step)",
+        "config": {"workload": f"plot vertices {n}^3 cells N={order}", "values": vals,
+                   "wall_ms_per_step_incl_host_copy": round(wall / args.steps * 1e3, 3)},
+        "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=output_measured_traffic(),
+                         kernel="plot_vertices_kernel<6> (equidistant vertex interpolation)",
+                         avg_launch_ms=round(k_ms, 4), algorithmic_bytes_per_launch=alg),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
 def bench_fv(capi, args):
     """Finite-volumes workload: the EulerFV demonstrator's Godunov solver in
     3-D, 54^3 cells of 10^3 subcells (540^3), periodic entropy wave + 8
@@ -544,14 +596,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fv-cells", type=int, default=54, help="fv: cells per direction")
     ap.add_argument("--patch-size", type=int, default=10, help="fv: subcells per cell and direction")
-    ap.add_argument("--workload", choices=("precision", "limiter", "batch", "elastic", "fv", "amr"),
+    ap.add_argument("--workload", choices=("precision", "limiter", "batch", "elastic", "fv", "amr", "output"),
                     default="precision",
                     help="precision: Euler N=5 81^3, FP32-bulk vs FP64 (the headline line); "
                          "limiter: Euler blast N=3 81^3 with the subcell limiter, FP64; "
                          "batch: 96 seeded uneven patches (orders 2..7), FP32-bulk vs FP64; "
                          "elastic: linear elasticity N=4 81^3, FP32-bulk vs FP64; "
                          "fv: Godunov finite volumes, 54^3 cells of 10^3 subcells, FP64; "
-                         "amr: volume prolongation + restriction, 27^3 -> 81^3 cells, N=5, FP64")
+                         "amr: volume prolongation + restriction, 27^3 -> 81^3 cells, N=5, FP64; "
+                         "output: VTK vertex interpolation of an 81^3 N=5 grid, FP64")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="batch: GB of patch state per wave")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 times): the known-size "
@@ -578,6 +631,9 @@ def main():
         sys.exit("--form linear applies to the elastic workload only")
     if args.workload == "amr":
         bench_amr(capi, args)
+        return
+    if args.workload == "output":
+        bench_output(capi, args)
         return
     if args.workload == "elastic":
         bench_elastic(capi, args)

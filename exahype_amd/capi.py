@@ -50,7 +50,7 @@ EXPORTED = (
     "exa_volume_unknowns_restriction", "exa_device_alloc", "exa_device_free", "exa_copy_to_device",
     "exa_copy_to_host", "exa_volume_unknowns_prolongation_batch", "exa_volume_unknowns_restriction_batch",
     "exa_face_unknowns_prolongation_batch", "exa_face_unknowns_restriction_batch", "exa_point_source_projection",
-    "exa_quadrature_node",
+    "exa_quadrature_node", "exa_plot_vertices", "exa_grid_plot_vertices",
 )
 
 
@@ -127,6 +127,8 @@ def lib():
     I, Lg, V = ctypes.c_int, ctypes.c_long, ctypes.c_void_p
     F = ctypes.POINTER(ctypes.c_float)
     L.exa_point_source_projection.argtypes = [S, _D, _D, _D, _D]
+    L.exa_plot_vertices.argtypes = [S, Lg, _D, _D]
+    L.exa_grid_plot_vertices.argtypes = [V, _D, F]
     L.exa_quadrature_node.argtypes = [I, I]
     L.exa_quadrature_node.restype = ctypes.c_double
     L.exa_face_unknowns_prolongation.argtypes = [S, _D, _D, _D, _D, I, I, _I]
@@ -327,6 +329,13 @@ def riemann_solver_linear(s, QL, QR, dt, direction):
     return FL, FR
 
 
+def plot_vertices(s, luh, ncells):
+    """The Cartesian VTK plotter's values at each cell's (N+1)^3 equidistant vertices."""
+    out = np.zeros(ncells * s.B ** 3 * s.D)
+    check(lib().exa_plot_vertices(ctypes.byref(s), ncells, _dp(_f64(luh)), _dp(out)), "plotVertexData")
+    return out
+
+
 def point_source_projection(s, PSi, x0, center, dx):
     """pointSource's projection: PSi (B+1) B^3 V forces, x0 [(B+1) B^3, 3] source positions."""
     PSi = np.array(PSi, dtype=np.float64).ravel()
@@ -498,6 +507,14 @@ class Grid:
         out = np.zeros(self.ncells * self.s.B ** 3 * self.s.D)
         check(lib().exa_grid_download_solution(self._h, _dp(out)), "download")
         return out
+
+    def plot_vertices(self, timed=False):
+        """Values at every cell's (N+1)^3 equidistant vertices (the Cartesian VTK
+        plotter's interpolation, on the device); with timed, also the kernel ms."""
+        out = np.zeros(self.ncells * self.s.B ** 3 * self.s.D)
+        t = ctypes.c_float(0.0)
+        check(lib().exa_grid_plot_vertices(self._h, _dp(out), ctypes.byref(t)), "exa_grid_plot_vertices")
+        return (out, t.value) if timed else out
 
     def cells(self, first, count):
         out = np.zeros(count * self.s.B ** 3 * self.s.D)

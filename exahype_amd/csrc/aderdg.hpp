@@ -218,6 +218,10 @@ hipError_t launch_linear_volume(const SolverDesc& sd, double* lduh, const double
 // surfaceIntegralLinear of one or more cells (lduh in/out, one dx for all)
 hipError_t launch_linear_surface(const SolverDesc& sd, double* lduh, const double* lFhbnd, const double* dx,
                                  long ncells, hipStream_t s);
+// The Cartesian VTK plotter's interpolation (output_kernels.hip): luh
+// [cells][B^3][D] -> values at each cell's (N+1)^3 equidistant vertices,
+// [cells][B^3][D] (x fastest); orders 1..6.
+hipError_t launch_plot_vertices(int order, int D, const double* luh, double* out, long ncells, hipStream_t s);
 // st->picard_total += sum(iters[0..ncells)), st->predictor_cells += ncells
 hipError_t launch_count_iterations(const int* iters, long ncells, StepState* st, hipStream_t s);
 

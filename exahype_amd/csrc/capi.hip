@@ -808,3 +808,48 @@ extern "C" int exa_grid_stage_times(exa_grid* g, double* ms, long* launches) {
   }
   return EXA_OK;
 }
+
+// ---- output: the Cartesian VTK plotter's vertex values ----
+
+extern "C" int exa_plot_vertices(const exa_solver* s, long ncells, const double* luh, double* vertices) {
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || ncells < 0 || (ncells > 0 && (!luh || !vertices))) return EXA_E_INVALID;
+  if (sd.order < 1 || sd.order > 6) return EXA_E_UNSUPPORTED;
+  const Sizes z(sd);
+  const size_t n = (size_t)ncells * z.B3 * z.D;
+  DevBuf d_in, d_out;
+  EXA_TRY(d_in.alloc(sizeof(double) * n));
+  EXA_TRY(d_out.alloc(sizeof(double) * n));
+  EXA_TRY(hipMemcpy(d_in.p, luh, sizeof(double) * n, hipMemcpyHostToDevice));
+  EXA_TRY(exa::launch_plot_vertices(sd.order, (int)z.D, d_in.d(), d_out.d(), ncells, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  EXA_TRY(hipMemcpy(vertices, d_out.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_plot_vertices(exa_grid* g, double* vertices, float* elapsed_ms) {
+  if (!g || !vertices) return EXA_E_INVALID;
+  if (g->sd.order < 1 || g->sd.order > 6) return EXA_E_UNSUPPORTED;
+  const Sizes z(g->sd);
+  const size_t n = (size_t)g->ncells * z.B3 * z.D;
+  DevBuf d_out;
+  EXA_TRY(d_out.alloc(sizeof(double) * n));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (elapsed_ms) {
+    EXA_TRY(hipEventCreate(&e0));
+    EXA_TRY(hipEventCreate(&e1));
+    EXA_TRY(hipEventRecord(e0, g->stream));
+  }
+  const hipError_t le = exa::launch_plot_vertices(g->sd.order, (int)z.D, g->luh, d_out.d(), g->ncells, g->stream);
+  if (elapsed_ms && le == hipSuccess) (void)hipEventRecord(e1, g->stream);
+  const hipError_t se = hipStreamSynchronize(g->stream);
+  if (elapsed_ms) {
+    if (le == hipSuccess && se == hipSuccess) (void)hipEventElapsedTime(elapsed_ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  EXA_TRY(le);
+  EXA_TRY(se);
+  EXA_TRY(hipMemcpy(vertices, d_out.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return EXA_OK;
+}

@@ -142,6 +142,14 @@ int exa_surface_integral_linear(const exa_solver* s, double* lduh, const double*
 int exa_riemann_solver_linear(const exa_solver* s, double* FL, double* FR, const double* QL,
                               const double* QR, double dt, int direction);
 
+/* ---- output: the Cartesian VTK plotter's vertex values (FP64) ----
+ * ADERDG2CartesianVTK::plotVertexData (exahype/plotters/VTK/ADERDG2CartesianVTK.cpp:288):
+ * each cell's V+P values at its (N+1)^3 equidistant vertices offset + i h/N
+ * (equidistantGridProjector1d), [cells][(N+1)^3][V+P] with x fastest; the
+ * identity post-processing.  Orders 1..6.  exahype_amd/vtk.py writes them as a
+ * legacy VTK file. */
+int exa_plot_vertices(const exa_solver* s, long ncells, const double* luh, double* vertices);
+
 /* ---- AMR transfer operators (FP64; K/c/3d/amrRoutines.cpph, Kernels.h:163-199).
  * Blocks: volumes luh (z,y,x,var) B^3 (V+P); faces lQhbnd (b,a,var) B^2 (V+P) and
  * lFhbnd B^2 V of one face.  subfaceIndex[2] / subcellIndex[3] in [0, 3^(fine -
@@ -264,6 +272,11 @@ int exa_grid_init_elastic(exa_grid* g, unsigned long long seed, double width, do
 
 /* Advance nsteps time steps (asynchronous on the grid's stream). */
 int exa_grid_step(exa_grid* g, int nsteps);
+
+/* The grid's vertex values for the Cartesian VTK plotter (exa_plot_vertices'
+ * layout, cells in grid order), computed on the device and copied to the host;
+ * elapsed_ms (optional): the kernel's time from HIP events on the grid's stream. */
+int exa_grid_plot_vertices(exa_grid* g, double* vertices, float* elapsed_ms);
 int exa_grid_synchronize(exa_grid* g);
 /* Current time stamp, time step size and number of completed steps. */
 int exa_grid_time(exa_grid* g, double* t, double* dt, long* steps);

@@ -76,6 +76,10 @@ void ora_riemann_solver_linear(const ora_solver* s, double* FL, double* FR, cons
 void ora_point_source_projection(const ora_solver* s, double* PSi, const double* x0, const double* center,
                                  const double* dx);
 
+/* ---- Cartesian VTK plotter interpolation (output.c) ---- */
+double ora_equi_proj(int order, int n, int m); /* equidistantGridProjector1d[order][n][m] */
+void ora_plot_vertices(const ora_solver* s, long ncells, const double* luh, double* out);
+
 /* ---- AMR routines (amr.c, K/c/3d/amrRoutines.cpph) ---- */
 double ora_fine_proj(int order, int j, int n, int m); /* fineGridProjector1d[order][j][n][m] */
 void ora_face_unknowns_prolongation(const ora_solver* s, double* lQhbndFine, double* lFhbndFine,

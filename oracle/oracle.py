@@ -100,6 +100,9 @@ def lib():
         L.ora_space_time_predictor_linear.argtypes = [S, P, P, P, P, P, P, P, P, P, D_, P]
         IP = ctypes.POINTER(ctypes.c_int)
         L.ora_point_source_projection.argtypes = [S, P, P, P, P]
+        L.ora_equi_proj.argtypes = [I, I, I]
+        L.ora_equi_proj.restype = D_
+        L.ora_plot_vertices.argtypes = [S, ctypes.c_long, P, P]
         L.ora_fine_proj.argtypes = [I, I, I, I]
         L.ora_fine_proj.restype = D_
         L.ora_face_unknowns_prolongation.argtypes = [S, P, P, P, P, I, I, IP]
@@ -592,4 +595,17 @@ def point_source_projection(s, PSi, x0, center, dx):
     """pointSource's projection: PSi (B+1) B^3 V forces, x0 [(B+1) B^3, 3] source positions."""
     out = _f(PSi).copy()
     lib().ora_point_source_projection(ctypes.byref(s), _p(out), _p(_f(x0)), _p(dx3(center)), _p(dx3(dx)))
+    return out
+
+
+# ------------------------------------------------------- VTK plotter interpolation
+
+def equi_proj(order, n, m):
+    return lib().ora_equi_proj(order, n, m)
+
+
+def plot_vertices(s, luh, ncells):
+    """Values at the (N+1)^3 equidistant vertices of every cell, [ncells][B^3][D]."""
+    out = np.zeros(len(_f(luh)))
+    lib().ora_plot_vertices(ctypes.byref(s), ncells, _p(_f(luh)), _p(out))
     return out

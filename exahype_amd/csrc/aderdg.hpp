@@ -31,6 +31,9 @@ struct SolverDesc {
 enum : int { kFP64 = 0, kFP32Bulk = 1 };
 
 // A face between two cells; `left` owns face 2*dir+1, `right` owns face 2*dir.
+// pad: 0 for such a face; in linear grids with transmissive boundaries, 1 marks
+// the boundary face 2*dir of cell `right` and 2 the boundary face 2*dir+1 of
+// cell `left` (the other index is unused).
 struct FaceDesc {
   int left;
   int right;

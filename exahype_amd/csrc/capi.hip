@@ -555,9 +555,11 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   if (!to_desc(s, &sd) || nx < 1 || ny < 1 || nz < 1 || !lo || !hi) return EXA_E_INVALID;
   if (boundary != EXA_BC_PERIODIC && boundary != EXA_BC_TRANSMISSIVE) return EXA_E_UNSUPPORTED;
   const bool periodic = boundary == EXA_BC_PERIODIC;
-  // Linear solvers: periodic grids only (at a transmissive face the linear
-  // Riemann solver's fluctuations vanish, which this grid does not model).
-  if (sd.linear ? !exa::linear_supported(sd) || !periodic : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  if (sd.linear ? !exa::linear_supported(sd) : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  // Linear solvers with transmissive boundaries: the linear Riemann solver's
+  // fluctuations vanish at a boundary face (outside = inside), so those faces
+  // join the face list to be zeroed.
+  const bool linear_bc = sd.linear && !periodic;
   if ((double)nx * ny * nz > 2.0e9) return EXA_E_INVALID;
   exa_grid* g = new exa_grid();
   g->sd = sd;
@@ -594,6 +596,7 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   // with transmissive values (outside = inside) Rusanov returns the inner
   // flux unchanged, so those faces need no work.
   g->nfaces = periodic ? 3 * g->ncells : (long)(nx - 1) * ny * nz + (long)nx * (ny - 1) * nz + (long)nx * ny * (nz - 1);
+  if (linear_bc) g->nfaces += 2 * ((long)ny * nz + (long)nx * nz + (long)nx * ny);
   A((void**)&g->faces, sizeof(FaceDesc) * g->nfaces);
   A((void**)&g->st, sizeof(exa::StepState));
   g->dt_hist_len = 1 << 16;
@@ -621,6 +624,17 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
           const int r = dir == 0 ? cid(i + 1, j, k) : (dir == 1 ? cid(i, j + 1, k) : cid(i, j, k + 1));
           faces[f++] = FaceDesc{c, r, dir, 0};
         }
+  if (linear_bc)
+    for (int dir = 0; dir < 3; ++dir)
+      for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) {
+            const int pos = dir == 0 ? i : (dir == 1 ? j : k);
+            const int nmax = dir == 0 ? nx : (dir == 1 ? ny : nz);
+            const int c = cid(i, j, k);
+            if (pos == 0) faces[f++] = FaceDesc{c, c, dir, 1};
+            if (pos == nmax - 1) faces[f++] = FaceDesc{c, c, dir, 2};
+          }
   exa::StepState st0{};
   st0.dt = 0.0;
   st0.t = 0.0;

@@ -271,6 +271,14 @@ __global__ __launch_bounds__(256) void linear_riemann_kernel(RiemannArgs a) {
   const int d = fd.dir;
   double* FL = static_cast<double*>(a.lFhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * V);
   double* FR = static_cast<double*>(a.lFhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * V);
+  if (fd.pad != 0) {
+    // A transmissive boundary face (pad 1: face 2d of cell `right`, pad 2: face
+    // 2d+1 of cell `left`): boundaryValues returns the inside state as the
+    // outside one, so QR - QL = 0 and both fluctuations vanish.
+    double* F = fd.pad == 1 ? FR : FL;
+    for (int e = lane; e < B2 * V; e += kWave) F[e] = 0.0;
+    return;
+  }
   const double* QL = static_cast<const double*>(a.lQhbnd) + ((size_t)fd.left * 6 + 2 * d + 1) * (B2 * D);
   const double* QR = static_cast<const double*>(a.lQhbnd) + ((size_t)fd.right * 6 + 2 * d) * (B2 * D);
   double* const sQL = sStage[w];

@@ -240,8 +240,8 @@ enum { EXA_BC_PERIODIC = 0, EXA_BC_TRANSMISSIVE = 1 };
  * ExaHyPE/kernels/aderdg/generic/c/3d/boundaryConditions.cpph:24, with
  * boundaryValues returning the inside state).  A solver with linear = 1 runs
  * the linear kernels (Cauchy-Kowalevski predictor + volumeIntegralLinear,
- * riemannSolverLinear, surfaceIntegralLinear + solutionUpdate) on periodic
- * grids only; EXA_E_UNSUPPORTED otherwise. */
+ * riemannSolverLinear, surfaceIntegralLinear + solutionUpdate); at a
+ * transmissive boundary face its fluctuations vanish (outside = inside). */
 int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo,
                     const double* hi, int boundary, exa_grid** out);
 int exa_grid_destroy(exa_grid* g);

@@ -118,8 +118,34 @@ def test_linear_euler_golden_vectors(exa):
 def test_linear_solver_support_flags(exa):
     assert exa.supported(exa.elastic_solver(4, linear=True))
     assert not exa.supported(exa.elastic_solver(7, linear=True))
-    with pytest.raises(exa.ExaError):  # linear grids are periodic only
-        exa.Grid(exa.elastic_solver(3, linear=True), 3, boundary=exa.BC_TRANSMISSIVE)
+    with pytest.raises(exa.ExaError):
+        exa.Grid(exa.elastic_solver(7, linear=True), 3)
+
+
+@pytest.mark.parametrize("order,n,steps", [(2, (4, 3, 2), 4), (4, (3, 2, 3), 3)])
+def test_linear_grid_transmissive_vs_oracle(exa, oracle, order, n, steps):
+    print(f"case: linear elastic grid transmissive vs oracle, N={order} {n} {steps} steps", flush=True)
+    nodes = np.array([oracle.table(oracle.T_NODES, order, i) for i in range(order + 1)])
+    # elastic_cells builds a cube; take a non-cubic grid's cells from a cube of the largest extent
+    m = max(n)
+    full = elastic_cells(order, m, seed=7, nodes=nodes, width=0.2).reshape(m, m, m, -1)
+    luh = np.ascontiguousarray(full[:n[2], :n[1], :n[0]]).reshape(-1)
+    g = exa.Grid(exa.elastic_solver(order, linear=True), n, boundary=exa.BC_TRANSMISSIVE)
+    g.upload(luh)
+    g.step(steps)
+    g.synchronize()
+    dts = g.dt_history(steps)
+    s = oracle.elastic_solver(order)
+    s.linear = 1
+    o = oracle.Grid(s, n, transmissive=True)
+    o.set_solution(luh)
+    odts, _ = o.step(steps, nthreads=0)
+    assert np.abs(dts - odts).max() <= 1e-13 * odts.max()
+    a, b = g.solution().reshape(-1, 12), o.solution().reshape(-1, 12)
+    err = max(np.abs(a[:, v] - b[:, v]).max() / max(np.abs(b[:, v]).max(), 1e-300) for v in range(12))
+    print(f"  max rel error per variable {err:.3e}", flush=True)
+    assert err <= 1e-12
+    g.close()
 
 
 @pytest.mark.parametrize("order,n,steps", [(2, 4, 5), (3, 3, 4), (4, 3, 3), (5, 2, 3)])

@@ -186,3 +186,41 @@ def test_point_source_projection_vs_oracle(exa, oracle, order):
     assert _rel(dev, ref) <= 1e-12
     assert np.array_equal(dev == 0.0, ref == 0.0)
     assert all(exa.lib().exa_quadrature_node(order, i) == oracle.table(oracle.T_NODES, order, i) for i in range(B))
+
+
+def test_linear_elastic_2d_golden_vectors_on_device(exa, oracle):
+    """ElasticityKernelTest2D.cpp testRiemannSolverLinear / testVolumeIntegralLinear /
+    testSurfaceIntegralLinear (order 4, non-zero coefficient matrix) through the
+    z-invariant 3-D embedding the oracle tests use, on the device, at the
+    reference test's eps 1e-9 (ElasticityKernelTest.h:69)."""
+    print("case: elastic 2-D golden vectors (embedded) on the device", flush=True)
+    EL = golden("elasticity_2d.json")
+    s = exa.elastic_solver(4)
+    B = 5
+    w = np.array([oracle.table(oracle.T_WEIGHTS, 4, i) for i in range(B)])
+    g = EL["testRiemannSolverLinear"]
+    q2L = np.concatenate([np.array(g["qL_IN"]).reshape(B, 9), np.array(g["paramL_IN"]).reshape(B, 3)], 1)
+    q2R = np.concatenate([np.array(g["qR_IN"]).reshape(B, 9), np.array(g["paramR_IN"]).reshape(B, 3)], 1)
+    QL, QR = np.broadcast_to(q2L, (B, B, 12)).ravel(), np.broadcast_to(q2R, (B, B, 12)).ravel()
+    FL, FR = exa.riemann_solver_linear(s, QL, QR, 1.916666666666667e-4, 1)
+    for got, key in ((FL, "FL_OUT"), (FR, "FR_OUT")):
+        ref = np.array(g[key]).reshape(B, 9)
+        for i in range(B):
+            assert np.abs(got.reshape(B, B, 9)[i] - ref).max() <= 1e-9, key
+    g = EL["testVolumeIntegralLinear"]
+    l2 = np.array(g["lFhi_IN"]).reshape(2, B, B, 9)
+    lFhi = np.zeros((4, B, B, B, 9))
+    lFhi[0], lFhi[1] = l2[0][None], l2[1][None]
+    lduh = exa.volume_integral_linear(s, lFhi.ravel()).reshape(B, B, B, 9)
+    ref = np.array(g["lduh_OUT"]).reshape(B, B, 9)
+    for i in range(B):
+        assert np.abs(lduh[i] / w[i] - ref).max() <= 1e-12 * np.abs(ref).max()
+    g = EL["testSurfaceIntegralLinear"]
+    F2 = np.array(g["lFbnd_IN"]).reshape(4, B, 9)
+    F = np.zeros((6, B, B, 9))
+    F[0], F[1], F[2], F[3] = F2[0][None], F2[1][None], F2[2][None], F2[3][None]
+    lduh_in = w[:, None, None, None] * np.array(g["lduh_IN"]).reshape(B, B, 9)[None]
+    out = exa.surface_integral_linear(s, lduh_in.ravel(), F.ravel(), (38.4615384615385, 35.7142857142857, 1.0))
+    ref = np.array(g["lduh_OUT"]).reshape(B, B, 9)
+    for i in range(B):
+        assert np.abs(out.reshape(B, B, B, 9)[i] / w[i] - ref).max() <= 1e-12 * np.abs(ref).max()

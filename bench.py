@@ -514,6 +514,38 @@ def bench_batch(capi, args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_baseline_elastic(seconds, threads=16):
+    """The oracle's grid driver (nonlinear path with the NCP, and the linear
+    kernels) on a periodic 6^3 sample of the elastic workload at N=4, OpenMP
+    threads; steps until about seconds/2 per form."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refutil import elastic_cells
+    order, m = 4, 6
+    B = order + 1
+    nodes = np.array([O.table(O.T_NODES, order, i) for i in range(B)])
+    luh = elastic_cells(order, m, seed=2024, nodes=nodes, width=0.05)
+    out = {}
+    for linear in (False, True):
+        s = O.elastic_solver(order)
+        s.linear = int(linear)
+        g = O.Grid(s, m)
+        g.set_solution(luh)
+        g.step(1, nthreads=threads)  # the initial prediction
+        steps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds / 2:
+            g.step(1, nthreads=threads)
+            steps += 1
+        el = time.perf_counter() - t0
+        out["linear" if linear else "nonlinear"] = (m ** 3 * B ** 3 * steps / el, steps, el)
+    v, st, el = out["nonlinear"]
+    return dict(value=v, unit="DOF-updates/s", cores=threads, kind="port",
+                linear_value=out["linear"][0],
+                sample=f"{st} steps of the oracle's grid driver (nonlinear, NCP) on a periodic {m}^3 sample of the "
+                       f"elastic workload at N={order}, OpenMP {threads} threads, {el:.1f} s; the linear kernels "
+                       f"{out['linear'][1]} steps in {out['linear'][2]:.1f} s (linear_value)")
+
+
 def bench_elastic(capi, args):
     """Elastic workload: velocity-stress NCP form, N=4, 81^3, Gaussian pulse
     of width 0.05 in a seeded two-layer material; FP32-bulk vs FP64, and the
@@ -576,7 +608,7 @@ def bench_elastic(capi, args):
                         "l2_rel_diff_vs_nonlinear_fp64": float(np.linalg.norm(res["linear"]["sol"] - b)
                                                                / np.linalg.norm(b))},
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_elastic(args.cpu_seconds),
     }
     print(json.dumps(out), flush=True)
 

@@ -514,6 +514,20 @@ def bench_batch(capi, args):
     print(json.dumps(out), flush=True)
 
 
+LINEAR_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "linear_elastic_n4_81cube_pmc_traffic.txt")
+
+
+def linear_predictor_traffic():
+    """HBM bytes per linear-predictor launch on the elastic workload (rocprofv3
+    --pmc, copy-calibrated); None otherwise."""
+    if not os.path.exists(LINEAR_TRAFFIC_FILE):
+        return None
+    for line in open(LINEAR_TRAFFIC_FILE):
+        if line.startswith("| linear predictor |"):
+            return int(line.split("|")[4].strip().replace(",", ""))
+    return None
+
+
 def cpu_baseline_elastic(seconds, threads=16):
     """The oracle's grid driver (nonlinear path with the NCP, and the linear
     kernels) on a periodic 6^3 sample of the elastic workload at N=4, OpenMP
@@ -544,6 +558,21 @@ def cpu_baseline_elastic(seconds, threads=16):
                 sample=f"{st} steps of the oracle's grid driver (nonlinear, NCP) on a periodic {m}^3 sample of the "
                        f"elastic workload at N={order}, OpenMP {threads} threads, {el:.1f} s; the linear kernels "
                        f"{out['linear'][1]} steps in {out['linear'][2]:.1f} s (linear_value)")
+
+
+def linear_roofline(stages, n, B):
+    """Roofline of the linear predictor (the linear form's dominant kernel): it
+    reads luh (B^3 x 12 doubles) and writes lduh (B^3 x 9) and both face arrays
+    (6 B^2 x (12 + 9)) per cell; launch time from the grid's HIP events."""
+    ms, launches = stages["predictor"]
+    avg = ms / max(launches, 1)
+    alg = n ** 3 * (B ** 3 * 12 + B ** 3 * 9 + 6 * B * B * 21) * 8
+    achieved = alg / (avg * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=linear_predictor_traffic(),
+                kernel="linear_predictor_kernel<5, Elastic> (Cauchy-Kowalevski + linear volume integral)",
+                avg_launch_ms=round(avg, 4), algorithmic_bytes_per_launch=alg,
+                note="latency-bound on LDS (neither HBM nor ALU saturated)")
 
 
 def bench_elastic(capi, args):
@@ -606,7 +635,8 @@ def bench_elastic(capi, args):
                         "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4)
                                               for k, v in res["linear"]["stages"].items()},
                         "l2_rel_diff_vs_nonlinear_fp64": float(np.linalg.norm(res["linear"]["sol"] - b)
-                                                               / np.linalg.norm(b))},
+                                                               / np.linalg.norm(b)),
+                        "roofline": linear_roofline(res["linear"]["stages"], n, B)},
         "roofline": None,
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_elastic(args.cpu_seconds),
     }

@@ -45,20 +45,40 @@ def parse_resource_remarks(text):
     return kernels
 
 
+def _compile(src, obj):
+    cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + [
+        "-c", "-Rpass-analysis=kernel-resource-usage", "-o", obj, src]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    return src, cmd, res
+
+
 def build(verbose=False):
-    """Compile for gfx950.  Fails if any kernel spills to scratch memory: a spill
-    stored inside a divergent region saves only the active lanes, so kernels
-    must stay within their register budget."""
+    """Compile for gfx950, one object per source in parallel (every device
+    symbol is local to its source file), then link.  Fails if any kernel
+    spills to scratch memory: a spill stored inside a divergent region saves
+    only the active lanes, so kernels must stay within their register budget."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
     srcs = [os.path.join(HERE, s) for s in SOURCES]
+    objs = [os.path.join(objdir, os.path.basename(s)[:-4] + ".o") for s in SOURCES]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    remarks = []
+    with ThreadPoolExecutor(jobs) as ex:
+        for src, cmd, res in ex.map(lambda so: _compile(*so), zip(srcs, objs)):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            if res.returncode != 0:
+                sys.stderr.write(res.stdout + res.stderr)
+                raise RuntimeError(f"hipcc failed on {os.path.basename(src)}")
+            remarks.append(res.stderr)
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-o", tmp] + srcs
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libexahype_amd.so")
-    kernels = parse_resource_remarks(res.stderr)
+        raise RuntimeError("linking libexahype_amd.so failed")
+    kernels = parse_resource_remarks("\n".join(remarks))
     with open(RESOURCES, "w") as f:
         for k in sorted(kernels):
             r = kernels[k]

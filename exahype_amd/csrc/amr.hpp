@@ -15,7 +15,11 @@ constexpr int kAmrMaxLevelDelta = 6;  // 3^6 = 729 subintervals per direction
 // destination overwritten.  Restriction: one workgroup per destination k over
 // items [ptr[k], ptr[k+1]), which must share one destination block; their
 // restricted blocks are added to it in list order.  Items with an index out of
-// range (or a mismatched destination) are skipped and set *error.
+// range (or a mismatched destination) are skipped and set *error.  Every
+// destination block may appear once per launch (one item of a prolongation,
+// one range of a restriction): each workgroup claims its destination's bit in
+// `claimed` (dst_blocks bits, zero on entry) before it writes, and a workgroup
+// that finds the bit taken writes nothing and sets *error.
 struct AmrArgs {
   double* dst;
   const double* src;
@@ -25,6 +29,7 @@ struct AmrArgs {
   long nitems;     // items in the list (bounds of the ptr ranges)
   const int* ptr;
   int* error;
+  unsigned* claimed;  // one bit per destination block
   int nv;
   int order;
   int level_delta;

@@ -32,10 +32,14 @@ bool amr_solver(const exa_solver* s, AmrSizes* z) {
 // Runs one launch on the null stream and waits for it; optionally times it
 // with HIP events on that stream.  Items the kernel rejected -> EXA_E_INVALID.
 int run_amr(exa::AmrArgs a, int ndim, bool restrict_, long ngroups, float* elapsed_ms) {
-  DevBuf d_err;
+  DevBuf d_err, d_claim;
   EXA_TRY(d_err.alloc(sizeof(int)));
   EXA_TRY(hipMemset(d_err.p, 0, sizeof(int)));
   a.error = static_cast<int*>(d_err.p);
+  const size_t claim_bytes = sizeof(unsigned) * (size_t)((a.dst_blocks + 31) / 32);
+  EXA_TRY(d_claim.alloc(claim_bytes));
+  EXA_TRY(hipMemset(d_claim.p, 0, claim_bytes ? claim_bytes : 8));
+  a.claimed = static_cast<unsigned*>(d_claim.p);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (elapsed_ms) {
     EXA_TRY(hipEventCreate(&e0));

@@ -62,6 +62,16 @@ __global__ __launch_bounds__(round_up(NDIM == 3 ? B * B * B : B * B, kWave)) voi
       if (tid == 0) atomicOr(a.error, 1);
       continue;
     }
+    if (!RESTRICT || dst_blk < 0) {
+      // claim the destination block: a second workgroup naming it writes nothing
+      __shared__ unsigned prev;
+      if (tid == 0) prev = atomicOr(a.claimed + (db >> 5), 1u << (db & 31));
+      __syncthreads();
+      if (prev & (1u << (db & 31))) {
+        if (tid == 0) atomicOr(a.error, 1);
+        return;  // uniform: every thread read the same prev
+      }
+    }
     if (RESTRICT && dst_blk < 0) {
       dst_blk = db;
       const double* dst = a.dst + db * a.dst_block;

@@ -50,8 +50,8 @@ static int predictor_impl(const exa_solver* s, long ncells, double* lQhbnd, doub
                           const double* luh, const double* dx, double dt, int* iters) {
   SolverDesc sd;
   if (!to_desc(s, &sd) || ncells < 1 || !lQhbnd || !lFhbnd || !luh || !dx) return EXA_E_INVALID;
-  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
-  if (!exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
+  // a linear solver's glue calls the linear kernels (spaceTimePredictorLinear)
+  if (sd.linear || !exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   const bool exp = lQi || lFi || lQhi || lFhi;
   DevBuf d_luh, d_qb, d_fb, d_du, d_dx, d_dt, d_it, d_qi, d_fi, d_qhi, d_fhi;
@@ -125,8 +125,7 @@ extern "C" int exa_volume_integral_nonlinear(const exa_solver* s, double* lduh, 
                                              const double* dx) {
   SolverDesc sd;
   if (!to_desc(s, &sd) || !lduh || !lFhi || !dx) return EXA_E_INVALID;
-  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
-  if (!exa::supported_sizes(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
+  if (sd.linear || !exa::supported_sizes(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   DevBuf d_du, d_fhi, d_dx, d_fi;
   EXA_TRY(d_du.alloc(sizeof(double) * z.B3 * z.V));
@@ -152,9 +151,12 @@ static int update_impl(const exa_solver* s, int mode, double* luh, double* lduh,
                        const double* dx, double dt, double* dt_out) {
   SolverDesc sd;
   if (!to_desc(s, &sd)) return EXA_E_INVALID;
-  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
-  if (sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const bool surface_only = mode == (exa::kUpdSurface | exa::kUpdWriteLduh);
+  // solutionUpdate and stableTimeStepSize serve linear and nonlinear solvers
+  // alike; a linear solver's surface integral is surfaceIntegralLinear
+  if (surface_only && sd.linear) return EXA_E_UNSUPPORTED;
+  sd.linear = 0;
+  if (sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   if (surface_only ? !exa::supported_sizes(sd) : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
   if (surface_only) {  // PDE-independent: run the Euler instantiation of this order
     sd.pde = exa::kPdeEuler;
@@ -219,8 +221,7 @@ extern "C" int exa_riemann_solver_nonlinear(const exa_solver* s, double* FL, dou
   (void)dt;  // unused by the nonlinear Rusanov solver, as in the reference
   SolverDesc sd;
   if (!to_desc(s, &sd) || !FL || !FR || !QL || !QR || direction < 0 || direction > 2) return EXA_E_INVALID;
-  sd.linear = 0;  // the nonlinear kernels, whatever the solver's flag
-  if (!exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
+  if (sd.linear || !exa::supported(sd) || sd.precision != exa::kFP64) return EXA_E_UNSUPPORTED;
   const Sizes z(sd);
   // Two cells: cell 0 owns face 2d+1 (left state), cell 1 owns face 2d (right state).
   DevBuf d_qb, d_fb, d_face;

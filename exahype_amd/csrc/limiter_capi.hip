@@ -256,7 +256,8 @@ extern "C" int exa_lgrid_create(const exa_solver* s, int nx, int ny, int nz, con
     return EXA_E_INVALID;
   if (boundary != EXA_BC_PERIODIC && boundary != EXA_BC_TRANSMISSIVE) return EXA_E_INVALID;
   if (fv_scheme != EXA_FV_GODUNOV && fv_scheme != EXA_FV_MUSCLHANCOCK) return EXA_E_INVALID;
-  if (!exa::limiter_supported(sd) || !exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  // the limiting ADER-DG solver runs the nonlinear kernels only
+  if (sd.linear || !exa::limiter_supported(sd) || !exa::supported(sd)) return EXA_E_UNSUPPORTED;
   const long ncells = (long)nx * ny * nz;
   if (ncells > 0x7fffffffL) return EXA_E_INVALID;
   EXA_TRY(exa::ensure_limiter_tables());

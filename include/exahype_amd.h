@@ -189,8 +189,12 @@ int exa_copy_to_host(void* dst, const void* src, size_t bytes);
  * overwritten.  Restriction: ndst destinations; ptr (device, ndst + 1 ints)
  * gives each destination's items [ptr[k], ptr[k+1]) of the nitems, all with
  * that destination block; their restricted blocks are added to it in list
- * order, as successive single calls would.  The calls block until done; items
- * with an index out of range are skipped and the call returns EXA_E_INVALID.
+ * order, as successive single calls would.  Each destination block may appear
+ * once per call (in one prolongation item, or in one restriction range); a
+ * repeated destination is written by one of its transfers only, the others are
+ * skipped, and the call returns EXA_E_INVALID.  The calls block until done;
+ * items with an index out of range are skipped and the call returns
+ * EXA_E_INVALID.
  * elapsed_ms (optional): the kernels' time from HIP events on their stream. */
 int exa_volume_unknowns_prolongation_batch(const exa_solver* s, double* luhFine, long nfine,
                                            const double* luhCoarse, long ncoarse, const int* items,

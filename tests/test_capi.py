@@ -70,6 +70,33 @@ def test_fp32_bulk_is_grid_only(exa):
         exa.space_time_predictor_nonlinear(s, np.ones(64 * 5), 0.1, 1e-3)
 
 
+def test_linear_solver_rejected_by_nonlinear_only_entry_points(exa):
+    """A solver with the linear flag has no nonlinear predictor, volume or
+    surface integral, Riemann solver or limiting grid: those entry points
+    return EXA_E_UNSUPPORTED (2) before any device work, instead of running the
+    Picard / Rusanov kernels for it.  solutionUpdate and stableTimeStepSize are
+    shared by both kinds of solver."""
+    import numpy as np
+    L = exa.lib()
+    s = exa.elastic_solver(4, linear=True)
+    B, D, V = 5, 12, 9
+    luh, dx = np.ones(B ** 3 * D), np.full(3, 0.1)
+    qb, fb = np.zeros(6 * B * B * D), np.zeros(6 * B * B * V)
+    P = ctypes.POINTER(ctypes.c_double)
+    dp = lambda a: a.ctypes.data_as(P)
+    assert L.exa_space_time_predictor_nonlinear(ctypes.byref(s), dp(qb), dp(fb), None, None, None, None, dp(luh),
+                                                dp(dx), 1e-3, None) == 2
+    lduh = np.zeros(B ** 3 * V)
+    assert L.exa_volume_integral_nonlinear(ctypes.byref(s), dp(lduh), dp(np.zeros(4 * B ** 3 * V)), dp(dx)) == 2
+    assert L.exa_surface_integral_nonlinear(ctypes.byref(s), dp(lduh), dp(fb), dp(dx)) == 2
+    fl, fr, ql, qr = np.zeros(B * B * V), np.zeros(B * B * V), np.zeros(B * B * D), np.zeros(B * B * D)
+    assert L.exa_riemann_solver_nonlinear(ctypes.byref(s), dp(fl), dp(fr), dp(ql), dp(qr), 1e-3, 0) == 2
+    h = ctypes.c_void_p()
+    lo = (ctypes.c_double * 3)(0, 0, 0)
+    hi = (ctypes.c_double * 3)(1, 1, 1)
+    assert L.exa_lgrid_create(ctypes.byref(s), 4, 4, 4, lo, hi, 1, 1, 0, 1e-4, 1e-3, 0, ctypes.byref(h)) == 2
+
+
 GLUE = r'''
 #include <exahype_amd/kernels.hpp>
 #include <vector>

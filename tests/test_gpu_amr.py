@@ -164,3 +164,36 @@ def test_batch_rejects_out_of_range_items(exa):
     good = exa.DeviceArray(np.array([(0, 0, 0, 0, 0)], dtype=np.int32))
     with pytest.raises(exa.ExaError):
         exa.volume_restriction_batch(s, d_c, d_f, bad_ptr, good, 1)
+
+
+def test_batch_rejects_repeated_destinations(exa, oracle):
+    """A destination block named twice in one call (two prolongation items, or
+    two restriction ranges) is written by one transfer only and the call fails."""
+    print("case: repeated AMR destinations", flush=True)
+    s, os_ = _solvers(exa, oracle, 2)
+    blk = 27 * 6
+    rng = np.random.default_rng(5)
+    coarse = rng.standard_normal((2, blk))
+    d_c, d_f = exa.DeviceArray(coarse), exa.DeviceArray(np.zeros((3, blk)))
+    dup = exa.DeviceArray(np.array([(1, 0, 0, 0, 0), (2, 1, 1, 1, 1), (1, 1, 2, 0, 1)], dtype=np.int32))
+    with pytest.raises(exa.ExaError):
+        exa.volume_prolongation_batch(s, d_f, d_c, dup, 1)
+    fine = d_f.get()
+    assert np.array_equal(fine[0], np.zeros(blk))  # named by no item
+    ref2 = oracle.volume_unknowns_prolongation(os_, coarse[1], 0, 1, (1, 1, 1))
+    assert _rel(fine[2], ref2) <= 1e-12  # the unique destination is written
+    cands = [oracle.volume_unknowns_prolongation(os_, coarse[0], 0, 1, (0, 0, 0)),
+             oracle.volume_unknowns_prolongation(os_, coarse[1], 0, 1, (2, 0, 1))]
+    assert min(_rel(fine[1], c) for c in cands) <= 1e-12  # exactly one of the two transfers landed
+    # restriction: two ranges with the same destination
+    start = rng.standard_normal((2, blk))
+    d_acc = exa.DeviceArray(start)
+    ritems = exa.DeviceArray(np.array([(0, 0, 0, 0, 0), (0, 1, 1, 0, 0), (1, 2, 2, 2, 2)], dtype=np.int32))
+    rptr = exa.DeviceArray(np.array([0, 1, 2, 3], dtype=np.int32))
+    d_fin = exa.DeviceArray(rng.standard_normal((3, blk)))
+    with pytest.raises(exa.ExaError):
+        exa.volume_restriction_batch(s, d_acc, d_fin, rptr, ritems, 1)
+    # the same ranges with distinct destinations are accepted
+    ok_items = exa.DeviceArray(np.array([(0, 0, 0, 0, 0), (0, 1, 1, 0, 0), (1, 2, 2, 2, 2)], dtype=np.int32))
+    ok_ptr = exa.DeviceArray(np.array([0, 2, 3], dtype=np.int32))
+    exa.volume_restriction_batch(s, exa.DeviceArray(start), d_fin, ok_ptr, ok_items, 1)

@@ -97,6 +97,17 @@ def test_batch_predictor_volume_vs_oracle(exa, oracle, order):
         assert rel_max(r["lduh"][c * nb3:(c + 1) * nb3], lduh) < TOL, c
 
 
+def test_golden_volume_integral(exa):
+    """testVolumeIntegralNonlinear::lduh (GenericEulerKernelTest3D.cpp:204-215
+    inputs, dx = 0.05: the mesh size the golden data was made with)."""
+    print("case: golden volume integral", flush=True)
+    lFhi = np.zeros(1280)
+    for i in range(0, 320, 5):
+        lFhi[i + 1] = lFhi[320 + i + 2] = lFhi[640 + i + 3] = 1.0
+    out = exa.volume_integral_nonlinear(mock(exa), lFhi, np.full(3, 0.05))
+    assert np.abs(out - np.array(G["testVolumeIntegralNonlinear"]["lduh"])).max() < EPS
+
+
 @pytest.mark.parametrize("order", [2, 3, 5, 7])
 def test_single_cell_kernels_vs_oracle(exa, oracle, order):
     rng = np.random.default_rng(7 + order)
@@ -162,3 +173,16 @@ def test_no_time_averaging_vs_oracle(exa, oracle, order):
         assert rel_max(r[k], o[k]) < TOL, k
     lFi = o["lFi"]
     assert rel_max(exa.volume_integral_nonlinear(s, lFi, dx), oracle.volume_integral(so, o["lFhi"], dx, lFi=lFi)) < TOL
+
+
+@pytest.mark.parametrize("order,seed,cfl", [(3, 1, 0.05), (5, 4, 0.07), (6, 6, 0.04)])
+def test_picard_iterates_in_closed_form(exa, order, seed, cfl):
+    """The device predictor on entropy-wave data, where every Picard iterate
+    is known in closed form (tests/test_oracle_picard.py): the exit
+    iteration, the last iterate in lQi/lQhi/lQhbnd and the one-iterate-stale
+    fluxes in lFi/lFhi/lFhbnd."""
+    print(f"case: Picard closed form N={order} seed={seed}", flush=True)
+    from test_oracle_picard import check_against_closed_form
+    s = exa.euler_solver(order)
+    check_against_closed_form(lambda luh, h, dt: exa.space_time_predictor_nonlinear(s, luh, h, dt, temps=True),
+                              order, seed, cfl)

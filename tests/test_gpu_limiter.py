@@ -172,3 +172,57 @@ def test_limiter_workload_full_size_invariants(exa):
     print(f"  FV-active fraction {frac:.4f}, troubled {(st == 3).mean():.4f}, irregular steps {irr}", flush=True)
     assert 0.0 < frac < 0.5
     g.close()
+
+
+@pytest.mark.parametrize("helper_layers,cure", [(1, 0), (2, 1), (1, 2)])
+def test_limiter_status_machine_injected_cell_on_device(exa, helper_layers, cure):
+    """The hand case of tests/test_oracle_limiter.py on the device: one cell
+    with negative density at one node on a uniform periodic grid; the
+    statuses after set-up and after every step in closed form."""
+    print(f"case: injected cell status machine h={helper_layers} cure={cure}", flush=True)
+    from test_oracle_limiter import expected_injected_statuses, injected_cell_case
+    order, n = 3, 7
+    c0 = (3 * n + 2) * n + 4
+    luh, lim_ic = injected_cell_case(order, n, c0)
+    g = exa.LimitingGrid(exa.euler_solver(order), n, boundary=exa.BC_PERIODIC, helper_layers=helper_layers,
+                         steps_till_cured=cure)
+    g.upload(luh, lim_ic)
+    st, irr = g.status()
+    assert np.array_equal(st, expected_injected_statuses(n, c0, helper_layers, cure, 0))
+    T = 2 * helper_layers + 1
+    for step in range(1, T + cure + 2):
+        g.step(1)
+        st, irr = g.status()
+        assert np.array_equal(st, expected_injected_statuses(n, c0, helper_layers, cure, step)), step
+    assert irr == 0
+    sol = g.solution().reshape(-1, 5)
+    assert np.abs(sol[:, 0] - 1.0).max() < 1e-12 and np.abs(sol[:, 1:4]).max() < 1e-12
+    g.close()
+
+
+@pytest.mark.timeout(600)
+def test_limiting_grid_vs_oracle_mid_size(exa, oracle):
+    """Statuses bit-exact against the oracle on a 16^3 transmissive blast over
+    20 steps (N=3), with the dt of every step and the solution at the end
+    (the oracle's limiting driver takes about 40 s of one host core here)."""
+    order, n, steps = 3, 16, 20
+    print(f"case: limiting grid vs oracle, N={order} {n}^3 blast {steps} steps", flush=True)
+    g, o = _grid_pair(exa, oracle, order, n, False, 0.1, steps)
+    assert np.array_equal(g.status()[0], o.status())
+    irr_prev = 0
+    for k in range(steps):
+        assert abs(g.time()[1] - o.dt()) <= 1e-13 * o.dt(), f"dt of step {k}"
+        g.step(1)
+        ost, _, oirr = o.step(1)
+        st, irr = g.status()
+        assert np.array_equal(st, ost[0]), f"limiter status after step {k}: {np.nonzero(st != ost[0])[0][:10]}"
+        assert irr - irr_prev == int(oirr[0]), f"irregular change in step {k}"
+        irr_prev = irr
+        if k % 5 == 4:
+            print(f"  step {k + 1}: statuses identical", flush=True)
+    sol, osol = g.solution().reshape(-1, 5), o.solution().reshape(-1, 5)
+    for v in range(5):
+        assert rel_max(sol[:, v], osol[:, v]) <= 1e-11, v
+    print(f"  FV-active {(st >= 2).mean():.3f}, irregular steps {irr_prev}", flush=True)
+    assert irr_prev > 0 and (st >= 2).any()
+    g.close()

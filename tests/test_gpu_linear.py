@@ -224,3 +224,21 @@ def test_linear_elastic_2d_golden_vectors_on_device(exa, oracle):
     ref = np.array(g["lduh_OUT"]).reshape(B, B, 9)
     for i in range(B):
         assert np.abs(out.reshape(B, B, B, 9)[i] / w[i] - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_linear_elastic_2d_predictor_golden_on_device(exa, oracle):
+    """ElasticityKernelTest2D.cpp testSpaceTimePredictorLinear (order 4, NCP
+    only) through the z-invariant 3-D embedding, on the device: lQhi, lFhi and
+    both face arrays at every z slice, at the reference test's eps 1e-9 and
+    1e-9 of each array's scale."""
+    print("case: elastic 2-D predictor golden (embedded) on the device", flush=True)
+    from test_oracle_linear import _check_stp_linear_elastic_golden
+    g = golden("elasticity_2d.json")["testSpaceTimePredictorLinear"]
+    s = exa.elastic_solver(4)
+    B = 5
+    luh2 = np.concatenate([np.array(g["luh_IN"]).reshape(B, B, 9), np.array(g["param_IN"]).reshape(B, B, 3)], 2)
+    luh = np.broadcast_to(luh2, (B, B, B, 12)).copy()
+    dx = (38.4615384615385, 35.7142857142857, 1.0)
+    dt = 0.813172798364530
+    o = exa.space_time_predictor_linear(s, luh.ravel(), dx, dt, temps=True)
+    _check_stp_linear_elastic_golden(g, o["lQhi"], o["lFhi"], o["lQhbnd"], o["lFhbnd"])

@@ -46,6 +46,28 @@ def test_surface_integral_golden(oracle):
     assert np.abs(out - G["testSurfaceIntegralNonlinear"]["lduh"]).max() < EPS
 
 
+def volume_integral_golden_inputs():
+    """GenericEulerKernelTest3D.cpp:204-215: lFhi = [x | y | z | src] blocks of
+    64 nodes x 5 variables; variable d+1 of block d set to 1, source zero."""
+    lFhi = np.zeros(1280)
+    for i in range(0, 320, 5):
+        lFhi[i + 1] = lFhi[320 + i + 2] = lFhi[640 + i + 3] = 1.0
+    return lFhi
+
+
+def test_volume_integral_golden(oracle):
+    """testVolumeIntegralNonlinear::lduh (generic_euler_testdata.cpp).  The
+    reference test body is commented out (GenericEulerKernelTest3D.cpp:217-225)
+    and names no mesh size; the golden data matches the seeded lFhi with
+    dx = 0.05 to 8e-13, and any other dx misses by O(1), which pins the 1/dx
+    scaling as well as the Kxi orientation and the y/z block index orders."""
+    s = mock_solver(oracle)
+    lFhi = volume_integral_golden_inputs()
+    ref = np.array(G["testVolumeIntegralNonlinear"]["lduh"])
+    assert np.abs(oracle.volume_integral(s, lFhi, 0.05) - ref).max() < EPS
+    assert np.abs(oracle.volume_integral(s, lFhi, 0.1) - ref).max() > 1.0
+
+
 def riemann_golden_inputs():
     QL = np.zeros(96)
     QR = np.zeros(96)

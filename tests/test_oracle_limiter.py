@@ -337,3 +337,57 @@ def test_limiting_driver_blast_invariants(oracle):
         for ax in range(3):
             assert np.abs(S3 - np.roll(S3, 1, axis=ax)).max() <= 1
     assert seen_troubled and seen_irregular
+
+
+def injected_cell_case(order, n, c0):
+    """A uniform state (rho=1, u=0, p=1) on an n^3 periodic grid in which cell
+    c0's DG solution has rho = -0.01 at one Gauss-Legendre node; every FV
+    patch starts uniform (the FV initial condition at subcell centres)."""
+    B, L = order + 1, 2 * order + 1
+    q = np.array([1.0, 0.0, 0.0, 0.0, 1.0 / (GAMMA - 1.0)])
+    luh = np.tile(q, (n ** 3, B ** 3, 1))
+    luh[c0, 5, 0] = -0.01
+    lim_ic = np.tile(q, (n ** 3, L ** 3, 1))
+    return luh.reshape(-1), lim_ic.reshape(-1)
+
+
+def expected_injected_statuses(n, c0, helper_layers, cure, step):
+    """Closed form of the reference's status machine for injected_cell_case
+    (ADERDGSolver.cpp:461-462, 2544-2596; LimitingADERDGSolver.cpp:955-994,
+    1031-1043): at set-up c0 is not admissible, so it is troubled (T = 2h+1)
+    and spreading gives every cell max(0, T - d), d its periodic face-graph
+    distance to c0.  Every later solution is uniform and admissible, so no
+    cell becomes troubled again: c0 keeps T while its cure counter (cure + 1
+    at set-up) runs down, then takes T - 1 and, like every other cell,
+    max over its faces of max(own, neighbour) - 1, one less per step.
+    Together: status = max(0, min(T - d, T - max(0, step - cure)))."""
+    T = 2 * helper_layers + 1
+    k = np.arange(n)
+    z, y, x = np.meshgrid(k, k, k, indexing="ij")
+    cz, cy, cx = c0 // (n * n), (c0 // n) % n, c0 % n
+    per = lambda a, b: np.minimum(np.abs(a - b), n - np.abs(a - b))
+    d = (per(x, cx) + per(y, cy) + per(z, cz)).ravel()
+    return np.maximum(0, np.minimum(T - d, T - max(0, step - cure)))
+
+
+@pytest.mark.parametrize("helper_layers,cure", [(1, 0), (2, 1), (1, 2)])
+def test_limiter_status_machine_injected_cell(oracle, helper_layers, cure):
+    """Hand case: one cell with negative density at one node on a uniform
+    grid; statuses after set-up and after each step in closed form, no
+    irregular change (nothing becomes newly troubled), the state stays
+    uniform."""
+    order, n = 3, 7
+    c0 = (3 * n + 2) * n + 4
+    luh, lim_ic = injected_cell_case(order, n, c0)
+    g = oracle.LimitingGrid(oracle.make_solver(order), n, periodic=True, helper_layers=helper_layers,
+                            steps_till_cured=cure)
+    g.set_solution(luh, lim_ic)
+    assert np.array_equal(g.status(), expected_injected_statuses(n, c0, helper_layers, cure, 0))
+    T = 2 * helper_layers + 1
+    for step in range(1, T + cure + 2):
+        st, _, irr = g.step(1)
+        assert not irr[0]
+        assert np.array_equal(st[0], expected_injected_statuses(n, c0, helper_layers, cure, step)), step
+    assert (g.status() == 0).all()
+    sol = g.solution().reshape(-1, 5)
+    assert np.abs(sol[:, 0] - 1.0).max() < 1e-12 and np.abs(sol[:, 1:4]).max() < 1e-12

@@ -139,6 +139,50 @@ def test_surface_integral_linear_elastic_golden_embedded(oracle):
     for i in range(B):
         assert np.abs(out[i] / w[i] - ref).max() <= 1e-12 * scale
 
+def _check_stp_linear_elastic_golden(g, lQhi, lFhi, lQhbnd, lFhbnd):
+    """Every z slice of the 3-D outputs against the 2-D golden: lQhi (y,x,var),
+    lFhi x block (y,x) and y block (x,y), face blocks 0-3 (x faces by y, y
+    faces by x).  Bar: the reference test's absolute eps 1e-9, and 1e-9 of
+    each array's scale (the outputs are ~1e-4, so the absolute bar alone
+    would let a dropped Taylor term through)."""
+    B = 5
+    checks = [(lQhi.reshape(B, B, B, 12)[..., :9], np.array(g["lQhi_OUT"]).reshape(B, B, 9), "lQhi")]
+    F = lFhi.reshape(4, B, B, B, 9)
+    rF = np.array(g["lFhi_OUT"]).reshape(2, B, B, 9)
+    checks += [(F[0], rF[0], "lFhi_x"), (F[1], rF[1], "lFhi_y")]
+    Q = lQhbnd.reshape(6, B, B, 12)[..., :9]
+    Fb = lFhbnd.reshape(6, B, B, 9)
+    rQ, rFb = np.array(g["lQbnd_OUT"]).reshape(4, B, 9), np.array(g["lFbnd_OUT"]).reshape(4, B, 9)
+    for f in range(4):
+        checks += [(Q[f], rQ[f], f"lQbnd{f}"), (Fb[f], rFb[f], f"lFbnd{f}")]
+    for got, ref, key in checks:
+        scale = np.abs(ref).max()
+        assert scale > 0, key
+        for z in range(B):
+            err = np.abs(got[z] - ref).max()
+            assert err <= EPS_ELASTIC and err <= 1e-9 * scale, (key, z, err / scale)
+
+
+def test_space_time_predictor_linear_elastic_golden_embedded(oracle):
+    """testSpaceTimePredictorLinear (ElasticityKernelTest2D.cpp:315-420; order
+    4, spaceTimePredictorLinear<false,false,false,true>, i.e. NCP only; the
+    reference flags the test as needing a fix and does not run it).  The
+    z-invariant embedding of luh (constant along z, so every z derivative
+    vanishes up to the tables' rounding) reproduces the 2-D time-averaged
+    state, time-averaged fluxes and face values at every z slice: this pins
+    the Cauchy-Kowalevski recursion, the Taylor factors dt^i/(i+1)!, the
+    directional B_d grad_d Q with non-constant material and the face
+    extrapolation on data that is not a plane wave."""
+    s = _elastic(4)
+    g = EL["testSpaceTimePredictorLinear"]
+    B = 5
+    luh2 = np.concatenate([np.array(g["luh_IN"]).reshape(B, B, 9), np.array(g["param_IN"]).reshape(B, B, 3)], 2)
+    luh = np.broadcast_to(luh2, (B, B, B, 12)).copy()
+    dx = (38.4615384615385, 35.7142857142857, 1.0)
+    dt = 0.813172798364530
+    o = oracle.space_time_predictor_linear(s, luh, dx, dt)
+    _check_stp_linear_elastic_golden(g, o["lQhi"], o["lFhi"], o["lQhbnd"], o["lFhbnd"])
+
 
 def test_coefficient_matrix_matches_directional_ncp(oracle):
     """Bn[l][k] = B_d(k, l): applying Bn^T to a gradient gives the directional NCP."""

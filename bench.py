@@ -66,10 +66,12 @@ def cpu_baseline(exa_grid, order, n, seconds, threads):
                        f"{elapsed:.2f} s")
 
 
-def run_form(capi, order, n, steps, warmup, seed, precision):
+def run_form(capi, order, n, steps, warmup, seed, precision, variant=0):
     """One timed run of the stepping loop; returns timings and the final solution."""
     s = capi.euler_solver(order, precision=precision)
     g = capi.Grid(s, n)
+    if variant:
+        g.set_predictor_variant(variant)
     g.init_euler(rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=8, mode_amp=0.01,
                  seed=seed)
     g.step(warmup)
@@ -478,7 +480,7 @@ def bench_batch(capi, args):
         if not all(np.isfinite(dt) and dt > 0 for _, dt in times):
             raise RuntimeError(f"{form}: non-finite time step in the patch set")
         sols = {j: ps.solution(j) for j, sp in enumerate(specs) if sp[0] ** 3 <= keep}
-        variants = {o: ps.lds_variant(o) for o in range(2, 8)}
+        variants = {o: ps.predictor_variant(o) for o in range(2, 8)}
         ps.close()
         res[form] = dict(sec=sec, dof=dof, nwaves=nwaves, wall=wall, sols=sols, variants=variants, times=times)
         print(f"batch {form}: {nwaves} wave(s), {dof / sec:.3e} DOF/s, {wall:.1f} s with setup", file=sys.stderr,
@@ -499,10 +501,10 @@ def bench_batch(capi, args):
                                f"{cells} cells, {dof_step} DOF per step, periodic CFL0.9",
                    "seed": args.seed, "hbm_budget_gb": args.hbm_budget, "waves": m["nwaves"],
                    "launches": "one per stage and order group, captured once per wave in a hipGraph",
-                   "lds_variant_by_order": m["variants"]},
+                   "predictor_variant_by_order": m["variants"]},
         "wall_s_including_setup": round(m["wall"], 3),
         "fp64": {"value": m64["dof"] / m64["sec"], "ms_per_step": m64["sec"] / args.steps * 1e3,
-                 "waves": m64["nwaves"], "lds_variant_by_order": m64["variants"],
+                 "waves": m64["nwaves"], "predictor_variant_by_order": m64["variants"],
                  "wall_s_including_setup": round(m64["wall"], 3)},
         "fp32_vs_fp64": {"speedup": m64["sec"] / m["sec"],
                          "l2_rel_err": (num / den) ** 0.5 if den > 0 else None,
@@ -654,6 +656,9 @@ def main():
     ap.add_argument("--form", choices=("both", "fp32", "fp64", "linear"), default="both",
                     help="both: time FP64 and FP32-bulk and report the error between them (elastic: and the "
                          "linear kernels); one form: time that form only (linear: elastic workload only)")
+    ap.add_argument("--predictor", choices=("auto", "mfma"), default="auto",
+                    help="precision workload, FP32-bulk form: auto = the default predictor (contractions on the "
+                         "vector ALUs); mfma = the predictor with its contractions on the matrix cores")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fv-cells", type=int, default=54, help="fv: cells per direction")
@@ -712,7 +717,8 @@ def main():
     sol64 = None
     for form in forms:
         prec = capi.PRECISION_FP64 if form == "fp64" else capi.PRECISION_FP32_BULK
-        r = run_form(capi, order, n, args.steps, args.warmup, args.seed, prec)
+        variant = capi.PREDICTOR_MFMA if (form == "fp32" and args.predictor == "mfma") else 0
+        r = run_form(capi, order, n, args.steps, args.warmup, args.seed, prec, variant)
         res[form] = r
         if form == "fp64" and "fp32" in forms:
             sol64 = r["grid"].solution()
@@ -725,7 +731,10 @@ def main():
     m = res[main_form]
     value = dof * args.steps / m["wall"]
     if main_form == "fp32":
-        roof = roofline_for(m, B, V, D, FP32_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,float> (predictor + volume integral)")
+        roof = roofline_for(m, B, V, D, FP32_PEAK_TFLOPS,
+                            "predictor_mfma_kernel<6,Euler,flux> (predictor + volume integral, contractions on "
+                            "the matrix cores)" if args.predictor == "mfma" else
+                            "predictor_kernel<6,Euler,flux,float> (predictor + volume integral, vector ALUs)")
     else:
         roof = roofline_for(m, B, V, D, FP64_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,double> (predictor + volume integral)")
     roof["traffic"] = measured_traffic(main_form, order, n)

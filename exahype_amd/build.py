@@ -10,7 +10,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libexahype_amd.so")
 SOURCES = ["csrc/aderdg_kernels.hip", "csrc/capi.hip", "csrc/limiter_kernels.hip", "csrc/limiter_capi.hip",
            "csrc/patchset_capi.hip", "csrc/fv_kernels.hip", "csrc/fv_capi.hip", "csrc/linear_kernels.hip", "csrc/amr_kernels.hip",
-           "csrc/amr_capi.hip", "csrc/output_kernels.hip"]
+           "csrc/amr_capi.hip", "csrc/output_kernels.hip", "csrc/predictor_mfma.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
     # no packed FP32 instructions (v_pk_*_f32) anywhere in device code
@@ -45,8 +45,14 @@ def parse_resource_remarks(text):
     return kernels
 
 
+# Per-source flags.  The matrix-core predictor keeps the MFMA accumulators in
+# ordinary VGPRs (the AGPR form copies them back and forth and needs more
+# registers in total).
+EXTRA_FLAGS = {"predictor_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src, obj):
-    cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + [
+    cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + EXTRA_FLAGS.get(os.path.basename(src), []) + [
         "-c", "-Rpass-analysis=kernel-resource-usage", "-o", obj, src]
     res = subprocess.run(cmd, capture_output=True, text=True)
     return src, cmd, res

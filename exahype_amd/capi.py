@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libexahype_amd.so")
 
 EXA_OK = 0
+EXA_E_HIP, EXA_E_UNSUPPORTED, EXA_E_INVALID, EXA_E_NOMEM, EXA_E_NONFINITE = 1, 2, 3, 4, 5
 PDE_EULER = 0
 PDE_ELASTIC = 1
 TERM_FLUX, TERM_SOURCE, TERM_NCP = 1, 2, 4
@@ -23,6 +24,7 @@ BC_PERIODIC = 0
 BC_TRANSMISSIVE = 1
 FV_GODUNOV, FV_MUSCLHANCOCK = 0, 1
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
+PREDICTOR_AUTO, PREDICTOR_VALU_LDS64, PREDICTOR_VALU_LDS144, PREDICTOR_MFMA = 0, 1, 2, 3
 
 # Every symbol declared in include/exahype_amd.h.
 EXPORTED = (
@@ -39,7 +41,7 @@ EXPORTED = (
     "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
-    "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_lds_variant",
+    "exa_grid_set_predictor_variant", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
     "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy",
     "exa_godunov_solution_update", "exa_fvgrid_create", "exa_fvgrid_destroy", "exa_fvgrid_num_cells",
     "exa_fvgrid_upload", "exa_fvgrid_download", "exa_fvgrid_init_euler", "exa_fvgrid_step",
@@ -161,6 +163,7 @@ def lib():
     L.exa_grid_dt_history.argtypes = [V, _D, ctypes.c_long]
     L.exa_grid_picard_iterations.argtypes = [V, _I]
     L.exa_grid_set_profiling.argtypes = [V, ctypes.c_int]
+    L.exa_grid_set_predictor_variant.argtypes = [V, ctypes.c_int]
     L.exa_grid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     L.exa_grid_download_cells.argtypes = [V, ctypes.c_long, ctypes.c_long, _D]
     L.exa_grid_picard_totals.argtypes = [V, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
@@ -188,7 +191,7 @@ def lib():
     L.exa_patchset_create.argtypes = [ctypes.POINTER(PatchSpec), i, i, d_, ctypes.c_long, ctypes.POINTER(V)]
     L.exa_patchset_run.argtypes = [V, i, i, i, i]
     L.exa_patchset_stats.argtypes = [V, _D, _D, _I]
-    L.exa_patchset_lds_variant.argtypes = [V, i]
+    L.exa_patchset_predictor_variant.argtypes = [V, i]
     L.exa_patchset_solution.argtypes = [V, i, _D]
     L.exa_patchset_time.argtypes = [V, i, _D, _D]
     L.exa_patchset_destroy.argtypes = [V]
@@ -550,6 +553,10 @@ class Grid:
     def set_profiling(self, on=True):
         check(lib().exa_grid_set_profiling(self._h, int(on)), "profiling")
 
+    def set_predictor_variant(self, variant):
+        """EXA_PREDICTOR_AUTO (0), _VALU_LDS64 (1), _VALU_LDS144 (2) or _MFMA (3)."""
+        check(lib().exa_grid_set_predictor_variant(self._h, int(variant)), "predictor variant")
+
     def stage_times(self):
         ms = np.zeros(4)
         n = (ctypes.c_long * 4)()
@@ -797,8 +804,8 @@ class PatchSet:
         check(lib().exa_patchset_stats(self._h, ctypes.byref(sec), ctypes.byref(dof), ctypes.byref(nw)), "stats")
         return sec.value, dof.value, nw.value
 
-    def lds_variant(self, order):
-        return lib().exa_patchset_lds_variant(self._h, order)
+    def predictor_variant(self, order):
+        return lib().exa_patchset_predictor_variant(self._h, order)
 
     def solution(self, j):
         n, order = self.specs[j][0], self.specs[j][1]

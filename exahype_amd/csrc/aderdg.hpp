@@ -85,9 +85,9 @@ struct PredictorArgs {
   // optional batch of independent patches: cell -> patch; dt and dx are then
   // per patch (dt[patch], dx[3 * patch])
   const int* cell_patch;
-  // LDS budget variant of the predictor: 0 = 144 KB, 1 = 64 KB (see
-  // PredictorLayout); the batch autotuner picks one per order
-  int lds_variant;
+  // Predictor variant (kPredictorAuto, kPredictorValu64, kPredictorValu144);
+  // the batch autotuner picks one per order
+  int variant;
 };
 
 struct RiemannArgs {
@@ -164,10 +164,26 @@ bool supported(const SolverDesc& sd);
 bool supported_sizes(const SolverDesc& sd);
 hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const PredictorArgs& a,
                             long ncells, hipStream_t s);
-// Bytes of PredictorArgs::ws one cell needs (0: no workspace), for an LDS variant.
-size_t predictor_ws_bytes_per_cell(const SolverDesc& sd, int lds_variant = 0);
-// Whether the 64 KB LDS variant is compiled for this solver.
-bool predictor_has_lds_variant(const SolverDesc& sd);
+// Predictor variants.  kPredictorAuto: the measured default, the vector-ALU
+// kernel (predictor_kernel) with its 144 KB LDS budget; kPredictorValu64 /
+// kPredictorValu144: that kernel with a 64 KB / 144 KB LDS budget for the
+// Picard iterate (see PredictorLayout); kPredictorMfma: the FP32-bulk kernel
+// with its contractions on the matrix cores (predictor_mfma.hip), where
+// compiled (predictor_mfma_supported).
+enum : int { kPredictorAuto = 0, kPredictorValu64 = 1, kPredictorValu144 = 2, kPredictorMfma = 3 };
+constexpr int kPredictorVariants = 4;
+// Bytes of PredictorArgs::ws one cell needs (0: no workspace), for a variant.
+size_t predictor_ws_bytes_per_cell(const SolverDesc& sd, int variant = kPredictorAuto);
+// Bit v set: variant v runs a kernel of its own for this solver (bit 0 always;
+// kPredictorValu144 is the Auto kernel, so its bit is never set).
+unsigned predictor_variants(const SolverDesc& sd);
+// Whether a variant can be requested for this solver (it may share Auto's kernel).
+bool predictor_variant_available(const SolverDesc& sd, int variant);
+// The FP32-bulk predictor with its contractions on the matrix cores
+// (predictor_mfma.hip): solvers it is compiled for, and its launch (no
+// workspace, no exports).
+bool predictor_mfma_supported(const SolverDesc& sd);
+hipError_t launch_predictor_mfma(const SolverDesc& sd, const PredictorArgs& a, long ncells, hipStream_t s);
 // Cells per workspace chunk the host allocates for (bounds the workspace size).
 constexpr long kPredictorWsCells = 4096;
 hipError_t launch_riemann(const SolverDesc& sd, const RiemannArgs& a, hipStream_t s);

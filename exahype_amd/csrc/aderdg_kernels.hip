@@ -216,6 +216,15 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       qi_set(t, m, q0[m]);
   T qn[B][V];
   int iters = MAX_ITER;
+  // Float form of a flux-only PDE: the first Picard iteration in closed form
+  // and the time-averaged fluxes accumulated while the iterations run (see
+  // below); the double form keeps the reference's order of operations.
+  constexpr bool FLOAT_FLUX_ONLY = !IS_FP64 && USE_FLUX && !USE_SRC_NCP;
+  // accumulating the averages needs 3V more registers: only at B = 5, where
+  // they fit the budget of four wavefronts per SIMD (measured register counts:
+  // B = 3, 4 and 6 would drop a wavefront per SIMD, B = 7 would spill)
+  constexpr bool ACC_FH = FLOAT_FLUX_ONLY && B == 5;
+  T fh[3][V];
   const T iweight = T(1.0) / w3;
   __syncthreads();
 
@@ -288,6 +297,12 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
     for (int l = 0; l < B; ++l)
 #pragma unroll
       for (int m = 0; m < V; ++m) qn[l][m] = T(0);
+    if constexpr (ACC_FH) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < V; ++m) fh[d][m] = T(0);
+    }
     // Rolled (large sizes): table entries come from constant memory with a
     // uniform index; unrolled: they are immediates.
     // One time slice; with SPLIT (rolled loops over slices that live partly
@@ -313,6 +328,13 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       if (USE_FLUX) {
         T F[3][V];
         PDE::flux(Q, F);
+        if constexpr (ACC_FH) {
+          // time average of the slice fluxes of this iterate (:367-396)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int m = 0; m < V; ++m) fh[d][m] += F[d][m] * wt;
+        }
         {
 #pragma unroll
           for (int d = 0; d < 3; ++d)
@@ -396,7 +418,53 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       }
       if (USE_FLUX || USE_NCP) __syncthreads();  // the next slice rewrites sF / sQ
     };
-    if constexpr (SPLIT) {
+    if (FLOAT_FLUX_ONLY && iter == 0) {
+      // First iteration of the float form: every slice of the initial
+      // iterate equals luh, so the slices share one flux contraction S and
+      // lQi^1(l) = sum_t iK1[l][t] (FLCoeff[t] luh - w_t S)
+      //          = (sum_t iK1[l][t] FLCoeff[t]) luh - (sum_t iK1[l][t] w_t) S,
+      // coefficients formed in double from the tables.
+      T F[3][V];
+      {
+        T Q[D];
+#pragma unroll
+        for (int m = 0; m < D; ++m) Q[m] = m < V ? q0i[m < V ? m : 0] : q0[m];
+        PDE::flux(Q, F);
+      }
+      T sumw = T(0);
+      {
+        double sw = 0.0;
+#pragma unroll
+        for (int t = 0; t < B; ++t) sw += exa_gl_weights[N][t];
+        sumw = tc<T>(sw);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+          sF[(d * V + m) * B3 + node] = F[d][m];
+          if constexpr (ACC_FH) fh[d][m] = F[d][m] * sumw;
+        }
+      __syncthreads();
+      contract_lines(T(0));
+      T S[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m)
+        S[m] = g_node[0] * sF[(0 * V + m) * B3 + node] + g_node[1] * sF[(1 * V + m) * B3 + node] +
+               g_node[2] * sF[(2 * V + m) * B3 + node];
+#pragma unroll
+      for (int l = 0; l < B; ++l) {
+        double al = 0.0, bl = 0.0;
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+          al += exa_ik1[N][l][t] * exa_fl_coeff[N][t];
+          bl += exa_ik1[N][l][t] * exa_gl_weights[N][t];
+        }
+#pragma unroll
+        for (int m = 0; m < V; ++m) qn[l][m] = tc<T>(al) * q0i[m] - tc<T>(bl) * S[m];
+      }
+      __syncthreads();  // the next iteration rewrites sF
+    } else if constexpr (SPLIT) {
 #pragma unroll 1
       for (int t = 0; t < NQL; ++t) picard_slice(t);
 #pragma unroll 1
@@ -504,13 +572,14 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   // Time-averaged fluxes of the last flux evaluation, i.e. of lQi^k = qi (the
   // reference computes lFi before the final lQi update, :96-122 vs :306):
   // lFhi = sum_t w_t F(lQi^k(t)), lShi likewise (:367-396).
-  T fh[3][V];
   T sh[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
-    fh[0][m] = T(0);
-    fh[1][m] = T(0);
-    fh[2][m] = T(0);
+    if constexpr (!ACC_FH) {
+      fh[0][m] = T(0);
+      fh[1][m] = T(0);
+      fh[2][m] = T(0);
+    }
     sh[m] = T(0);
   }
   auto average_slice = [&](const int t) __device__ __attribute__((always_inline)) {
@@ -541,7 +610,9 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
       if (USE_NCP) __syncthreads();
     }
   };
-  if constexpr (SPLIT) {
+  if constexpr (ACC_FH) {
+    // fh holds the time average of the last iteration's fluxes already
+  } else if constexpr (SPLIT) {
 #pragma unroll 1
     for (int t = 0; t < NQL; ++t) average_slice(t);
 #pragma unroll 1
@@ -1043,7 +1114,11 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
                             hipStream_t s) {
   if (ncells <= 0) return hipSuccess;
   if (export_temps && sd.precision != kFP64) return hipErrorNotSupported;
-  const size_t ws = predictor_ws_bytes_per_cell(sd, a.lds_variant);
+  if (a.variant == kPredictorMfma) {
+    if (export_temps) return hipErrorNotSupported;
+    return launch_predictor_mfma(sd, a, ncells, s);
+  }
+  const size_t ws = predictor_ws_bytes_per_cell(sd, a.variant);
   if (ws > 0 && (a.ws == nullptr || a.ws_cells <= 0)) return hipErrorInvalidValue;
   const long chunk = ws > 0 ? a.ws_cells : ncells;
   return dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
@@ -1062,12 +1137,12 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kNCP, false, double>), grid, block, 0, s, ac);
       } else if (sd.precision == kFP32Bulk) {
         if (sd.terms != kFlux) return hipErrorNotSupported;
-        if (a.lds_variant == 1 && PDE::P == 0)
+        if (a.variant == kPredictorValu64 && PDE::P == 0)
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float, 64>), grid, block, 0, s, ac);
         else
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, float>), grid, block, 0, s, ac);
       } else if (sd.terms == kFlux) {
-        if (a.lds_variant == 1 && PDE::P == 0 && !export_temps)
+        if (a.variant == kPredictorValu64 && PDE::P == 0 && !export_temps)
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, false, double, 64>), grid, block, 0, s, ac);
         else if (export_temps)
           hipLaunchKernelGGL((predictor_kernel<B, PDE, kFlux, true, double>), grid, block, 0, s, ac);
@@ -1089,15 +1164,28 @@ hipError_t launch_predictor(const SolverDesc& sd, bool export_temps, const Predi
   });
 }
 
-bool predictor_has_lds_variant(const SolverDesc& sd) {
+static bool predictor_has_lds64(const SolverDesc& sd) {
   return sd.pde == kPdeEuler && sd.npar == 0 && sd.terms == kFlux;
 }
 
-size_t predictor_ws_bytes_per_cell(const SolverDesc& sd, int lds_variant) {
+unsigned predictor_variants(const SolverDesc& sd) {
+  unsigned v = 1u << kPredictorAuto;
+  if (predictor_has_lds64(sd)) v |= 1u << kPredictorValu64;
+  if (predictor_mfma_supported(sd)) v |= 1u << kPredictorMfma;
+  return v;
+}
+
+bool predictor_variant_available(const SolverDesc& sd, int variant) {
+  if (sd.linear) return variant == kPredictorAuto;
+  if (variant == kPredictorAuto || variant == kPredictorValu144) return true;
+  return variant >= 0 && variant < kPredictorVariants && (predictor_variants(sd) & (1u << variant)) != 0;
+}
+
+size_t predictor_ws_bytes_per_cell(const SolverDesc& sd, int variant) {
   size_t bytes = 0;
   (void)dispatch_pde(sd, [&]<int B, class PDE>() -> hipError_t {
     constexpr int V = PDE::V, D = PDE::V + PDE::P;
-    if (lds_variant == 1 && predictor_has_lds_variant(sd)) {
+    if (variant == kPredictorValu64 && predictor_has_lds64(sd)) {
       bytes = sd.precision == kFP32Bulk ? PredictorLayout<B, V, D, false, float, 64>::ws_bytes
                                         : PredictorLayout<B, V, D, false, double, 64>::ws_bytes;
       return hipSuccess;

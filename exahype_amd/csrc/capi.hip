@@ -425,6 +425,8 @@ struct exa_grid {
   int* iters = nullptr;
   void* ws = nullptr;      // predictor workspace (large orders only)
   long ws_cells = 0;
+  size_t ws_bytes = 0;
+  int predictor_variant = exa::kPredictorAuto;
   FaceDesc* faces = nullptr;
   exa::StepState* st = nullptr;
   double* dt_hist = nullptr;
@@ -515,6 +517,7 @@ hipError_t launch_predict(exa_grid* g) {
   a.cell0 = 0;
   a.ws = g->ws;
   a.ws_cells = g->ws_cells;
+  a.variant = g->predictor_variant;
   hipError_t e = staged(g, EXA_STAGE_PREDICTOR, [&] { return exa::launch_predictor(g->sd, false, a, g->ncells, g->stream); });
   if (e == hipSuccess && g->profiling) e = exa::launch_count_iterations(g->iters, g->ncells, g->st, g->stream);
   return e;
@@ -589,7 +592,8 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   A((void**)&g->iters, sizeof(int) * g->ncells);
   if (const size_t wsb = sd.linear ? 0 : exa::predictor_ws_bytes_per_cell(sd)) {
     g->ws_cells = std::min(g->ncells, exa::kPredictorWsCells);
-    A(&g->ws, wsb * g->ws_cells);
+    g->ws_bytes = wsb * g->ws_cells;
+    A(&g->ws, g->ws_bytes);
   }
   // Periodic: every cell owns its +x, +y, +z face.  Transmissive: interior
   // faces only -- at a boundary face the reference solves a Riemann problem
@@ -800,6 +804,25 @@ extern "C" int exa_grid_picard_iterations(exa_grid* g, int* out) {
 extern "C" int exa_grid_set_profiling(exa_grid* g, int on) {
   if (!g) return EXA_E_INVALID;
   g->profiling = on != 0;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_set_predictor_variant(exa_grid* g, int variant) {
+  if (!g || variant < EXA_PREDICTOR_AUTO || variant > EXA_PREDICTOR_MFMA) return EXA_E_INVALID;
+  if (!exa::predictor_variant_available(g->sd, variant)) return EXA_E_UNSUPPORTED;
+  // the variant's Picard workspace (vector-ALU kernels at large orders)
+  const long cells = std::min(g->ncells, exa::kPredictorWsCells);
+  const size_t need = exa::predictor_ws_bytes_per_cell(g->sd, variant) * cells;
+  if (need > g->ws_bytes) {
+    EXA_TRY(hipStreamSynchronize(g->stream));
+    void* ws = nullptr;
+    EXA_TRY(hipMalloc(&ws, need));
+    (void)hipFree(g->ws);
+    g->ws = ws;
+    g->ws_bytes = need;
+  }
+  if (need > 0) g->ws_cells = cells;
+  g->predictor_variant = variant;
   return EXA_OK;
 }
 

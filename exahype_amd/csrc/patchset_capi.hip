@@ -5,7 +5,8 @@
 // cell -> patch map.  Patches that do not fit the HBM budget together run in
 // waves (patches are independent, so this is exact).  The time step of a wave
 // is captured once in a hipGraph and replayed; an autotuner picks, per order,
-// the predictor's LDS-budget variant by timing both on the wave's first state.
+// the predictor variant (matrix-core or vector-ALU contractions, LDS budget)
+// by timing each on the wave's first state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,7 +34,7 @@ struct Group {
   long nfaces = 0;
   void* ws = nullptr;
   long ws_cells = 0;
-  int lds_variant = 0;
+  int variant = exa::kPredictorAuto;
 };
 
 size_t bytes_per_cell(const SolverDesc& sd) {
@@ -54,7 +55,7 @@ struct exa_patchset {
   std::vector<std::vector<double>> solutions;  // per patch with at most keep_max_cells cells
   std::vector<double> t, dt;                   // per patch, after the run
   double seconds = 0, dof_updates = 0;
-  int lds_choice[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int variant_choice[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 };
 
 namespace {
@@ -105,7 +106,7 @@ exa::PredictorArgs pred_args(const Wave& w, const Group& g) {
   a.ws = g.ws;
   a.ws_cells = g.ws_cells;
   a.cell_patch = g.cell_patch;
-  a.lds_variant = g.lds_variant;
+  a.variant = g.variant;
   return a;
 }
 
@@ -238,19 +239,21 @@ int build_wave(exa_patchset* b, const std::vector<int>& ids, Wave& w) {
   return EXA_OK;
 }
 
-// Time the predictor of each order group with both LDS variants on the
+// Time the predictor of each order group in each of its variants on the
 // wave's current state (the predictor is a pure function of luh and dt, so
-// repeated launches leave the state unchanged) and keep the faster one.
+// repeated launches leave the state unchanged) and keep the fastest.
 int autotune(exa_patchset* b, Wave& w) {
   hipEvent_t e0, e1;
   EXA_TRY(hipEventCreate(&e0));
   EXA_TRY(hipEventCreate(&e1));
   for (auto& g : w.groups) {
-    if (!exa::predictor_has_lds_variant(g.sd)) continue;
+    const unsigned variants = exa::predictor_variants(g.sd);
+    if (variants == 1u) continue;
     float best = 1e30f;
     int choice = 0;
-    for (int v = 0; v < 2; ++v) {
-      g.lds_variant = v;
+    for (int v = 0; v < exa::kPredictorVariants; ++v) {
+      if (!(variants & (1u << v))) continue;
+      g.variant = v;
       EXA_TRY(exa::launch_predictor(g.sd, false, pred_args(w, g), g.ncells, b->stream));  // warm-up
       EXA_TRY(hipEventRecord(e0, b->stream));
       for (int r = 0; r < 3; ++r) EXA_TRY(exa::launch_predictor(g.sd, false, pred_args(w, g), g.ncells, b->stream));
@@ -263,8 +266,8 @@ int autotune(exa_patchset* b, Wave& w) {
         choice = v;
       }
     }
-    g.lds_variant = choice;
-    b->lds_choice[g.order] = choice;
+    g.variant = choice;
+    b->variant_choice[g.order] = choice;
     // leave the state as the chosen variant computes it
     EXA_TRY(exa::launch_predictor(g.sd, false, pred_args(w, g), g.ncells, b->stream));
   }
@@ -339,7 +342,7 @@ extern "C" int exa_patchset_run(exa_patchset* b, int warmup_steps, int nsteps, i
     if (st == EXA_OK && autotune_variants) st = autotune(b, w);
     if (st == EXA_OK && !autotune_variants)
       for (auto& g : w.groups)
-        if (b->lds_choice[g.order] >= 0) g.lds_variant = b->lds_choice[g.order];
+        if (b->variant_choice[g.order] >= 0) g.variant = b->variant_choice[g.order];
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (st == EXA_OK && use_graph) {
@@ -423,9 +426,9 @@ extern "C" int exa_patchset_stats(exa_patchset* b, double* seconds, double* dof_
   return EXA_OK;
 }
 
-extern "C" int exa_patchset_lds_variant(exa_patchset* b, int order) {
+extern "C" int exa_patchset_predictor_variant(exa_patchset* b, int order) {
   if (!b || order < 0 || order > 9) return -1;
-  return b->lds_choice[order];
+  return b->variant_choice[order];
 }
 
 extern "C" int exa_patchset_solution(exa_patchset* b, int patch, double* luh) {

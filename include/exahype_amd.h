@@ -52,6 +52,16 @@ enum { EXA_TERM_FLUX = 1, EXA_TERM_SOURCE = 2, EXA_TERM_NCP = 4 };
  * rounding.  Grid entry points only; the single-cell ones are FP64. */
 enum { EXA_PRECISION_FP64 = 0, EXA_PRECISION_FP32_BULK = 1 };
 
+/* Space-time predictor variants (the device kernel behind
+ * spaceTimePredictorNonlinear + volumeIntegralNonlinear).  AUTO: the default,
+ * the tensor contractions on the vector ALUs with a 144 KB LDS budget for the
+ * Picard iterate (measured fastest); VALU_LDS64 / VALU_LDS144: the vector-ALU
+ * kernel with a 64 KB / 144 KB LDS budget; MFMA: the FP32-bulk form of the
+ * Euler solvers (no parameters, orders 2..7) with the contractions on the
+ * matrix cores (v_mfma_f32_4x4x1f32).  The variants compute the same
+ * mathematics and differ in rounding order only. */
+enum { EXA_PREDICTOR_AUTO = 0, EXA_PREDICTOR_VALU_LDS64 = 1, EXA_PREDICTOR_VALU_LDS144 = 2, EXA_PREDICTOR_MFMA = 3 };
+
 /* SolverType's compile-time constants (AbstractGenericADERDGSolverHeader.template:32-35). */
 typedef struct exa_solver {
   int pde;               /* exa_pde */
@@ -293,6 +303,9 @@ int exa_grid_picard_iterations(exa_grid* g, int* out);
 enum { EXA_STAGE_PREDICTOR = 0, EXA_STAGE_RIEMANN = 1, EXA_STAGE_UPDATE = 2, EXA_STAGE_DT = 3,
        EXA_NUM_STAGES = 4 };
 int exa_grid_set_profiling(exa_grid* g, int on);
+/* Predictor variant of the grid's later predictor launches (EXA_PREDICTOR_*;
+ * default AUTO).  EXA_E_UNSUPPORTED for a variant the solver does not compile. */
+int exa_grid_set_predictor_variant(exa_grid* g, int variant);
 /* Sum of per-cell Picard iteration counts and of cells over every predictor
  * launch made while profiling was on (the work of the predictor launches). */
 int exa_grid_picard_totals(exa_grid* g, unsigned long long* iterations, unsigned long long* cells);
@@ -464,15 +477,15 @@ typedef struct exa_patchset exa_patchset;
 int exa_patchset_create(const exa_patch_spec* patches, int npatches, int precision, double hbm_budget_gb,
                         long keep_max_cells, exa_patchset** out);
 /* Set up every wave (initial condition, first dt and predictor), optionally
- * autotune the predictor's LDS variant per order, and advance each patch
+ * autotune the predictor variant per order (EXA_PREDICTOR_*), and advance each patch
  * warmup_steps + nsteps time steps, replaying a captured hipGraph when
  * use_graph != 0; the last nsteps are timed.  Synchronous. */
 int exa_patchset_run(exa_patchset* b, int warmup_steps, int nsteps, int use_graph, int autotune_variants);
 /* Device seconds of the timed steps (HIP events around each wave's timed steps),
  * degrees of freedom updated in them, and the number of waves. */
 int exa_patchset_stats(exa_patchset* b, double* seconds, double* dof_updates, int* nwaves);
-/* LDS variant the autotuner chose for an order (0: 144 KB, 1: 64 KB; -1 none). */
-int exa_patchset_lds_variant(exa_patchset* b, int order);
+/* Predictor variant the autotuner chose for an order (EXA_PREDICTOR_*; -1 none). */
+int exa_patchset_predictor_variant(exa_patchset* b, int order);
 /* Solution [n^3][B^3][5] of one patch after the run (kept patches only). */
 int exa_patchset_solution(exa_patchset* b, int patch, double* luh);
 /* Time stamp and the dt of the next step of one patch after the run. */

@@ -27,6 +27,9 @@ METRIC = "DOF-updates/s, 3D Euler N=5 81^3: FP32-MFMA vs FP64 GPU at L2 err ≤1
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+# FP32 FMA rate of the vector ALUs without packed FP32 instructions (which this
+# build does not use): 256 CUs x 4 SIMDs x 16 lanes x 2 FLOP x 2.4 GHz
+FP32_VALU_PEAK_TFLOPS = 78.6
 EULER_FLUX_FLOPS = 40  # flops of one Euler flux evaluation (3 directions)
 
 
@@ -39,6 +42,22 @@ def predictor_flops(B, V, D, picard_total, cells):
     per_iter = 8 * V * B ** 5 + (EULER_FLUX_FLOPS + 4 * V) * B ** 4
     fixed = 2 * (3 * V + D) * B ** 4 + 12 * (D + V) * B ** 3 + 6 * V * B ** 4
     return per_iter * picard_total + fixed * cells
+
+
+def predictor_executed_flops_fp32(B, V, picard_total, cells):
+    """FLOPs the default FP32-bulk predictor executes for a flux-only PDE (the
+    algorithmic count above counts every Picard iteration in full): the first
+    iteration evaluates one flux and one slice's Kxi contractions per node
+    (c_F + 6 V B) and forms lQi^1 in closed form and its residual (8 V B + 5 V);
+    later iterations are full; the epilogue adds the once-per-cell terms and,
+    except at B = 5 where the flux averages are accumulated during the
+    iterations, re-evaluates the B slice fluxes of the last iterate (c_F B^4)."""
+    c_f = EULER_FLUX_FLOPS
+    first = B ** 3 * (c_f + 6 * V * B + 8 * V * B + 5 * V)
+    per_iter = 8 * V * B ** 5 + (c_f + 4 * V) * B ** 4
+    fixed = 2 * (3 * V + V) * B ** 4 + 12 * (V + V) * B ** 3 + 6 * V * B ** 4
+    recompute = 0 if B == 5 else c_f * B ** 4
+    return cells * first + (picard_total - cells) * per_iter + cells * (fixed + recompute)
 
 
 def cpu_baseline(exa_grid, order, n, seconds, threads):
@@ -108,15 +127,27 @@ def measured_traffic(form, order, n):
     return None
 
 
-def roofline_for(r, B, V, D, peak, kernel):
+def roofline_for(r, B, V, D, peak, kernel, compute_unit, valu_peak=None, executed=None):
+    """Compute roofline of the predictor.  `bound` is the contract's category
+    (compute, i.e. "mfma", not "hbm"); `compute_unit` names the unit that
+    executes the contractions, `valu_frac` the fraction of the vector ALUs'
+    peak, `executed_frac` the fraction for the FLOPs the kernel executes."""
     pred_ms, pred_n = r["stages"]["predictor"]
     avg_ms = pred_ms / max(pred_n, 1)
     flops_per_launch = predictor_flops(B, V, D, r["picard_total"], r["pred_cells"]) / max(pred_n, 1)
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    return dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
-                frac=round(achieved / peak, 4), traffic=None, kernel=kernel, avg_launch_ms=round(avg_ms, 4),
-                flops_per_launch=flops_per_launch,
-                picard_mean=round(r["picard_total"] / max(r["pred_cells"], 1), 4))
+    out = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+               frac=round(achieved / peak, 4), traffic=None, kernel=kernel, compute_unit=compute_unit,
+               avg_launch_ms=round(avg_ms, 4), flops_per_launch=flops_per_launch,
+               picard_mean=round(r["picard_total"] / max(r["pred_cells"], 1), 4))
+    if valu_peak:
+        out["valu_peak"] = valu_peak
+        out["valu_frac"] = round(achieved / valu_peak, 4)
+    if executed is not None:
+        ex = executed(r["picard_total"], r["pred_cells"]) / max(pred_n, 1)
+        out["executed_flops_per_launch"] = ex
+        out["executed_frac"] = round(ex / (avg_ms * 1e-3) / 1e12 / peak, 4)
+    return out
 
 
 def copy_calibration(mib):
@@ -731,12 +762,19 @@ def main():
     m = res[main_form]
     value = dof * args.steps / m["wall"]
     if main_form == "fp32":
-        roof = roofline_for(m, B, V, D, FP32_PEAK_TFLOPS,
-                            "predictor_mfma_kernel<6,Euler,flux> (predictor + volume integral, contractions on "
-                            "the matrix cores)" if args.predictor == "mfma" else
-                            "predictor_kernel<6,Euler,flux,float> (predictor + volume integral, vector ALUs)")
+        if args.predictor == "mfma":
+            roof = roofline_for(m, B, V, D, FP32_PEAK_TFLOPS,
+                                "predictor_mfma_kernel<6,Euler,flux> (predictor + volume integral)",
+                                "matrix cores (v_mfma_f32_4x4x1f32) for the contractions, vector ALUs for the rest")
+        else:
+            roof = roofline_for(m, B, V, D, FP32_PEAK_TFLOPS,
+                                "predictor_kernel<6,Euler,flux,float> (predictor + volume integral)",
+                                "vector ALUs (FP32 FMA, no packed FP32; no MFMA instruction)",
+                                valu_peak=FP32_VALU_PEAK_TFLOPS,
+                                executed=lambda p, c: predictor_executed_flops_fp32(B, V, p, c))
     else:
-        roof = roofline_for(m, B, V, D, FP64_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,double> (predictor + volume integral)")
+        roof = roofline_for(m, B, V, D, FP64_PEAK_TFLOPS, "predictor_kernel<6,Euler,flux,double> (predictor + volume integral)",
+                            "vector ALUs (FP64 FMA)")
     roof["traffic"] = measured_traffic(main_form, order, n)
     # algorithmic HBM bytes per launch: read luh (double), write lduh, lQhbnd, lFhbnd (bulk type)
     bulk = 4 if main_form == "fp32" else 8
@@ -772,7 +810,7 @@ def main():
         out["fp64"] = {"value": dof * args.steps / r64["wall"], "ms_per_step": r64["wall"] / args.steps * 1e3,
                        "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in r64["stages"].items()},
                        "roofline": dict(roofline_for(r64, B, V, D, FP64_PEAK_TFLOPS,
-                                                     "predictor_kernel<6,Euler,flux,double>"),
+                                                     "predictor_kernel<6,Euler,flux,double>", "vector ALUs (FP64 FMA)"),
                                         traffic=measured_traffic("fp64", order, n))}
     if "fp32" in res and "fp64" in res:
         out["fp32_vs_fp64"] = {"speedup": out["fp64"]["ms_per_step"] / out["ms_per_step"],

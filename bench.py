@@ -85,12 +85,13 @@ def cpu_baseline(exa_grid, order, n, seconds, threads):
                        f"{elapsed:.2f} s")
 
 
-def run_form(capi, order, n, steps, warmup, seed, precision, variant=0):
+def run_form(capi, order, n, steps, warmup, seed, precision, variant=0, fused=True):
     """One timed run of the stepping loop; returns timings and the final solution."""
     s = capi.euler_solver(order, precision=precision)
     g = capi.Grid(s, n)
     if variant:
         g.set_predictor_variant(variant)
+    g.set_fused_corrector(fused)
     g.init_euler(rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=8, mode_amp=0.01,
                  seed=seed)
     g.step(warmup)
@@ -690,6 +691,9 @@ def main():
     ap.add_argument("--predictor", choices=("auto", "mfma"), default="auto",
                     help="precision workload, FP32-bulk form: auto = the default predictor (contractions on the "
                          "vector ALUs); mfma = the predictor with its contractions on the matrix cores")
+    ap.add_argument("--corrector", choices=("fused", "two-pass"), default="fused",
+                    help="precision workload: fused = Riemann + surface + update + dt in one kernel per step "
+                         "(default); two-pass = face pass writing the flux blocks, then the update pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fv-cells", type=int, default=54, help="fv: cells per direction")
@@ -749,7 +753,8 @@ def main():
     for form in forms:
         prec = capi.PRECISION_FP64 if form == "fp64" else capi.PRECISION_FP32_BULK
         variant = capi.PREDICTOR_MFMA if (form == "fp32" and args.predictor == "mfma") else 0
-        r = run_form(capi, order, n, args.steps, args.warmup, args.seed, prec, variant)
+        r = run_form(capi, order, n, args.steps, args.warmup, args.seed, prec, variant,
+                     fused=args.corrector == "fused")
         res[form] = r
         if form == "fp64" and "fp32" in forms:
             sol64 = r["grid"].solution()
@@ -800,7 +805,10 @@ def main():
                    "form": ("FP32-bulk: predictor/volume/Riemann f32, surface/update/dt f64"
                             if main_form == "fp32" else "plain FP64"),
                    "cells": ncells, "order": order, "dof_per_step": dof,
-                   "scalar_unknowns_per_step": dof * V, "seed": args.seed},
+                   "scalar_unknowns_per_step": dof * V, "seed": args.seed,
+                   "predictor": args.predictor,
+                   "corrector": ("one kernel: Riemann on each cell's faces + surface + update + dt"
+                                 if args.corrector == "fused" else "two passes: face Riemann, then update + dt")},
         "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in m["stages"].items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -10,7 +10,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libexahype_amd.so")
 SOURCES = ["csrc/aderdg_kernels.hip", "csrc/capi.hip", "csrc/limiter_kernels.hip", "csrc/limiter_capi.hip",
            "csrc/patchset_capi.hip", "csrc/fv_kernels.hip", "csrc/fv_capi.hip", "csrc/linear_kernels.hip", "csrc/amr_kernels.hip",
-           "csrc/amr_capi.hip", "csrc/output_kernels.hip", "csrc/predictor_mfma.hip"]
+           "csrc/amr_capi.hip", "csrc/output_kernels.hip", "csrc/predictor_mfma.hip",
+           "csrc/corrector_kernels.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
     # no packed FP32 instructions (v_pk_*_f32) anywhere in device code

@@ -41,7 +41,7 @@ EXPORTED = (
     "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
-    "exa_grid_set_predictor_variant", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
+    "exa_grid_set_predictor_variant", "exa_grid_set_fused_corrector", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
     "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy",
     "exa_godunov_solution_update", "exa_fvgrid_create", "exa_fvgrid_destroy", "exa_fvgrid_num_cells",
     "exa_fvgrid_upload", "exa_fvgrid_download", "exa_fvgrid_init_euler", "exa_fvgrid_step",
@@ -164,6 +164,7 @@ def lib():
     L.exa_grid_picard_iterations.argtypes = [V, _I]
     L.exa_grid_set_profiling.argtypes = [V, ctypes.c_int]
     L.exa_grid_set_predictor_variant.argtypes = [V, ctypes.c_int]
+    L.exa_grid_set_fused_corrector.argtypes = [V, ctypes.c_int]
     L.exa_grid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     L.exa_grid_download_cells.argtypes = [V, ctypes.c_long, ctypes.c_long, _D]
     L.exa_grid_picard_totals.argtypes = [V, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
@@ -552,6 +553,10 @@ class Grid:
 
     def set_profiling(self, on=True):
         check(lib().exa_grid_set_profiling(self._h, int(on)), "profiling")
+
+    def set_fused_corrector(self, on):
+        """One-kernel corrector (Riemann + surface + update + dt) or the two passes."""
+        check(lib().exa_grid_set_fused_corrector(self._h, int(on)), "fused corrector")
 
     def set_predictor_variant(self, variant):
         """EXA_PREDICTOR_AUTO (0), _VALU_LDS64 (1), _VALU_LDS144 (2) or _MFMA (3)."""

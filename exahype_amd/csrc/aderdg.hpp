@@ -118,6 +118,24 @@ struct UpdateArgs {
   int slots_per_patch;
 };
 
+// The fused corrector of a uniform Cartesian grid (corrector_kernels.hip):
+// Riemann solves on each cell's six faces from the cell's and its neighbours'
+// face blocks, surface integral, solution update and the cell's stable dt
+// into dt_slots.  Cells are (k, j, i) row-major, periodic or transmissive.
+struct CorrectorArgs {
+  double* luh;
+  const void* lduh;     // (bulk)
+  const void* lQhbnd;   // (bulk)
+  const void* lFhbnd;   // (bulk)
+  const double* dx;     // 3 doubles
+  const double* dt;     // device scalar
+  double cfl_pnpm;      // CFL * PNPM[order]
+  unsigned long long* dt_slots;
+  int nx, ny, nz;
+  int periodic;
+};
+hipError_t launch_corrector(const SolverDesc& sd, const CorrectorArgs& a, hipStream_t s);
+
 struct VolumeArgs {
   double* lduh;
   const double* lFhi;

@@ -414,6 +414,8 @@ struct exa_grid {
   int nx, ny, nz;
   long ncells;
   double lo[3], hi[3], dx[3];
+  bool periodic = true;
+  bool fused_corrector = true;  // Riemann + surface + update + dt in one kernel (nonlinear solvers)
   long nfaces;
   hipStream_t stream = nullptr;
   double* luh = nullptr;
@@ -524,6 +526,26 @@ hipError_t launch_predict(exa_grid* g) {
 }
 
 hipError_t launch_one_step(exa_grid* g) {
+  if (!g->sd.linear && g->fused_corrector) {
+    exa::CorrectorArgs ca{};
+    ca.luh = g->luh;
+    ca.lduh = g->lduh;
+    ca.lQhbnd = g->lQhbnd;
+    ca.lFhbnd = g->lFhbnd;
+    ca.dx = g->d_dx;
+    ca.dt = &g->st->dt;
+    ca.cfl_pnpm = g->sd.cfl * kPNPM[g->sd.order];
+    ca.dt_slots = g->st->dt_slots;
+    ca.nx = g->nx;
+    ca.ny = g->ny;
+    ca.nz = g->nz;
+    ca.periodic = g->periodic ? 1 : 0;
+    hipError_t e = staged(g, EXA_STAGE_UPDATE, [&] { return exa::launch_corrector(g->sd, ca, g->stream); });
+    if (e != hipSuccess) return e;
+    e = staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, true, g->stream); });
+    if (e != hipSuccess) return e;
+    return launch_predict(g);
+  }
   exa::RiemannArgs ra{g->faces, g->nfaces, g->lQhbnd, g->lFhbnd};
   hipError_t e = staged(g, EXA_STAGE_RIEMANN, [&] {
     return g->sd.linear ? exa::launch_linear_riemann(g->sd, ra, g->stream) : exa::launch_riemann(g->sd, ra, g->stream);
@@ -575,6 +597,7 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
     g->lo[d] = lo[d];
     g->hi[d] = hi[d];
   }
+  g->periodic = periodic;
   g->dx[0] = (hi[0] - lo[0]) / nx;
   g->dx[1] = (hi[1] - lo[1]) / ny;
   g->dx[2] = (hi[2] - lo[2]) / nz;
@@ -804,6 +827,13 @@ extern "C" int exa_grid_picard_iterations(exa_grid* g, int* out) {
 extern "C" int exa_grid_set_profiling(exa_grid* g, int on) {
   if (!g) return EXA_E_INVALID;
   g->profiling = on != 0;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_set_fused_corrector(exa_grid* g, int on) {
+  if (!g) return EXA_E_INVALID;
+  if (on && g->sd.linear) return EXA_E_UNSUPPORTED;
+  g->fused_corrector = on != 0;
   return EXA_OK;
 }
 

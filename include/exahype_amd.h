@@ -306,6 +306,13 @@ int exa_grid_set_profiling(exa_grid* g, int on);
 /* Predictor variant of the grid's later predictor launches (EXA_PREDICTOR_*;
  * default AUTO).  EXA_E_UNSUPPORTED for a variant the solver does not compile. */
 int exa_grid_set_predictor_variant(exa_grid* g, int variant);
+/* Corrector of the nonlinear solvers' steps.  on (the default): one kernel per
+ * step solves the Riemann problems of each cell's six faces from the cell's
+ * and its neighbours' face blocks and applies surface integral, solution
+ * update and the stable dt (the fluxes of a face are identical in both cells);
+ * off: a face pass writing both cells' flux blocks, then the update pass.
+ * Linear solvers always run the two passes (EXA_E_UNSUPPORTED for on). */
+int exa_grid_set_fused_corrector(exa_grid* g, int on);
 /* Sum of per-cell Picard iteration counts and of cells over every predictor
  * launch made while profiling was on (the work of the predictor launches). */
 int exa_grid_picard_totals(exa_grid* g, unsigned long long* iterations, unsigned long long* cells);

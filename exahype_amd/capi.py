@@ -51,6 +51,7 @@ EXPORTED = (
     "exa_face_unknowns_prolongation", "exa_face_unknowns_restriction", "exa_volume_unknowns_prolongation",
     "exa_volume_unknowns_restriction", "exa_device_alloc", "exa_device_free", "exa_copy_to_device",
     "exa_copy_to_host", "exa_volume_unknowns_prolongation_batch", "exa_volume_unknowns_restriction_batch",
+    "exa_volume_unknowns_restriction_batch_noalloc",
     "exa_face_unknowns_prolongation_batch", "exa_face_unknowns_restriction_batch", "exa_point_source_projection",
     "exa_quadrature_node", "exa_plot_vertices", "exa_grid_plot_vertices",
 )
@@ -144,6 +145,7 @@ def lib():
     L.exa_copy_to_host.argtypes = [V, V, ctypes.c_size_t]
     L.exa_volume_unknowns_prolongation_batch.argtypes = [S, V, Lg, V, Lg, V, Lg, I, F]
     L.exa_volume_unknowns_restriction_batch.argtypes = [S, V, Lg, V, Lg, V, Lg, V, Lg, I, F]
+    L.exa_volume_unknowns_restriction_batch_noalloc.argtypes = [S, V, Lg, V, Lg, V, Lg, V, Lg, I]
     L.exa_face_unknowns_prolongation_batch.argtypes = [S, V, V, Lg, V, V, Lg, V, Lg, I, F]
     L.exa_face_unknowns_restriction_batch.argtypes = [S, V, V, Lg, V, V, Lg, V, Lg, V, Lg, I, F]
     L.exa_stable_time_step_size.argtypes = [S, _D, _D, _D]

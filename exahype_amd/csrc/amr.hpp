@@ -40,4 +40,8 @@ struct AmrArgs {
 // destinations (restriction).
 hipError_t launch_amr(const AmrArgs& a, int ndim, bool restrict_, long ngroups, hipStream_t s);
 
+// Resolves the current GPU's address of the allocation-free restriction's
+// scratch (amr_capi.hip); called by exa_init.
+int noalloc_scratch_setup();
+
 }  // namespace exa

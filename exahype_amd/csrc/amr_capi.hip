@@ -1,12 +1,52 @@
 // amr_capi.hip -- C-ABI of the AMR transfer operators and the device-buffer
 // helpers their batched forms take.
 #include <cstring>
+#include <mutex>
 
 #include "../../include/exahype_amd.h"
 #include "amr.hpp"
 #include "capi_util.hpp"
 
+// Scratch of exa_volume_unknowns_restriction_batch_noalloc, one copy per GPU
+// for the life of the process: word 0 is the call's error flag, words
+// kNoallocBitmapWord.. the claim bitmap (one bit per destination block).  In
+// host code `scratch` only names the symbol; its device address on each GPU
+// comes from hipGetSymbolAddress (noalloc_scratch below).
+constexpr long kNoallocBitmapWord = 64;                        // 256 bytes: the flag's own line
+constexpr long kNoallocBitmapWords = 1L << 18;                 // 1 MiB of bitmap
+constexpr long kNoallocMaxBlocks = 32L * kNoallocBitmapWords;  // 8,388,608 destination blocks
+static __device__ __align__(256) unsigned scratch[kNoallocBitmapWord + kNoallocBitmapWords];
+
 namespace {
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+unsigned* g_noalloc_scratch[kNoallocMaxDevices] = {};  // device address of `scratch` per GPU
+
+// Device address of `scratch` on the current GPU, resolved once per GPU.
+int noalloc_scratch(unsigned** out) {
+  int dev = 0;
+  EXA_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoallocMaxDevices) return EXA_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return EXA_E_HIP;
+    if (!p) return EXA_E_HIP;
+    g_noalloc_scratch[dev] = static_cast<unsigned*>(p);
+  }
+  *out = g_noalloc_scratch[dev];
+  return EXA_OK;
+}
+
+// Zeroes the error flag and the first `words` bitmap words (a kernel, not a
+// memset, so that the call records no memset node if a caller captures it).
+__global__ __launch_bounds__(256) void noalloc_clear_kernel(unsigned* s, long words) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) s[0] = 0u;
+  if (i < words) s[kNoallocBitmapWord + i] = 0u;
+}
 
 long ipow3(int e) {
   long r = 1;
@@ -115,6 +155,39 @@ int check_single(int coarse_level, int fine_level, const int* sub, int ndim, boo
 }
 
 }  // namespace
+
+namespace exa {
+int noalloc_scratch_setup() {
+  unsigned* p = nullptr;
+  return noalloc_scratch(&p);
+}
+}  // namespace exa
+
+extern "C" int exa_volume_unknowns_restriction_batch_noalloc(const exa_solver* s, double* luhCoarse, long ncoarse,
+                                                             const double* luhFine, long nfine, const int* ptr,
+                                                             long ndst, const int* items, long nitems,
+                                                             int levelDelta) {
+  AmrSizes z;
+  if (!amr_solver(s, &z) || !luhFine || !luhCoarse || (ndst > 0 && (!items || !ptr)) || ndst < 0 || nitems < 0 ||
+      nfine < 0 || ncoarse < 0 || levelDelta < 1)
+    return EXA_E_INVALID;
+  if (levelDelta > exa::kAmrMaxLevelDelta) return EXA_E_UNSUPPORTED;
+  if (ncoarse > kNoallocMaxBlocks) return EXA_E_INVALID;  // beyond the claim bitmap
+  unsigned* sp = nullptr;
+  if (const int st = noalloc_scratch(&sp)) return st;
+  const long words = (ncoarse + 31) / 32;
+  hipLaunchKernelGGL(noalloc_clear_kernel, dim3((unsigned)((words > 1 ? words : 1) + 255) / 256), dim3(256), 0,
+                     nullptr, sp, words);
+  EXA_TRY(hipGetLastError());
+  exa::AmrArgs a = amr_args(z, 3, z.D, luhCoarse, ncoarse, luhFine, nfine, items, nitems, ptr, levelDelta);
+  a.error = reinterpret_cast<int*>(sp);
+  a.claimed = sp + kNoallocBitmapWord;
+  EXA_TRY(exa::launch_amr(a, 3, true, ndst, nullptr));
+  EXA_TRY(hipDeviceSynchronize());
+  unsigned err = 0;
+  EXA_TRY(hipMemcpy(&err, sp, sizeof(unsigned), hipMemcpyDeviceToHost));
+  return err ? EXA_E_INVALID : EXA_OK;
+}
 
 extern "C" void* exa_device_alloc(size_t bytes) {
   void* p = nullptr;

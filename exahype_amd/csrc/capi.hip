@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "aderdg.hpp"
+#include "amr.hpp"
 
 using exa::FaceDesc;
 using exa::SolverDesc;
@@ -33,7 +34,7 @@ extern "C" const char* exa_status_string(int status) {
 
 extern "C" int exa_init(int device) {
   EXA_TRY(hipSetDevice(device));
-  return EXA_OK;
+  return exa::noalloc_scratch_setup();
 }
 
 extern "C" int exa_solver_supported(const exa_solver* s) {

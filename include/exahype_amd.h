@@ -221,6 +221,22 @@ int exa_face_unknowns_restriction_batch(const exa_solver* s, double* lQhbndCoars
                                         long nfine_faces, const int* ptr, long ndst, const int* items,
                                         long nitems, int levelDelta, float* elapsed_ms);
 
+/* exa_volume_unknowns_restriction_batch without any device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, semantics and results as the batch call
+ * (without its timing argument); the error flag and the destination claims
+ * live in a fixed device array of the library, one per GPU for the life of
+ * the process, resolved by exa_init (or the first call on a GPU): at most
+ * 8,388,608 destination blocks (ncoarse) per call, beyond which the call
+ * returns EXA_E_INVALID before launching anything.  Every call on a GPU
+ * reuses that GPU's array, so calls that overlap on one GPU (several host
+ * threads, or several processes sharing it) race: serialise them.
+ * Synchronous. */
+int exa_volume_unknowns_restriction_batch_noalloc(const exa_solver* s, double* luhCoarse, long ncoarse,
+                                                  const double* luhFine, long nfine, const int* ptr, long ndst,
+                                                  const int* items, long nitems, int levelDelta);
+
 /* solutionUpdate (K/c/3d/solutionUpdate.cpph:23); luh in/out. */
 int exa_solution_update(const exa_solver* s, double* luh, const double* lduh, double dt);
 

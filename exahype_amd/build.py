@@ -17,6 +17,7 @@ HIPCC_FLAGS = [
     # no packed FP32 instructions (v_pk_*_f32) anywhere in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
+    "-I" + os.path.join(ROOT, "include", "exahype_amd", "device"),
 ]
 
 
@@ -97,7 +98,31 @@ def build(verbose=False):
     if spilling:
         raise RuntimeError("kernels spill to scratch memory:\n  " + "\n  ".join(spilling))
     os.replace(tmp, LIB)
+    build_user_example(verbose)
     return LIB
+
+
+USER_EXAMPLE = os.path.join(HERE, "libexa_user_example.so")
+
+
+def build_user_example(verbose=False):
+    """A user's PDE compiled into the device kernels through
+    include/exahype_amd/user_solver.hpp, as a user would build it: its own
+    shared object, independent of the library's."""
+    src = os.path.join(HERE, "examples", "user_euler.hip")
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-o", USER_EXAMPLE + ".tmp", src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building the user-solver example failed")
+    kernels = parse_resource_remarks(res.stderr)
+    spilling = sorted(k for k, r in kernels.items() if r.get("ScratchSize", 0) > 0)
+    if spilling:
+        raise RuntimeError("user-solver kernels spill to scratch memory:\n  " + "\n  ".join(spilling))
+    os.replace(USER_EXAMPLE + ".tmp", USER_EXAMPLE)
+    return USER_EXAMPLE
 
 
 if __name__ == "__main__":

@@ -22,6 +22,8 @@ STAGE_PREDICTOR, STAGE_RIEMANN, STAGE_UPDATE, STAGE_DT = range(4)
 STAGE_NAMES = ("predictor", "riemann", "update", "dt")
 BC_PERIODIC = 0
 BC_TRANSMISSIVE = 1
+BC_USER = 2
+PDE_USER = 100
 FV_GODUNOV, FV_MUSCLHANCOCK = 0, 1
 PRECISION_FP64, PRECISION_FP32_BULK = 0, 1
 PREDICTOR_AUTO, PREDICTOR_VALU_LDS64, PREDICTOR_VALU_LDS144, PREDICTOR_MFMA = 0, 1, 2, 3
@@ -41,7 +43,8 @@ EXPORTED = (
     "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
-    "exa_grid_set_predictor_variant", "exa_grid_set_fused_corrector", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
+    "exa_grid_set_predictor_variant", "exa_grid_set_fused_corrector", "exa_grid_create_user",
+    "exa_grid_set_adjust_solution", "exa_grid_adjust_solution", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
     "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy",
     "exa_godunov_solution_update", "exa_fvgrid_create", "exa_fvgrid_destroy", "exa_fvgrid_num_cells",
     "exa_fvgrid_upload", "exa_fvgrid_download", "exa_fvgrid_init_euler", "exa_fvgrid_step",
@@ -75,6 +78,16 @@ class Solver(ctypes.Structure):
     @property
     def D(self):
         return self.num_variables + self.num_parameters
+
+
+class UserSolverTable(ctypes.Structure):
+    """exa_user_solver: a user PDE compiled into the kernel templates
+    (include/exahype_amd/user_solver.hpp); the launch functions are opaque here."""
+    _fields_ = [("abi_version", ctypes.c_int), ("order", ctypes.c_int), ("num_variables", ctypes.c_int),
+                ("num_parameters", ctypes.c_int), ("terms", ctypes.c_int), ("cfl", ctypes.c_double),
+                ("has_boundary_values", ctypes.c_int), ("has_adjust", ctypes.c_int),
+                ("predictor", ctypes.c_void_p), ("corrector", ctypes.c_void_p), ("stable_dt", ctypes.c_void_p),
+                ("adjust", ctypes.c_void_p)]
 
 
 class EulerIC(ctypes.Structure):
@@ -167,6 +180,9 @@ def lib():
     L.exa_grid_set_profiling.argtypes = [V, ctypes.c_int]
     L.exa_grid_set_predictor_variant.argtypes = [V, ctypes.c_int]
     L.exa_grid_set_fused_corrector.argtypes = [V, ctypes.c_int]
+    L.exa_grid_create_user.argtypes = [ctypes.POINTER(UserSolverTable), I, I, I, I, _D, _D, I, ctypes.POINTER(V)]
+    L.exa_grid_set_adjust_solution.argtypes = [V, I]
+    L.exa_grid_adjust_solution.argtypes = [V]
     L.exa_grid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     L.exa_grid_download_cells.argtypes = [V, ctypes.c_long, ctypes.c_long, _D]
     L.exa_grid_picard_totals.argtypes = [V, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
@@ -469,6 +485,23 @@ def batch_predictor_volume(s, luh, dx, dt):
     return out
 
 
+USER_EXAMPLE_PATH = os.path.join(HERE, "libexa_user_example.so")
+_user_example = None
+
+
+def user_example():
+    """The example user solver (examples/user_euler.hip): an Euler PDE with a
+    reflecting-wall boundaryValues and an entropy-wave adjustPointSolution,
+    compiled through include/exahype_amd/user_solver.hpp into its own shared
+    object.  Tables: exa_user_euler_order3, exa_user_euler_order5."""
+    global _user_example
+    if _user_example is None:
+        if not os.path.exists(USER_EXAMPLE_PATH):
+            raise ExaError(f"{USER_EXAMPLE_PATH} is missing: build it with `python -m exahype_amd.build`")
+        _user_example = ctypes.CDLL(USER_EXAMPLE_PATH)
+    return _user_example
+
+
 # ---------------------------------------------------------------------------
 # Device-resident grid
 # ---------------------------------------------------------------------------
@@ -496,6 +529,39 @@ class Grid:
             self.close()
         except Exception:
             pass
+
+    @classmethod
+    def from_user_solver(cls, table_fn, n, precision=PRECISION_FP64, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                         boundary=BC_PERIODIC):
+        """A grid whose PDE is a user solver: table_fn is the ctypes function
+        EXA_USER_SOLVER exported (returns const exa_user_solver*)."""
+        table_fn.restype = ctypes.POINTER(UserSolverTable)
+        table_fn.argtypes = []
+        us = table_fn()
+        if not us:
+            raise ExaError("user solver table is NULL")
+        t = us.contents
+        if isinstance(n, int):
+            n = (n, n, n)
+        self = cls.__new__(cls)
+        self.s = Solver(PDE_USER, t.order, t.num_variables, t.num_parameters, t.terms, 0, t.cfl, precision, 0)
+        self.n = tuple(n)
+        self.user = t
+        h = ctypes.c_void_p()
+        check(lib().exa_grid_create_user(us, precision, n[0], n[1], n[2], _dp(_dx3(lo)), _dp(_dx3(hi)), boundary,
+                                         ctypes.byref(h)), "exa_grid_create_user")
+        self._h = h
+        self.ncells = lib().exa_grid_num_cells(h)
+        return self
+
+    def set_adjust_solution(self, on):
+        """adjustPointSolution after every update (user solvers with the hook)."""
+        check(lib().exa_grid_set_adjust_solution(self._h, int(on)), "adjust solution")
+
+    def adjust_solution(self):
+        """adjustPointSolution at the grid's time with dt = 0 on every cell (the
+        initial condition at t = 0)."""
+        check(lib().exa_grid_adjust_solution(self._h), "exa_grid_adjust_solution")
 
     def init_euler(self, rho0=1.0, amp=0.2, k=(1, 1, 1), vel=(1.0, 1.0, 1.0), p=1.0, nmodes=0,
                    mode_amp=0.0, seed=0):

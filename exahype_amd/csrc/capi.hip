@@ -416,7 +416,9 @@ struct exa_grid {
   long ncells;
   double lo[3], hi[3], dx[3];
   bool periodic = true;
+  int boundary = exa::kBoundaryPeriodic;
   bool fused_corrector = true;  // Riemann + surface + update + dt in one kernel (nonlinear solvers)
+  bool adjust = false;          // the PDE's adjust_point_solution after every update
   long nfaces;
   hipStream_t stream = nullptr;
   double* luh = nullptr;
@@ -430,6 +432,7 @@ struct exa_grid {
   long ws_cells = 0;
   size_t ws_bytes = 0;
   int predictor_variant = exa::kPredictorAuto;
+  const exa_user_solver* user = nullptr;  // a user solver's kernels (exa_grid_create_user)
   FaceDesc* faces = nullptr;
   exa::StepState* st = nullptr;
   double* dt_hist = nullptr;
@@ -489,7 +492,10 @@ hipError_t launch_initial_dt(exa_grid* g) {
   a.dt_slots = g->st->dt_slots;
   a.dt_cell = nullptr;
   a.cell0 = 0;
-  hipError_t e = staged(g, EXA_STAGE_UPDATE, [&] { return exa::launch_update(g->sd, 0, a, g->ncells, g->stream); });
+  hipError_t e = staged(g, EXA_STAGE_UPDATE, [&] {
+    return g->user ? (hipError_t)g->user->stable_dt(&a, g->ncells, g->stream)
+                   : exa::launch_update(g->sd, 0, a, g->ncells, g->stream);
+  });
   if (e != hipSuccess) return e;
   return staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, false, g->stream); });
 }
@@ -521,13 +527,16 @@ hipError_t launch_predict(exa_grid* g) {
   a.ws = g->ws;
   a.ws_cells = g->ws_cells;
   a.variant = g->predictor_variant;
-  hipError_t e = staged(g, EXA_STAGE_PREDICTOR, [&] { return exa::launch_predictor(g->sd, false, a, g->ncells, g->stream); });
+  hipError_t e = staged(g, EXA_STAGE_PREDICTOR, [&] {
+    return g->user ? (hipError_t)g->user->predictor(&a, g->ncells, g->sd.precision, g->stream)
+                   : exa::launch_predictor(g->sd, false, a, g->ncells, g->stream);
+  });
   if (e == hipSuccess && g->profiling) e = exa::launch_count_iterations(g->iters, g->ncells, g->st, g->stream);
   return e;
 }
 
 hipError_t launch_one_step(exa_grid* g) {
-  if (!g->sd.linear && g->fused_corrector) {
+  if (!g->sd.linear && (g->fused_corrector || g->user)) {
     exa::CorrectorArgs ca{};
     ca.luh = g->luh;
     ca.lduh = g->lduh;
@@ -537,11 +546,20 @@ hipError_t launch_one_step(exa_grid* g) {
     ca.dt = &g->st->dt;
     ca.cfl_pnpm = g->sd.cfl * kPNPM[g->sd.order];
     ca.dt_slots = g->st->dt_slots;
+    ca.t = &g->st->t;
+    for (int d = 0; d < 3; ++d) {
+      ca.lo[d] = g->lo[d];
+      ca.hdx[d] = g->dx[d];
+    }
     ca.nx = g->nx;
     ca.ny = g->ny;
     ca.nz = g->nz;
-    ca.periodic = g->periodic ? 1 : 0;
-    hipError_t e = staged(g, EXA_STAGE_UPDATE, [&] { return exa::launch_corrector(g->sd, ca, g->stream); });
+    ca.boundary = g->boundary;
+    ca.adjust = g->adjust ? 1 : 0;
+    hipError_t e = staged(g, EXA_STAGE_UPDATE, [&] {
+      return g->user ? (hipError_t)g->user->corrector(&ca, g->sd.precision, g->stream)
+                     : exa::launch_corrector(g->sd, ca, g->stream);
+    });
     if (e != hipSuccess) return e;
     e = staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, true, g->stream); });
     if (e != hipSuccess) return e;
@@ -574,15 +592,11 @@ hipError_t launch_one_step(exa_grid* g) {
 
 }  // namespace
 
-extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo,
-                               const double* hi, int boundary, exa_grid** out) {
-  if (!out) return EXA_E_INVALID;
-  *out = nullptr;
-  SolverDesc sd;
-  if (!to_desc(s, &sd) || nx < 1 || ny < 1 || nz < 1 || !lo || !hi) return EXA_E_INVALID;
-  if (boundary != EXA_BC_PERIODIC && boundary != EXA_BC_TRANSMISSIVE) return EXA_E_UNSUPPORTED;
+// Grid allocation and face lists, shared by the library's solvers and user
+// solvers (validated by the callers).
+static int grid_create_impl(const SolverDesc& sd, const exa_user_solver* user, int nx, int ny, int nz,
+                            const double* lo, const double* hi, int boundary, exa_grid** out) {
   const bool periodic = boundary == EXA_BC_PERIODIC;
-  if (sd.linear ? !exa::linear_supported(sd) : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
   // Linear solvers with transmissive boundaries: the linear Riemann solver's
   // fluctuations vanish at a boundary face (outside = inside), so those faces
   // join the face list to be zeroed.
@@ -590,6 +604,7 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
   if ((double)nx * ny * nz > 2.0e9) return EXA_E_INVALID;
   exa_grid* g = new exa_grid();
   g->sd = sd;
+  g->user = user;
   g->nx = nx;
   g->ny = ny;
   g->nz = nz;
@@ -599,6 +614,8 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
     g->hi[d] = hi[d];
   }
   g->periodic = periodic;
+  g->boundary = boundary == EXA_BC_PERIODIC ? exa::kBoundaryPeriodic
+                                            : (boundary == EXA_BC_USER ? exa::kBoundaryUser : exa::kBoundaryTransmissive);
   g->dx[0] = (hi[0] - lo[0]) / nx;
   g->dx[1] = (hi[1] - lo[1]) / ny;
   g->dx[2] = (hi[2] - lo[2]) / nz;
@@ -681,6 +698,74 @@ extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, cons
     return from_hip(e);
   }
   *out = g;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_create(const exa_solver* s, int nx, int ny, int nz, const double* lo,
+                               const double* hi, int boundary, exa_grid** out) {
+  if (!out) return EXA_E_INVALID;
+  *out = nullptr;
+  SolverDesc sd;
+  if (!to_desc(s, &sd) || nx < 1 || ny < 1 || nz < 1 || !lo || !hi) return EXA_E_INVALID;
+  if (boundary != EXA_BC_PERIODIC && boundary != EXA_BC_TRANSMISSIVE && boundary != EXA_BC_USER)
+    return EXA_E_UNSUPPORTED;
+  if (boundary == EXA_BC_USER && (sd.linear || !exa::corrector_supports(sd, exa::kBoundaryUser, false)))
+    return EXA_E_UNSUPPORTED;
+  if (sd.linear ? !exa::linear_supported(sd) : !exa::supported(sd)) return EXA_E_UNSUPPORTED;
+  return grid_create_impl(sd, nullptr, nx, ny, nz, lo, hi, boundary, out);
+}
+
+extern "C" int exa_grid_create_user(const exa_user_solver* us, int precision, int nx, int ny, int nz,
+                                    const double* lo, const double* hi, int boundary, exa_grid** out) {
+  if (!out) return EXA_E_INVALID;
+  *out = nullptr;
+  if (!us || nx < 1 || ny < 1 || nz < 1 || !lo || !hi) return EXA_E_INVALID;
+  if (us->abi_version != EXA_USER_SOLVER_ABI || !us->predictor || !us->corrector || !us->stable_dt ||
+      !us->adjust)
+    return EXA_E_INVALID;
+  if (us->order < 1 || us->order > 7 || us->num_variables < 1 || us->num_parameters < 0 ||
+      us->num_variables + us->num_parameters > 64 || !(us->cfl > 0))
+    return EXA_E_INVALID;
+  if (precision != EXA_PRECISION_FP64 && precision != EXA_PRECISION_FP32_BULK) return EXA_E_INVALID;
+  if (boundary != EXA_BC_PERIODIC && boundary != EXA_BC_TRANSMISSIVE && boundary != EXA_BC_USER)
+    return EXA_E_UNSUPPORTED;
+  if (boundary == EXA_BC_USER && !us->has_boundary_values) return EXA_E_UNSUPPORTED;
+  SolverDesc sd{};
+  sd.pde = exa::kPdeUser;
+  sd.order = us->order;
+  sd.nvar = us->num_variables;
+  sd.npar = us->num_parameters;
+  sd.terms = us->terms;
+  sd.cfl = us->cfl;
+  sd.precision = precision;
+  return grid_create_impl(sd, us, nx, ny, nz, lo, hi, boundary, out);
+}
+
+extern "C" int exa_grid_set_adjust_solution(exa_grid* g, int on) {
+  if (!g) return EXA_E_INVALID;
+  if (on && !(g->user && g->user->has_adjust)) return EXA_E_UNSUPPORTED;
+  g->adjust = on != 0;
+  if (on) g->fused_corrector = true;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_adjust_solution(exa_grid* g) {
+  if (!g) return EXA_E_INVALID;
+  if (!(g->user && g->user->has_adjust)) return EXA_E_UNSUPPORTED;
+  exa::AdjustArgs a{};
+  a.luh = g->luh;
+  a.t = &g->st->t;
+  a.dt = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    a.lo[d] = g->lo[d];
+    a.dx[d] = g->dx[d];
+  }
+  a.nx = g->nx;
+  a.ny = g->ny;
+  a.nz = g->nz;
+  EXA_TRY((hipError_t)g->user->adjust(&a, g->ncells, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  g->predicted = false;  // the solution changed: the next step starts with a new dt and prediction
   return EXA_OK;
 }
 
@@ -834,6 +919,8 @@ extern "C" int exa_grid_set_profiling(exa_grid* g, int on) {
 extern "C" int exa_grid_set_fused_corrector(exa_grid* g, int on) {
   if (!g) return EXA_E_INVALID;
   if (on && g->sd.linear) return EXA_E_UNSUPPORTED;
+  // boundary values and the per-step adjustment run in the one-kernel corrector
+  if (!on && (g->boundary == exa::kBoundaryUser || g->adjust || g->user)) return EXA_E_UNSUPPORTED;
   g->fused_corrector = on != 0;
   return EXA_OK;
 }

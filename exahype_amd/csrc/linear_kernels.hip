@@ -300,8 +300,8 @@ __global__ __launch_bounds__(256) void linear_riemann_kernel(RiemannArgs a) {
     QavR[k] = wave_sum(weight * qR[k]);
     Qav[k] = 0.5 * (QavL[k] + QavR[k]);
   }
-  const double sL = PDE::max_abs_eigenvalue(QavL, d);  // :68-78
-  const double sR = PDE::max_abs_eigenvalue(QavR, d);
+  const double sL = pde_max_abs_eigenvalue<PDE>(QavL, d);  // :68-78
+  const double sR = pde_max_abs_eigenvalue<PDE>(QavR, d);
   const double smax = (sL < sR) ? sR : sL;
   double dq[V], F[V];
 #pragma unroll

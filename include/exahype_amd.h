@@ -263,7 +263,11 @@ int exa_batch_predictor_volume(const exa_solver* s, long ncells, double* lQhbnd,
  * ------------------------------------------------------------------------- */
 typedef struct exa_grid exa_grid;
 
-enum { EXA_BC_PERIODIC = 0, EXA_BC_TRANSMISSIVE = 1 };
+/* EXA_BC_USER: the PDE's boundary_values (boundaryValues,
+ * UserSolverInterface.h:171) at every node of a boundary face, the outside
+ * state left of even faces and right of odd ones (ADERDGSolver.cpp:2952-2960);
+ * for PDEs that provide the hook (user PDEs, exahype_amd/user_solver.hpp). */
+enum { EXA_BC_PERIODIC = 0, EXA_BC_TRANSMISSIVE = 1, EXA_BC_USER = 2 };
 
 /* Uniform nx*ny*nz Cartesian grid on [lo, hi]; cell c = (k*ny + j)*nx + i;
  * boundary EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE (boundaryConditions,
@@ -329,6 +333,49 @@ int exa_grid_set_predictor_variant(exa_grid* g, int variant);
  * off: a face pass writing both cells' flux blocks, then the update pass.
  * Linear solvers always run the two passes (EXA_E_UNSUPPORTED for on). */
 int exa_grid_set_fused_corrector(exa_grid* g, int on);
+
+/* ---------------------------------------------------------------------------
+ * User PDEs on the device.  The reference's user solver implements flux,
+ * eigenvalues, nonConservativeProduct, algebraicSource, boundaryValues and
+ * adjustPointSolution (UserSolverInterface.h:64-171) and the generic kernels
+ * are templates over it.  Here a user writes those as a struct of static
+ * __device__ functions and instantiates the library's kernel templates for it
+ * in a shared object of their own with exahype_amd/user_solver.hpp
+ * (EXA_USER_SOLVER), which exports this table; the grid below drives it.
+ * ------------------------------------------------------------------------- */
+#define EXA_USER_SOLVER_ABI 1
+typedef struct exa_user_solver {
+  int abi_version;      /* EXA_USER_SOLVER_ABI */
+  int order;            /* SolverType::Order */
+  int num_variables;    /* SolverType::NumberOfVariables */
+  int num_parameters;   /* SolverType::NumberOfParameters */
+  int terms;            /* EXA_TERM_* */
+  double cfl;           /* SolverType::CFL */
+  int has_boundary_values;  /* boundaryValues compiled (EXA_BC_USER available) */
+  int has_adjust;           /* adjustPointSolution compiled */
+  /* Kernel launches on a HIP stream (passed as void*); the argument structs
+   * are the library's (include/exahype_amd/device/aderdg.hpp); the return
+   * value is a hipError_t (0: launched). */
+  int (*predictor)(const void* predictor_args, long ncells, int precision, void* stream);
+  int (*corrector)(const void* corrector_args, int precision, void* stream);
+  int (*stable_dt)(const void* update_args, long ncells, void* stream);
+  int (*adjust)(const void* adjust_args, long ncells, void* stream);
+} exa_user_solver;
+
+/* A grid (as exa_grid_create) whose PDE is a user solver; precision
+ * EXA_PRECISION_*; boundary EXA_BC_PERIODIC, EXA_BC_TRANSMISSIVE or, when the
+ * solver has boundary values, EXA_BC_USER.  The steps run the one-kernel
+ * corrector.  Initial data: exa_grid_upload_solution, or
+ * exa_grid_adjust_solution (adjustPointSolution at t = 0, as
+ * setInitialConditions does, ADERDGSolver.cpp:2161-2180). */
+int exa_grid_create_user(const exa_user_solver* us, int precision, int nx, int ny, int nz, const double* lo,
+                         const double* hi, int boundary, exa_grid** out);
+/* Apply the PDE's adjustPointSolution at every node after each update, at
+ * t + dt (ADERDGSolver.cpp:2231-2236); EXA_E_UNSUPPORTED without the hook. */
+int exa_grid_set_adjust_solution(exa_grid* g, int on);
+/* solutionAdjustment of every cell now, at the grid's time with dt = 0
+ * (solutionAdjustment.cpph:17-47); EXA_E_UNSUPPORTED without the hook. */
+int exa_grid_adjust_solution(exa_grid* g);
 /* Sum of per-cell Picard iteration counts and of cells over every predictor
  * launch made while profiling was on (the work of the predictor launches). */
 int exa_grid_picard_totals(exa_grid* g, unsigned long long* iterations, unsigned long long* cells);

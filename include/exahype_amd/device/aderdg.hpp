@@ -6,7 +6,9 @@
 
 namespace exa {
 
-enum : int { kPdeEuler = 0, kPdeElastic = 1 };
+// kPdeUser: a PDE compiled by the user (exahype_amd/user_solver.hpp); the
+// grid then launches the user's instantiations through exa_user_solver.
+enum : int { kPdeEuler = 0, kPdeElastic = 1, kPdeUser = 100 };
 // Terms of the PDE (the spec file's `terms`): flux, algebraic source, NCP.
 enum : int { kFlux = 1, kSource = 2, kNCP = 4 };
 
@@ -118,23 +120,46 @@ struct UpdateArgs {
   int slots_per_patch;
 };
 
-// The fused corrector of a uniform Cartesian grid (corrector_kernels.hip):
-// Riemann solves on each cell's six faces from the cell's and its neighbours'
-// face blocks, surface integral, solution update and the cell's stable dt
-// into dt_slots.  Cells are (k, j, i) row-major, periodic or transmissive.
+// Boundary treatment of a grid's outer faces.  kBoundaryTransmissive: the
+// outside state and flux are the inside ones; kBoundaryUser: the PDE's
+// boundary_values (UserSolverInterface.h:171) at every face node.
+enum : int { kBoundaryPeriodic = 0, kBoundaryTransmissive = 1, kBoundaryUser = 2 };
+
+// The fused corrector of a uniform Cartesian grid (corrector.hpp): Riemann
+// solves on each cell's six faces from the cell's and its neighbours' face
+// blocks (or the boundary values), surface integral, solution update,
+// optionally the PDE's adjust_point_solution at t + dt, and the cell's stable
+// dt into dt_slots.  Cells are (k, j, i) row-major on [lo, lo + n dx].
 struct CorrectorArgs {
   double* luh;
   const void* lduh;     // (bulk)
   const void* lQhbnd;   // (bulk)
   const void* lFhbnd;   // (bulk)
-  const double* dx;     // 3 doubles
-  const double* dt;     // device scalar
+  const double* dx;     // 3 doubles (device)
+  const double* dt;     // device scalar: the corrector time step size
+  const double* t;      // device scalar: the corrector time stamp
+  double lo[3];         // domain origin (node coordinates of boundary values / adjustment)
+  double hdx[3];        // the cell size on the host side of the call (node coordinates)
   double cfl_pnpm;      // CFL * PNPM[order]
   unsigned long long* dt_slots;
   int nx, ny, nz;
-  int periodic;
+  int boundary;         // kBoundary*
+  int adjust;           // apply the PDE's adjust_point_solution after the update
 };
 hipError_t launch_corrector(const SolverDesc& sd, const CorrectorArgs& a, hipStream_t s);
+// Whether the library's instantiation for this solver supports a boundary
+// treatment and the per-step adjustment (the library's PDEs have neither hook).
+bool corrector_supports(const SolverDesc& sd, int boundary, bool adjust);
+
+// solutionAdjustment (solutionAdjustment.cpph:17-47) of every cell of a grid:
+// the PDE's adjust_point_solution at every node, at time *t with step size dt_val.
+struct AdjustArgs {
+  double* luh;
+  const double* t;  // device scalar
+  double dt;
+  double lo[3], dx[3];
+  int nx, ny, nz;
+};
 
 struct VolumeArgs {
   double* lduh;

@@ -290,13 +290,81 @@ struct Elastic {
   }
 };
 
+// ---------------------------------------------------------------------------
+// PDE functor interface.  Required: V, P, and, templated on the arithmetic
+// type T (float and double), flux(Q, F[3][V]), algebraic_source(Q, S) and
+// ncp(Q, gradQ[3][V], BgradQ) as the terms the solver uses need them
+// (UserSolverInterface.h:64-127), plus either the reference's
+// eigenvalues(Q, d, lambda[V]) or max_abs_eigenvalue(Q, d).  Optional:
+//   boundary_values(x[3], t, dt, faceIndex, normalNonZero, Fin, Qin, Fout, Qout)
+//     (UserSolverInterface.h:171): the outside state and flux at a boundary
+//     face node of a grid with EXA_BC_USER;
+//   adjust_point_solution(x[3], t, dt, Q) (UserSolverInterface.h:149): applied
+//     at every node after each update of a grid that has it switched on.
+//
+// The hooks are __device__ functions.  Each test below runs inside a
+// __host__ __device__ constexpr function: a requires-expression met first in
+// host code sees a __device__ function template as not callable, and the
+// concept's cached answer is then false for the whole translation unit.
+// ---------------------------------------------------------------------------
+namespace hook_detect {
+template <class PDE, typename T>
+__host__ __device__ constexpr bool max_abs_eigenvalue() {
+  return requires(const T* q, int d) { PDE::max_abs_eigenvalue(q, d); };
+}
+template <class PDE, typename T>
+__host__ __device__ constexpr bool eigenvalues() {
+  return requires(const T* q, int d, T* l) { PDE::eigenvalues(q, d, l); };
+}
+template <class PDE, typename T>
+__host__ __device__ constexpr bool boundary_values() {
+  return requires(const double* x, double t, int f, const T* fin, const T* qin, T* fo, T* qo) {
+    PDE::boundary_values(x, t, t, f, f, fin, qin, fo, qo);
+  };
+}
+template <class PDE>
+__host__ __device__ constexpr bool adjust_point_solution() {
+  return requires(const double* x, double t, double* q) { PDE::adjust_point_solution(x, t, t, q); };
+}
+}  // namespace hook_detect
+
+template <class PDE, typename T>
+concept HasMaxAbsEigenvalue = hook_detect::max_abs_eigenvalue<PDE, T>();
+template <class PDE, typename T>
+concept HasEigenvalues = hook_detect::eigenvalues<PDE, T>();
+template <class PDE, typename T>
+concept HasBoundaryValues = hook_detect::boundary_values<PDE, T>();
+template <class PDE>
+concept HasAdjustPointSolution = hook_detect::adjust_point_solution<PDE>();
+
+// Largest |eigenvalue| in direction d: the PDE's own, or the maximum of
+// |lambda| over the eigenvalues (stableTimeStepSize.cpph:44-52,
+// riemannSolverNonlinear.cpph:68-78: the same value for finite lambda).
+template <class PDE, typename T>
+__device__ inline T pde_max_abs_eigenvalue(const T* Q, int d) {
+  if constexpr (HasMaxAbsEigenvalue<PDE, T>) {
+    return PDE::max_abs_eigenvalue(Q, d);
+  } else {
+    static_assert(HasEigenvalues<PDE, T>, "the PDE needs eigenvalues() or max_abs_eigenvalue()");
+    T lambda[PDE::V];
+    PDE::eigenvalues(Q, d, lambda);
+    T m = fabs(lambda[0]);
+#pragma unroll
+    for (int v = 1; v < PDE::V; ++v) {
+      const T a = fabs(lambda[v]);
+      m = (m < a) ? a : m;
+    }
+    return m;
+  }
+}
+
 // Stable time step of one node, stableTimeStepSize.cpph:37-57 (the caller takes
 // the minimum).  maxEigenvalue = max over variables of |lambda|.
 template <class PDE>
 __device__ inline double node_stable_dt(const double* Q, const double invDx[3], double cfl_pnpm) {
   double denominator = 0.0;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) denominator += PDE::max_abs_eigenvalue(Q, d) * invDx[d];
+  for (int d = 0; d < 3; ++d) denominator += pde_max_abs_eigenvalue<PDE>(Q, d) * invDx[d];
   return cfl_pnpm / denominator;
 }
 

@@ -233,6 +233,61 @@ void pointSource(SolverType& solver, const double t, const double dt, const DxVe
   exahype_amd::check(st, "pointSource");
 }
 
+// boundaryConditions (Kernels.h:145, 3d/boundaryConditions.cpph:24-85): the
+// user's boundaryValues at every node of a boundary face, on the host, as the
+// reference calls it.  Face blocks are [i][j][var] with i along the second
+// and j along the first free direction.  Grids with EXA_BC_USER call a
+// compiled boundary_values on the device instead (exahype_amd/user_solver.hpp).
+template <typename SolverType, typename DxVector>
+void boundaryConditions(SolverType& solver, double* FOut, double* QOut, const double* const FIn,
+                        const double* const QIn, const DxVector& cellCentre, const DxVector& cellSize,
+                        const double t, const double dt, const int faceIndex, const int normalNonZero) {
+  constexpr int V = SolverType::NumberOfVariables;
+  constexpr int D = V + SolverType::NumberOfParameters;
+  constexpr int order = SolverType::Order;
+  constexpr int B = order + 1;
+  if (normalNonZero < 0 || normalNonZero > 2) throw exahype_amd::Error("boundaryConditions: normalNonZero must be 0, 1 or 2");
+  const int f = faceIndex - 2 * normalNonZero;
+  const int d1 = normalNonZero == 0 ? 1 : 0;
+  const int d2 = normalNonZero == 2 ? 1 : 2;
+  double x[3];
+  x[normalNonZero] = cellCentre[normalNonZero] + (-0.5 + f) * cellSize[normalNonZero];
+  for (int i = 0; i < B; ++i) {
+    x[d2] = cellCentre[d2] + cellSize[d2] * (exa_quadrature_node(order, i) - 0.5);
+    for (int j = 0; j < B; ++j) {
+      x[d1] = cellCentre[d1] + cellSize[d1] * (exa_quadrature_node(order, j) - 0.5);
+      solver.boundaryValues(x, t, dt, faceIndex, normalNonZero, FIn + (i * B + j) * V, QIn + (i * B + j) * D,
+                            FOut + (i * B + j) * V, QOut + (i * B + j) * D);
+    }
+  }
+}
+
+// solutionAdjustment (Kernels.h:128, 3d/solutionAdjustment.cpph:17-47): the
+// user's adjustPointSolution at every node of luh [z][y][x][var], on the
+// host.  Grids call a compiled adjust_point_solution on the device instead
+// (exa_grid_set_adjust_solution, exa_grid_adjust_solution).
+template <typename SolverType, typename DxVector>
+void solutionAdjustment(SolverType& solver, double* luh, const DxVector& center, const DxVector& dx, const double t,
+                        const double dt) {
+  constexpr int D = SolverType::NumberOfVariables + SolverType::NumberOfParameters;
+  constexpr int order = SolverType::Order;
+  constexpr int B = order + 1;
+  double x[3];
+  for (int i = 0; i < B; ++i) {
+    const double qi = exa_quadrature_node(order, i);
+    for (int j = 0; j < B; ++j) {
+      const double qj = exa_quadrature_node(order, j);
+      for (int k = 0; k < B; ++k) {
+        const double qk = exa_quadrature_node(order, k);
+        x[2] = center[2] + dx[2] * (qi - 0.5);
+        x[1] = center[1] + dx[1] * (qj - 0.5);
+        x[0] = center[0] + dx[0] * (qk - 0.5);
+        solver.adjustPointSolution(x, t, dt, luh + ((i * B + j) * B + k) * D);
+      }
+    }
+  }
+}
+
 // ---- AMR transfer operators (Kernels.h:163-199, amrRoutines.cpph) ----
 
 template <int numberOfVariables, int numberOfParameters, int basisSize, typename FaceIndex>

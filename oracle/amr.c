@@ -20,7 +20,7 @@
 #include "oracle.h"
 
 #define EXA_TABLE static const
-#include "../exahype_amd/csrc/dg_tables.inc"
+#include "../include/exahype_amd/device/dg_tables.inc"
 #undef EXA_TABLE
 
 double ora_fine_proj(int order, int j, int n, int m) { return exa_fine_proj[order][j][n][m]; }

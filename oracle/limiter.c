@@ -31,7 +31,7 @@
 #endif
 
 #define EXA_TABLE static const
-#include "../exahype_amd/csrc/dg_tables.inc"
+#include "../include/exahype_amd/device/dg_tables.inc"
 
 static inline double std_min(double a, double b) { return (b < a) ? b : a; }
 static inline double std_max(double a, double b) { return (a < b) ? b : a; }

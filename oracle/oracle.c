@@ -21,7 +21,7 @@
 #endif
 
 #define EXA_TABLE static const
-#include "../exahype_amd/csrc/dg_tables.inc"
+#include "../include/exahype_amd/device/dg_tables.inc"
 
 #define W(o, i) exa_gl_weights[o][i]
 #define FLC(o, i) exa_fl_coeff[o][i]
@@ -769,6 +769,86 @@ double ora_stable_time_step_size(const ora_solver* s, const double* luh, const d
 }
 
 /* ------------------------------------------------------------------------ */
+/* boundaryConditions (K/c/3d/boundaryConditions.cpph:24-85): the user's     */
+/* boundaryValues at every node of the face, x on the face and the two free   */
+/* coordinates from the nodes (i along d2, j along d1).                       */
+/* ------------------------------------------------------------------------ */
+void ora_boundary_conditions(const ora_solver* s, ora_boundary_values_fn bv, double* FOut, double* QOut,
+                             const double* FIn, const double* QIn, const double* centre, const double* size,
+                             double t, double dt, int faceIndex, int normalNonZero) {
+  const int V = s->nvar, D = s->nvar + s->npar, B = s->order + 1;
+  const int f = faceIndex - 2 * normalNonZero;
+  int d1 = 0, d2 = 0;
+  switch (normalNonZero) {
+    case 0: d1 = 1; d2 = 2; break;
+    case 1: d1 = 0; d2 = 2; break;
+    default: d1 = 0; d2 = 1; break;
+  }
+  double x[3];
+  x[normalNonZero] = centre[normalNonZero] + (-0.5 + f) * size[normalNonZero];
+  for (int i = 0; i < B; i++) {
+    x[d2] = centre[d2] + size[d2] * (ora_table(ORA_T_NODES, s->order, i, 0) - 0.5);
+    for (int j = 0; j < B; j++) {
+      x[d1] = centre[d1] + size[d1] * (ora_table(ORA_T_NODES, s->order, j, 0) - 0.5);
+      bv(s, x, t, dt, faceIndex, normalNonZero, FIn + (i * B + j) * V, QIn + (i * B + j) * D,
+         FOut + (i * B + j) * V, QOut + (i * B + j) * D);
+    }
+  }
+}
+
+/* solutionAdjustment (K/c/3d/solutionAdjustment.cpph:17-47): the user's     */
+/* adjustPointSolution at every node, x = centre + dx (xi - 1/2).            */
+void ora_solution_adjustment(const ora_solver* s, ora_adjust_fn adj, double* luh, const double* centre,
+                             const double* dx, double t, double dt) {
+  const int D = s->nvar + s->npar, B = s->order + 1;
+  double x[3];
+  for (int i = 0; i < B; i++) {
+    const double qi = ora_table(ORA_T_NODES, s->order, i, 0);
+    for (int j = 0; j < B; j++) {
+      const double qj = ora_table(ORA_T_NODES, s->order, j, 0);
+      for (int k = 0; k < B; k++) {
+        const double qk = ora_table(ORA_T_NODES, s->order, k, 0);
+        x[2] = centre[2] + dx[2] * (qi - 0.5);
+        x[1] = centre[1] + dx[1] * (qj - 0.5);
+        x[0] = centre[0] + dx[0] * (qk - 0.5);
+        adj(s, x, t, dt, luh + ((i * B + j) * B + k) * D);
+      }
+    }
+  }
+}
+
+/* The test solver's boundaryValues: the reference Euler demonstrator's      */
+/* reflecting wall (Demonstrators/EulerADERDG/MyEulerSolver_ADERDG.cpp:       */
+/* 63-88): the inside state with the normal momentum negated, its normal flux. */
+void ora_bv_reflecting_wall(const ora_solver* s, const double* x, double t, double dt, int faceIndex,
+                            int normalNonZero, const double* FIn, const double* QIn, double* FOut, double* QOut) {
+  (void)x; (void)t; (void)dt; (void)faceIndex; (void)FIn;
+  const int V = s->nvar;
+  for (int m = 0; m < V; m++) QOut[m] = QIn[m];
+  QOut[1 + normalNonZero] = -QOut[1 + normalNonZero];
+  double f0[16], f1[16], f2[16];
+  double* F[3] = {f0, f1, f2};
+  ora_flux(s, QOut, F);
+  for (int m = 0; m < V; m++) FOut[m] = F[normalNonZero][m];
+}
+
+/* The test solver's adjustPointSolution: at t = 0 the entropy wave          */
+/* rho = 1 + 0.2 sin(2 pi (x + y + z)), u = v = w = 1, p = 1; afterwards      */
+/* Q += dt/2 (Q_0(x) - Q).                                                    */
+void ora_adjust_entropy_wave(const ora_solver* s, const double* x, double t, double dt, double* Q) {
+  const double twopi = 6.283185307179586476925286766559;
+  const double rho = 1.0 + 0.2 * sin(twopi * (1.0 * x[0] + 1.0 * x[1] + 1.0 * x[2]));
+  const double Q0[5] = {rho, rho * 1.0, rho * 1.0, rho * 1.0,
+                        1.0 / (1.4 - 1.0) + 0.5 * rho * (1.0 * 1.0 + 1.0 * 1.0 + 1.0 * 1.0)};
+  const int V = s->nvar < 5 ? s->nvar : 5;
+  if (t == 0.0) {
+    for (int m = 0; m < V; m++) Q[m] = Q0[m];
+  } else {
+    for (int m = 0; m < V; m++) Q[m] += 0.5 * dt * (Q0[m] - Q[m]);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Grid driver: non-fused global time stepping on a uniform periodic grid.   */
 /* Restates the phase order of Runner::runOneTimeStepWithThreeSeparate-      */
 /* AlgorithmicSteps (ExaHyPE/exahype/runners/Runner.cpp:1116-1162):           */
@@ -783,6 +863,9 @@ struct ora_grid {
   ora_solver s;
   int nx, ny, nz;
   int transmissive;
+  int user_boundary;  /* boundary faces: the test solver's boundaryValues (reflecting wall) */
+  int adjust;         /* the test solver's adjustPointSolution after every update */
+  double lo[3];
   long ncells;
   double dx[3];
   double t, dt;
@@ -807,6 +890,9 @@ ora_grid* ora_grid_create_bc(const ora_solver* s, int nx, int ny, int nz, const 
                              const double* hi, int transmissive) {
   ora_grid* g = (ora_grid*)calloc(1, sizeof(ora_grid));
   g->transmissive = transmissive;
+  g->lo[0] = lo[0];
+  g->lo[1] = lo[1];
+  g->lo[2] = lo[2];
   g->s = *s;
   g->nx = nx;
   g->ny = ny;
@@ -940,6 +1026,13 @@ static void riemann_face(ora_grid* g, long cl, long cr, int d) {
     ora_riemann_solver_nonlinear(s, FL, FR, QL, QR, g->dt, d);
 }
 
+static void cell_centre(const ora_grid* g, long c, double* centre) {
+  const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
+  centre[0] = g->lo[0] + (i + 0.5) * g->dx[0];
+  centre[1] = g->lo[1] + (j + 0.5) * g->dx[1];
+  centre[2] = g->lo[2] + (k + 0.5) * g->dx[2];
+}
+
 static void boundary_face(ora_grid* g, long c, int face) {
   const ora_solver* s = &g->s;
   const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
@@ -949,6 +1042,12 @@ static void boundary_face(ora_grid* g, long c, int face) {
   double FOut[10 * 10 * 16], QOut[10 * 10 * 16];
   memcpy(FOut, FIn, sizeof(double) * B2 * V);
   memcpy(QOut, QIn, sizeof(double) * B2 * D);
+  if (g->user_boundary) {
+    double centre[3];
+    cell_centre(g, c, centre);
+    ora_boundary_conditions(s, ora_bv_reflecting_wall, FOut, QOut, FIn, QIn, centre, g->dx, g->t, g->dt, face,
+                            face / 2);
+  }
   if (s->linear) {
     if (face % 2 == 0)
       ora_riemann_solver_linear(s, FOut, FIn, QOut, QIn, g->dt, face / 2);
@@ -972,7 +1071,29 @@ static void update_cells(ora_grid* g, long c0, long c1) {
     else
       ora_surface_integral_nonlinear(s, g->lduh + c * B3 * V, g->lFhbnd + c * 6 * B2 * V, g->dx);
     ora_solution_update(s, g->luh + c * B3 * D, g->lduh + c * B3 * V, g->dt);
+    if (g->adjust) { /* adjustSolution(t + dt, dt) after the update (ADERDGSolver.cpp:2231-2236) */
+      double centre[3];
+      cell_centre(g, c, centre);
+      ora_solution_adjustment(s, ora_adjust_entropy_wave, g->luh + c * B3 * D, centre, g->dx, g->t + g->dt, g->dt);
+    }
   }
+}
+
+void ora_grid_set_user(ora_grid* g, int user_boundary, int adjust) {
+  g->user_boundary = user_boundary;
+  if (user_boundary) g->transmissive = 1; /* the boundary faces are not wrapped */
+  g->adjust = adjust;
+}
+
+void ora_grid_adjust(ora_grid* g) {
+  const long B = g->s.order + 1, D = g->s.nvar + g->s.npar;
+  const long B3 = B * B * B;
+  for (long c = 0; c < g->ncells; c++) {
+    double centre[3];
+    cell_centre(g, c, centre);
+    ora_solution_adjustment(&g->s, ora_adjust_entropy_wave, g->luh + c * B3 * D, centre, g->dx, g->t, 0.0);
+  }
+  g->have_prediction = 0;
 }
 
 void ora_grid_step(ora_grid* g, int nsteps, int nthreads, double* dt_out, int* iters_out) {

@@ -128,6 +128,32 @@ void ora_grid_picard_iterations(const ora_grid* g, int* out);
  * dt) for the first m cells; solution of those cells is advanced. */
 void ora_grid_sample_step(ora_grid* g, long m, int nthreads);
 
+/* ---- user hooks: boundaryValues and adjustPointSolution ---- */
+typedef void (*ora_boundary_values_fn)(const ora_solver* s, const double* x, double t, double dt, int faceIndex,
+                                       int normalNonZero, const double* FIn, const double* QIn, double* FOut,
+                                       double* QOut);
+typedef void (*ora_adjust_fn)(const ora_solver* s, const double* x, double t, double dt, double* Q);
+/* boundaryConditions: bv at every node of face faceIndex of the cell at
+ * centre with size; FOut/QOut [B][B][V|D] (i along d2, j along d1). */
+void ora_boundary_conditions(const ora_solver* s, ora_boundary_values_fn bv, double* FOut, double* QOut,
+                             const double* FIn, const double* QIn, const double* centre, const double* size,
+                             double t, double dt, int faceIndex, int normalNonZero);
+/* solutionAdjustment: adj at every node of luh (z,y,x,var). */
+void ora_solution_adjustment(const ora_solver* s, ora_adjust_fn adj, double* luh, const double* centre,
+                             const double* dx, double t, double dt);
+/* test hooks: the Euler demonstrator's reflecting wall; the entropy wave
+ * rho = 1 + 0.2 sin(2 pi (x+y+z)), u = v = w = 1, p = 1 at t = 0 and the
+ * relaxation Q += dt/2 (Q_0(x) - Q) afterwards */
+void ora_bv_reflecting_wall(const ora_solver* s, const double* x, double t, double dt, int faceIndex,
+                            int normalNonZero, const double* FIn, const double* QIn, double* FOut, double* QOut);
+void ora_adjust_entropy_wave(const ora_solver* s, const double* x, double t, double dt, double* Q);
+/* grid with the test hooks: user_boundary = the reflecting wall on every
+ * boundary face (non-periodic), adjust = the entropy-wave adjustment after
+ * every update at t + dt */
+void ora_grid_set_user(ora_grid* g, int user_boundary, int adjust);
+/* adjustSolution(t, 0) on every cell (the initial condition at t = 0) */
+void ora_grid_adjust(ora_grid* g);
+
 /* ---- limiter path (limiter.c) ---- */
 /* Projection tables of one order: uh2lim (B x L), uh2lob (B x B), lim2uh
  * (L x B), L = 2N+1.  Returns 0 if the least-squares matrix is singular. */

@@ -71,6 +71,14 @@ def lib():
         L.ora_grid_time.restype = ctypes.c_double
         L.ora_grid_picard_iterations.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.ora_grid_sample_step.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int]
+        L.ora_grid_set_user.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.ora_grid_adjust.argtypes = [ctypes.c_void_p]
+        L.ora_boundary_conditions.argtypes = [S, ctypes.c_void_p, P, P, P, P, P, P, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.ora_solution_adjustment.argtypes = [S, ctypes.c_void_p, P, P, P, ctypes.c_double, ctypes.c_double]
+        L.ora_bv_reflecting_wall.argtypes = [S, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                             P, P, P, P]
+        L.ora_adjust_entropy_wave.argtypes = [S, P, ctypes.c_double, ctypes.c_double, P]
         I = ctypes.c_int
         IP = ctypes.POINTER(ctypes.c_int)
         D_ = ctypes.c_double
@@ -287,6 +295,47 @@ class Grid:
 
     def sample_step(self, m, nthreads=0):
         lib().ora_grid_sample_step(self._h, m, nthreads)
+
+    def set_user(self, user_boundary, adjust):
+        """The test hooks: reflecting walls on every boundary face, and/or the
+        entropy-wave adjustPointSolution after every update."""
+        lib().ora_grid_set_user(self._h, int(user_boundary), int(adjust))
+
+    def adjust(self):
+        """adjustSolution(t, 0) on every cell (the initial condition at t = 0)."""
+        lib().ora_grid_adjust(self._h)
+
+
+# ---------------------------------------------------------------------------
+# user hooks: boundaryConditions / solutionAdjustment with a callback
+# ---------------------------------------------------------------------------
+BV_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double))
+ADJ_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                          ctypes.c_double, ctypes.POINTER(ctypes.c_double))
+
+
+def boundary_conditions(s, bv, FIn, QIn, centre, size, t, dt, face):
+    """ora_boundary_conditions with bv a BV_FN (a Python callback) or None for
+    the reflecting wall; returns (FOut, QOut) [B*B*V], [B*B*D]."""
+    FIn = np.ascontiguousarray(FIn, dtype=np.float64)
+    QIn = np.ascontiguousarray(QIn, dtype=np.float64)
+    FOut, QOut = FIn.copy(), QIn.copy()
+    fn = ctypes.cast(lib().ora_bv_reflecting_wall, ctypes.c_void_p) if bv is None else ctypes.cast(bv, ctypes.c_void_p)
+    lib().ora_boundary_conditions(ctypes.byref(s), fn, _p(FOut), _p(QOut), _p(FIn), _p(QIn), _p(dx3(centre)),
+                                  _p(dx3(size)), t, dt, face, face // 2)
+    return FOut, QOut
+
+
+def solution_adjustment(s, adj, luh, centre, dx, t, dt):
+    """ora_solution_adjustment with adj an ADJ_FN or None for the entropy
+    wave; returns the adjusted copy of luh."""
+    out = np.ascontiguousarray(luh, dtype=np.float64).copy()
+    fn = ctypes.cast(lib().ora_adjust_entropy_wave, ctypes.c_void_p) if adj is None else ctypes.cast(adj, ctypes.c_void_p)
+    lib().ora_solution_adjustment(ctypes.byref(s), fn, _p(out), _p(dx3(centre)), _p(dx3(dx)), t, dt)
+    return out
 
 
 def _ip(a):

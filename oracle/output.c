@@ -15,7 +15,7 @@
 #include "oracle.h"
 
 #define EXA_TABLE static const
-#include "../exahype_amd/csrc/dg_tables.inc"
+#include "../include/exahype_amd/device/dg_tables.inc"
 #undef EXA_TABLE
 
 double ora_equi_proj(int order, int n, int m) { return exa_equi_proj[order][n][m]; }

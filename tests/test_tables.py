@@ -68,7 +68,7 @@ def test_tables_are_the_reference_literals():
     if not os.path.exists(ref):
         pytest.skip("reference sources not present on this machine")
     here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    inc = open(os.path.join(here, "exahype_amd", "csrc", "dg_tables.inc")).read()
+    inc = open(os.path.join(here, "include", "exahype_amd", "device", "dg_tables.inc")).read()
 
     def block(name):
         m = re.search(r"EXA_TABLE double " + name + r"\[[^=]*=\s*\{(.*?)\};", inc, re.S)

@@ -44,6 +44,7 @@ EXPORTED = (
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
     "exa_grid_set_predictor_variant", "exa_grid_set_fused_corrector", "exa_grid_create_user",
+    "exa_grid_set_fused_time_stepping", "exa_grid_predictor_reruns",
     "exa_grid_set_adjust_solution", "exa_grid_adjust_solution", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
     "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy",
     "exa_godunov_solution_update", "exa_fvgrid_create", "exa_fvgrid_destroy", "exa_fvgrid_num_cells",
@@ -182,6 +183,8 @@ def lib():
     L.exa_grid_set_fused_corrector.argtypes = [V, ctypes.c_int]
     L.exa_grid_create_user.argtypes = [ctypes.POINTER(UserSolverTable), I, I, I, I, _D, _D, I, ctypes.POINTER(V)]
     L.exa_grid_set_adjust_solution.argtypes = [V, I]
+    L.exa_grid_set_fused_time_stepping.argtypes = [V, I, ctypes.c_double]
+    L.exa_grid_predictor_reruns.argtypes = [V, ctypes.POINTER(Lg)]
     L.exa_grid_adjust_solution.argtypes = [V]
     L.exa_grid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     L.exa_grid_download_cells.argtypes = [V, ctypes.c_long, ctypes.c_long, _D]
@@ -625,6 +628,15 @@ class Grid:
     def set_fused_corrector(self, on):
         """One-kernel corrector (Riemann + surface + update + dt) or the two passes."""
         check(lib().exa_grid_set_fused_corrector(self._h, int(on)), "fused corrector")
+
+    def set_fused_time_stepping(self, on=True, weight=0.9):
+        """fuse-algorithmic-steps with fuse-algorithmic-steps-factor weight."""
+        check(lib().exa_grid_set_fused_time_stepping(self._h, int(on), float(weight)), "fused time stepping")
+
+    def predictor_reruns(self):
+        n = ctypes.c_long()
+        check(lib().exa_grid_predictor_reruns(self._h, ctypes.byref(n)), "predictor reruns")
+        return n.value
 
     def set_predictor_variant(self, variant):
         """EXA_PREDICTOR_AUTO (0), _VALU_LDS64 (1), _VALU_LDS144 (2) or _MFMA (3)."""

@@ -37,6 +37,55 @@ __global__ __launch_bounds__(256) void dt_finalize_kernel(StepState* st, int adv
   }
 }
 
+// Fused time stepping (fuse-algorithmic-steps), the end of a sweep: stable =
+// min(slots), the admissible dt of the updated solution; the stability check
+// of the dt the sweep's prediction used and the next predictor dt
+// (TimeStepSizeComputation.cpp:167-204), then startNewTimeStep
+// (ADERDGSolver.cpp:560-592): t += dt, dt <- dtp, dtp <- next.  init: the
+// start (updateTimeStepSizesFused, ADERDGSolver.cpp:594-616), dt = dtp =
+// stable.  Contraction off: the oracle's arithmetic, so that the check takes
+// the same branch.
+__global__ __launch_bounds__(256) void dt_finalize_fused_kernel(StepState* st, double weight, int init) {
+#pragma clang fp contract(off)
+  __shared__ double sRed[256 / kWave];
+  double m = 1.0e308;
+  for (int i = threadIdx.x; i < kDtSlots; i += 256) {
+    const double v = __longlong_as_double((long long)atomic_take(&st->dt_slots[i], kDtSlotEmpty));
+    m = v < m ? v : m;
+  }
+  m = block_min<256>(m, sRed);
+  if (threadIdx.x != 0) return;
+  if (init) {
+    st->dt = m;
+    st->dtp = m;
+    st->violated = 0;
+  } else {
+    const double stable = m;
+    double next = stable;
+    double used = st->dtp;
+    if (used == 0.0) used = stable;
+    const int violated = used > stable ? 1 : 0;
+    if (violated) {
+      next = fmin(next, weight * stable);
+      st->dtp = weight * stable;
+    } else {
+      next = fmin(next, 0.5 * (used + weight * stable));
+    }
+    st->t += st->dt;
+    st->steps += 1;
+    st->dt = st->dtp;
+    st->dtp = next;
+    st->violated = violated;
+    st->reruns += violated;
+  }
+  if (st->dt_hist && st->steps < st->dt_hist_len) st->dt_hist[st->steps] = st->dt;
+}
+
+hipError_t launch_dt_finalize_fused(StepState* st, double weight, bool init, hipStream_t s) {
+  hipLaunchKernelGGL(dt_finalize_fused_kernel, dim3(1), dim3(256), 0, s, st, weight, init ? 1 : 0);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Stand-alone per-cell kernels for the drop-in single-cell / batch API (FP64).
 // ---------------------------------------------------------------------------

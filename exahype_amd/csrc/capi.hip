@@ -419,6 +419,8 @@ struct exa_grid {
   int boundary = exa::kBoundaryPeriodic;
   bool fused_corrector = true;  // Riemann + surface + update + dt in one kernel (nonlinear solvers)
   bool adjust = false;          // the PDE's adjust_point_solution after every update
+  bool fused_ts = false;        // fused time stepping (fuse-algorithmic-steps)
+  double fused_weight = 0.9;    // fuse-algorithmic-steps-factor
   long nfaces;
   hipStream_t stream = nullptr;
   double* luh = nullptr;
@@ -497,10 +499,15 @@ hipError_t launch_initial_dt(exa_grid* g) {
                    : exa::launch_update(g->sd, 0, a, g->ncells, g->stream);
   });
   if (e != hipSuccess) return e;
-  return staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, false, g->stream); });
+  return staged(g, EXA_STAGE_DT, [&] {
+    return g->fused_ts ? exa::launch_dt_finalize_fused(g->st, g->fused_weight, true, g->stream)
+                       : exa::launch_dt_finalize(g->st, false, g->stream);
+  });
 }
 
-hipError_t launch_predict(exa_grid* g) {
+// The prediction with the time step size at dt (device scalar); run_if (a
+// device flag, may be null) gates the whole launch.
+hipError_t launch_predict(exa_grid* g, const double* dt, const int* run_if = nullptr) {
   if (g->sd.linear) {  // Cauchy-Kowalevski predictor + volumeIntegralLinear
     exa::LinearPredictorArgs la{};
     la.luh = g->luh;
@@ -509,8 +516,9 @@ hipError_t launch_predict(exa_grid* g) {
     la.lFhbnd = static_cast<double*>(g->lFhbnd);
     la.dx = g->d_dx;
     la.dx_stride = 0;
-    la.dt = &g->st->dt;
+    la.dt = dt;
     la.cell0 = 0;
+    if (run_if) return hipErrorNotSupported;  // fused time stepping is for nonlinear solvers
     return staged(g, EXA_STAGE_PREDICTOR,
                   [&] { return exa::launch_linear_predictor(g->sd, false, la, g->ncells, g->stream); });
   }
@@ -521,12 +529,13 @@ hipError_t launch_predict(exa_grid* g) {
   a.lFhbnd = g->lFhbnd;
   a.dx = g->d_dx;
   a.dx_stride = 0;
-  a.dt = &g->st->dt;
+  a.dt = dt;
   a.iters = g->iters;
   a.cell0 = 0;
   a.ws = g->ws;
   a.ws_cells = g->ws_cells;
   a.variant = g->predictor_variant;
+  a.run_if = run_if;
   hipError_t e = staged(g, EXA_STAGE_PREDICTOR, [&] {
     return g->user ? (hipError_t)g->user->predictor(&a, g->ncells, g->sd.precision, g->stream)
                    : exa::launch_predictor(g->sd, false, a, g->ncells, g->stream);
@@ -535,7 +544,9 @@ hipError_t launch_predict(exa_grid* g) {
   return e;
 }
 
-hipError_t launch_one_step(exa_grid* g) {
+// The corrector of one step: Riemann problems, surface integral, update with
+// the corrector dt, the admissible dt of every cell into the dt slots.
+hipError_t launch_correct(exa_grid* g) {
   if (!g->sd.linear && (g->fused_corrector || g->user)) {
     exa::CorrectorArgs ca{};
     ca.luh = g->luh;
@@ -556,14 +567,10 @@ hipError_t launch_one_step(exa_grid* g) {
     ca.nz = g->nz;
     ca.boundary = g->boundary;
     ca.adjust = g->adjust ? 1 : 0;
-    hipError_t e = staged(g, EXA_STAGE_UPDATE, [&] {
+    return staged(g, EXA_STAGE_UPDATE, [&] {
       return g->user ? (hipError_t)g->user->corrector(&ca, g->sd.precision, g->stream)
                      : exa::launch_corrector(g->sd, ca, g->stream);
     });
-    if (e != hipSuccess) return e;
-    e = staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, true, g->stream); });
-    if (e != hipSuccess) return e;
-    return launch_predict(g);
   }
   exa::RiemannArgs ra{g->faces, g->nfaces, g->lQhbnd, g->lFhbnd};
   hipError_t e = staged(g, EXA_STAGE_RIEMANN, [&] {
@@ -581,13 +588,32 @@ hipError_t launch_one_step(exa_grid* g) {
   ua.dt_slots = g->st->dt_slots;
   ua.dt_cell = nullptr;
   ua.cell0 = 0;
-  e = staged(g, EXA_STAGE_UPDATE, [&] {
+  return staged(g, EXA_STAGE_UPDATE, [&] {
     return exa::launch_update(g->sd, exa::kUpdSurface | exa::kUpdUpdate, ua, g->ncells, g->stream);
   });
+}
+
+// One step.  Standard: correct, the new global dt, predict with it.  Fused
+// time stepping (Runner::runOneTimeStepWithFusedAlgorithmicSteps, Runner.cpp:
+// 1053-1110; ADERDGSolver::fusedTimeStep, ADERDGSolver.cpp:1853-1879):
+// correct, predict with the predictor dt chosen one step earlier (no global
+// reduction in between), the stability check, and the prediction again,
+// with the new predictor dt, when the check failed (a launch that exits at
+// once otherwise: no host round trip).
+hipError_t launch_one_step(exa_grid* g) {
+  hipError_t e = launch_correct(g);
   if (e != hipSuccess) return e;
-  e = staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, true, g->stream); });
+  if (!g->fused_ts) {
+    e = staged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->st, true, g->stream); });
+    if (e != hipSuccess) return e;
+    return launch_predict(g, &g->st->dt);
+  }
+  e = launch_predict(g, &g->st->dtp);
   if (e != hipSuccess) return e;
-  return launch_predict(g);
+  e = staged(g, EXA_STAGE_DT,
+             [&] { return exa::launch_dt_finalize_fused(g->st, g->fused_weight, false, g->stream); });
+  if (e != hipSuccess) return e;
+  return launch_predict(g, &g->st->dtp, &g->st->violated);
 }
 
 }  // namespace
@@ -852,7 +878,7 @@ extern "C" int exa_grid_step(exa_grid* g, int nsteps) {
   if (!g || nsteps < 0) return EXA_E_INVALID;
   if (!g->predicted) {
     EXA_TRY(launch_initial_dt(g));
-    EXA_TRY(launch_predict(g));
+    EXA_TRY(launch_predict(g, g->fused_ts ? &g->st->dtp : &g->st->dt));
     g->predicted = true;
   }
   for (int n = 0; n < nsteps; ++n) EXA_TRY(launch_one_step(g));
@@ -922,6 +948,26 @@ extern "C" int exa_grid_set_fused_corrector(exa_grid* g, int on) {
   // boundary values and the per-step adjustment run in the one-kernel corrector
   if (!on && (g->boundary == exa::kBoundaryUser || g->adjust || g->user)) return EXA_E_UNSUPPORTED;
   g->fused_corrector = on != 0;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_set_fused_time_stepping(exa_grid* g, int on, double weight) {
+  if (!g) return EXA_E_INVALID;
+  if (on && !(weight > 0.0 && weight <= 1.0)) return EXA_E_INVALID;
+  if (on && g->sd.linear) return EXA_E_UNSUPPORTED;
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  g->fused_ts = on != 0;
+  if (on) g->fused_weight = weight;
+  g->predicted = false;  // the next step starts with a new dt and prediction
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_predictor_reruns(exa_grid* g, long* reruns) {
+  if (!g || !reruns) return EXA_E_INVALID;
+  exa::StepState st;
+  EXA_TRY(hipMemcpyAsync(&st, g->st, sizeof(st), hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  *reruns = (long)st.reruns;
   return EXA_OK;
 }
 

@@ -333,6 +333,23 @@ int exa_grid_set_predictor_variant(exa_grid* g, int variant);
  * off: a face pass writing both cells' flux blocks, then the update pass.
  * Linear solvers always run the two passes (EXA_E_UNSUPPORTED for on). */
 int exa_grid_set_fused_corrector(exa_grid* g, int on);
+/* Fused time stepping: the optimisation `fuse-algorithmic-steps` with
+ * `fuse-algorithmic-steps-factor` weight in (0, 1] (Parser.cpp:486-505;
+ * Runner::runOneTimeStepWithFusedAlgorithmicSteps, Runner.cpp:1053-1110).
+ * Each step corrects with the corrector dt and predicts with the predictor
+ * dt chosen one step earlier; the new solution's stable dt s then checks it
+ * (TimeStepSizeComputation.cpp:167-204): when it exceeded s, the prediction
+ * is re-run with dt = weight*s, which the next step also corrects with;
+ * otherwise the next predictor dt is (used + weight*s)/2.  The first step
+ * uses the stable dt of the current solution (no weighting: the reference
+ * weighs the first dt only in MPI builds, TimeStepSizeComputation.cpp:
+ * 229-234).  Takes effect at the next exa_grid_step, which starts with a new
+ * dt and prediction.  Nonlinear solvers only (EXA_E_UNSUPPORTED for linear);
+ * exa_grid_time reports the corrector dt, exa_grid_dt_history the corrector
+ * dt of every step. */
+int exa_grid_set_fused_time_stepping(exa_grid* g, int on, double weight);
+/* Predictions re-run after a failed stability check (Runner.cpp:773). */
+int exa_grid_predictor_reruns(exa_grid* g, long* reruns);
 
 /* ---------------------------------------------------------------------------
  * User PDEs on the device.  The reference's user solver implements flux,

@@ -56,6 +56,12 @@ struct StepState {
   double* dt_hist;  // dt of every step (device), may be null
   long dt_hist_len;
   unsigned long long dt_slots[kDtSlots];  // per-slot minima of the next dt
+  // fused time stepping: dt above is the corrector time step size, dtp the
+  // predictor's; violated = the last stability check failed (the prediction
+  // is re-run with the new dtp); reruns counts those
+  double dtp;
+  int violated;
+  long long reruns;
 };
 
 // Arrays marked (bulk) hold double in the kFP64 form and float in kFP32Bulk.
@@ -90,6 +96,9 @@ struct PredictorArgs {
   // Predictor variant (kPredictorAuto, kPredictorValu64, kPredictorValu144);
   // the batch autotuner picks one per order
   int variant;
+  // optional gate for the whole launch: it does nothing unless *run_if != 0
+  // (the fused time stepping's re-run after a violated stability condition)
+  const int* run_if;
 };
 
 struct RiemannArgs {
@@ -237,6 +246,9 @@ enum : int { kUpdSurface = 1, kUpdUpdate = 2, kUpdWriteLduh = 4 };
 hipError_t launch_update(const SolverDesc& sd, int mode, const UpdateArgs& a, long ncells,
                          hipStream_t s);
 hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s);
+// fused time stepping: the stability check, next predictor dt and
+// startNewTimeStep (init: dt = dtp = the stable dt)
+hipError_t launch_dt_finalize_fused(StepState* st, double weight, bool init, hipStream_t s);
 // Per-patch time steps of a batch: dt[p] = min of the patch's slots (then
 // reset); with advance_time, t[p] += the old dt[p] first.
 hipError_t launch_patch_dt_finalize(double* dt, double* t, unsigned long long* slots, int npatches,

@@ -153,6 +153,7 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   const bool active = tid < B3;
   const int node = active ? tid : 0;
   const int z = node / B2, y = (node / B) % B, x = node % B;
+  if (a.run_if && *a.run_if == 0) return;  // whole launch
   if (a.status && a.status[cell] >= a.status_limit) {  // whole workgroup
     if (a.iters && tid == 0) a.iters[cell] = 0;
     return;

@@ -863,12 +863,17 @@ struct ora_grid {
   ora_solver s;
   int nx, ny, nz;
   int transmissive;
-  int user_boundary;  /* boundary faces: the test solver's boundaryValues (reflecting wall) */
-  int adjust;         /* the test solver's adjustPointSolution after every update */
+  ora_boundary_values_fn bv;  /* boundary faces: the user's boundaryValues (NULL: none) */
+  ora_adjust_fn adj;          /* the user's adjustPointSolution after every update (NULL: none) */
   double lo[3];
   long ncells;
   double dx[3];
   double t, dt;
+  /* fused time stepping (fuse-algorithmic-steps): dt is the corrector time
+   * step size, dtp the predictor's; weight is fuse-algorithmic-steps-factor */
+  int fused;
+  double weight, dtp;
+  long reruns;
   int have_prediction;
   double *luh, *lduh, *lQhbnd, *lFhbnd;
   int* iters;
@@ -987,7 +992,7 @@ static void scratch_free(scratch_t* t) {
 }
 
 /* Prediction adapter: predictor + volume integral on cells [c0, c1). */
-static void predict_cells(ora_grid* g, long c0, long c1) {
+static void predict_cells(ora_grid* g, long c0, long c1, double dt) {
   const ora_solver* s = &g->s;
   const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
   const long B2 = B * B, B3 = B2 * B;
@@ -998,14 +1003,14 @@ static void predict_cells(ora_grid* g, long c0, long c1) {
     for (long c = c0; c < c1; c++) {
       if (s->linear) { /* the glue's isLinear branch (AbstractGenericADERDGSolverImplementation.template:53-86) */
         ora_space_time_predictor_linear(s, g->lQhbnd + c * 6 * B2 * D, g->lFhbnd + c * 6 * B2 * V, t.lQi, t.lFi,
-                                        t.gradQ, t.lQhi, t.lFhi, g->luh + c * B3 * D, g->dx, g->dt, NULL);
+                                        t.gradQ, t.lQhi, t.lFhi, g->luh + c * B3 * D, g->dx, dt, NULL);
         ora_volume_integral_linear(s, g->lduh + c * B3 * V, t.lFhi);
         g->iters[c] = 0;
         continue;
       }
       g->iters[c] = ora_space_time_predictor_nonlinear(
           s, g->lQhbnd + c * 6 * B2 * D, g->lFhbnd + c * 6 * B2 * V, t.lQi, t.rhs, t.lFi, t.gradQ,
-          t.lQhi, t.lFhi, g->luh + c * B3 * D, g->dx, g->dt);
+          t.lQhi, t.lFhi, g->luh + c * B3 * D, g->dx, dt);
       ora_volume_integral_nonlinear(s, g->lduh + c * B3 * V, t.lFi, t.lFhi, g->dx);
     }
     scratch_free(&t);
@@ -1042,11 +1047,10 @@ static void boundary_face(ora_grid* g, long c, int face) {
   double FOut[10 * 10 * 16], QOut[10 * 10 * 16];
   memcpy(FOut, FIn, sizeof(double) * B2 * V);
   memcpy(QOut, QIn, sizeof(double) * B2 * D);
-  if (g->user_boundary) {
+  if (g->bv) {
     double centre[3];
     cell_centre(g, c, centre);
-    ora_boundary_conditions(s, ora_bv_reflecting_wall, FOut, QOut, FIn, QIn, centre, g->dx, g->t, g->dt, face,
-                            face / 2);
+    ora_boundary_conditions(s, g->bv, FOut, QOut, FIn, QIn, centre, g->dx, g->t, g->dt, face, face / 2);
   }
   if (s->linear) {
     if (face % 2 == 0)
@@ -1071,19 +1075,31 @@ static void update_cells(ora_grid* g, long c0, long c1) {
     else
       ora_surface_integral_nonlinear(s, g->lduh + c * B3 * V, g->lFhbnd + c * 6 * B2 * V, g->dx);
     ora_solution_update(s, g->luh + c * B3 * D, g->lduh + c * B3 * V, g->dt);
-    if (g->adjust) { /* adjustSolution(t + dt, dt) after the update (ADERDGSolver.cpp:2231-2236) */
+    if (g->adj) { /* adjustSolution(t + dt, dt) after the update (ADERDGSolver.cpp:2231-2236) */
       double centre[3];
       cell_centre(g, c, centre);
-      ora_solution_adjustment(s, ora_adjust_entropy_wave, g->luh + c * B3 * D, centre, g->dx, g->t + g->dt, g->dt);
+      ora_solution_adjustment(s, g->adj, g->luh + c * B3 * D, centre, g->dx, g->t + g->dt, g->dt);
     }
   }
 }
 
-void ora_grid_set_user(ora_grid* g, int user_boundary, int adjust) {
-  g->user_boundary = user_boundary;
-  if (user_boundary) g->transmissive = 1; /* the boundary faces are not wrapped */
-  g->adjust = adjust;
+void ora_grid_set_hooks(ora_grid* g, ora_boundary_values_fn bv, ora_adjust_fn adj) {
+  g->bv = bv;
+  if (bv) g->transmissive = 1; /* the boundary faces are not wrapped */
+  g->adj = adj;
 }
+
+void ora_grid_set_user(ora_grid* g, int user_boundary, int adjust) {
+  ora_grid_set_hooks(g, user_boundary ? ora_bv_reflecting_wall : NULL, adjust ? ora_adjust_entropy_wave : NULL);
+}
+
+void ora_grid_set_fused(ora_grid* g, int on, double weight) {
+  g->fused = on;
+  g->weight = weight;
+  g->have_prediction = 0;
+}
+
+long ora_grid_predictor_reruns(const ora_grid* g) { return g->reruns; }
 
 void ora_grid_adjust(ora_grid* g) {
   const long B = g->s.order + 1, D = g->s.nvar + g->s.npar;
@@ -1091,56 +1107,131 @@ void ora_grid_adjust(ora_grid* g) {
   for (long c = 0; c < g->ncells; c++) {
     double centre[3];
     cell_centre(g, c, centre);
-    ora_solution_adjustment(&g->s, ora_adjust_entropy_wave, g->luh + c * B3 * D, centre, g->dx, g->t, 0.0);
+    ora_solution_adjustment(&g->s, g->adj ? g->adj : ora_adjust_entropy_wave, g->luh + c * B3 * D, centre, g->dx, g->t,
+                            0.0);
   }
   g->have_prediction = 0;
 }
 
-void ora_grid_step(ora_grid* g, int nsteps, int nthreads, double* dt_out, int* iters_out) {
-  set_threads(nthreads);
+/* Merging: each interior face once (periodic wrap included), then the
+ * boundary faces (ADERDGSolver::applyBoundaryConditions, ADERDGSolver.cpp:
+ * 2893-2966): boundaryValues (the inside state and flux when there is no
+ * user hook: transmissive), then the Riemann solve with the outside state
+ * left of even faces. */
+static void merge_faces(ora_grid* g) {
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < g->ncells; c++) {
+    const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
+    if (!g->transmissive || i + 1 < g->nx) riemann_face(g, c, cell_index(g, i + 1, j, k), 0);
+  }
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < g->ncells; c++) {
+    const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
+    if (!g->transmissive || j + 1 < g->ny) riemann_face(g, c, cell_index(g, i, j + 1, k), 1);
+  }
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < g->ncells; c++) {
+    const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
+    if (!g->transmissive || k + 1 < g->nz) riemann_face(g, c, cell_index(g, i, j, k + 1), 2);
+  }
+  if (g->transmissive) {
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < g->ncells; c++) {
+      const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
+      const int at[6] = {i == 0, i == g->nx - 1, j == 0, j == g->ny - 1, k == 0, k == g->nz - 1};
+      for (int f = 0; f < 6; f++)
+        if (at[f]) boundary_face(g, c, f);
+    }
+  }
+}
+
+/* Fused time stepping (fuse-algorithmic-steps, global time stepping, one
+ * rank).  Start (Runner::initialiseMesh, TimeStepSizeComputation with
+ * updateTimeStepSizesFused, TimeStepSizeComputation.cpp:226-237 and
+ * ADERDGSolver.cpp:594-616): corrector and predictor dt = the stable dt of
+ * the initial solution (the weighting at :230-234 is compiled only into MPI
+ * builds), then the first Prediction sweep (Runner.cpp:691-699).  Each sweep
+ * (Runner::runOneTimeStepWithFusedAlgorithmicSteps, Runner.cpp:1053-1110):
+ * Merging with the corrector dt, then per cell fusedTimeStep
+ * (ADERDGSolver.cpp:1853-1879): updateSolution with the corrector dt, the
+ * prediction with the predictor dt, the admissible dt of the new solution;
+ * then SolutionUpdate::endIteration (mappings/SolutionUpdate.cpp:307-318):
+ * reinitialiseTimeStepDataIfLastPredictorTimeStepSizeWasInstable
+ * (TimeStepSizeComputation.cpp:167-204) and startNewTimeStep
+ * (ADERDGSolver.cpp:560-592); a violated stability condition re-runs the
+ * prediction with the new predictor dt (Runner.cpp:1104-1110). */
+static void grid_step_fused(ora_grid* g, int nsteps, int nthreads, double* dt_out, int* iters_out) {
   if (!g->have_prediction) {
-    g->dt = ora_grid_stable_dt(g, nthreads);
-    predict_cells(g, 0, g->ncells);
+    const double s = ora_grid_stable_dt(g, nthreads);
+    g->dt = s;
+    g->dtp = s;
+    predict_cells(g, 0, g->ncells, g->dtp);
     g->have_prediction = 1;
   }
   for (int n = 0; n < nsteps; n++) {
     if (iters_out) memcpy(iters_out + (long)n * g->ncells, g->iters, sizeof(int) * g->ncells);
     if (dt_out) dt_out[n] = g->dt;
-    /* Merging: each interior face once (periodic wrap included). */
-#pragma omp parallel for schedule(static)
-    for (long c = 0; c < g->ncells; c++) {
-      const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
-      if (!g->transmissive || i + 1 < g->nx) riemann_face(g, c, cell_index(g, i + 1, j, k), 0);
+    merge_faces(g);
+    update_cells(g, 0, g->ncells);
+    predict_cells(g, 0, g->ncells, g->dtp);
+    const double stable = ora_grid_stable_dt(g, nthreads); /* min over cells of startNewTimeStep(cell) */
+    double next = stable;                                   /* _minNextPredictorTimeStepSize */
+    double used = g->dtp;                                   /* _minPredictorTimeStepSize */
+    if (used == 0.0) used = stable;
+    const int violated = used > stable;
+    if (violated) {
+      next = fmin(next, g->weight * stable);
+      g->dtp = g->weight * stable;
+    } else {
+      next = fmin(next, 0.5 * (used + g->weight * stable));
     }
-#pragma omp parallel for schedule(static)
-    for (long c = 0; c < g->ncells; c++) {
-      const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
-      if (!g->transmissive || j + 1 < g->ny) riemann_face(g, c, cell_index(g, i, j + 1, k), 1);
+    g->t += g->dt; /* corrector time stamp <- predictor time stamp */
+    g->dt = g->dtp;
+    g->dtp = next;
+    if (violated) {
+      predict_cells(g, 0, g->ncells, g->dtp);
+      g->reruns++;
     }
-#pragma omp parallel for schedule(static)
-    for (long c = 0; c < g->ncells; c++) {
-      const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
-      if (!g->transmissive || k + 1 < g->nz) riemann_face(g, c, cell_index(g, i, j, k + 1), 2);
-    }
-    /* boundary faces (ADERDGSolver::applyBoundaryConditions, ADERDGSolver.cpp:2893-2966):
-     * boundaryValues returns the inside state and flux (transmissive), then
-     * the Riemann solve with the outside state left of even faces */
-    if (g->transmissive) {
-#pragma omp parallel for schedule(static)
-      for (long c = 0; c < g->ncells; c++) {
-        const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
-        const int at[6] = {i == 0, i == g->nx - 1, j == 0, j == g->ny - 1, k == 0, k == g->nz - 1};
-        for (int f = 0; f < 6; f++)
-          if (at[f]) boundary_face(g, c, f);
-      }
-    }
+  }
+}
+
+void ora_grid_step(ora_grid* g, int nsteps, int nthreads, double* dt_out, int* iters_out) {
+  set_threads(nthreads);
+  if (g->fused) {
+    grid_step_fused(g, nsteps, nthreads, dt_out, iters_out);
+    return;
+  }
+  if (!g->have_prediction) {
+    g->dt = ora_grid_stable_dt(g, nthreads);
+    predict_cells(g, 0, g->ncells, g->dt);
+    g->have_prediction = 1;
+  }
+  for (int n = 0; n < nsteps; n++) {
+    if (iters_out) memcpy(iters_out + (long)n * g->ncells, g->iters, sizeof(int) * g->ncells);
+    if (dt_out) dt_out[n] = g->dt;
+    merge_faces(g);
     /* SolutionUpdate, then the new global time step size. */
     update_cells(g, 0, g->ncells);
     g->t += g->dt;
     g->dt = ora_grid_stable_dt(g, nthreads);
     /* Prediction for the next step. */
-    predict_cells(g, 0, g->ncells);
+    predict_cells(g, 0, g->ncells, g->dt);
   }
+}
+
+/* The standard scheme with prescribed time step sizes: step n predicts and
+ * corrects with dts[n] (a test device: the fused scheme must equal it when
+ * given the fused scheme's corrector dt sequence).  Leaves no prediction. */
+void ora_grid_step_dts(ora_grid* g, int nsteps, const double* dts, int nthreads) {
+  set_threads(nthreads);
+  for (int n = 0; n < nsteps; n++) {
+    g->dt = dts[n];
+    predict_cells(g, 0, g->ncells, g->dt);
+    merge_faces(g);
+    update_cells(g, 0, g->ncells);
+    g->t += g->dt;
+  }
+  g->have_prediction = 0;
 }
 
 void ora_grid_sample_step(ora_grid* g, long m, int nthreads) {
@@ -1150,7 +1241,7 @@ void ora_grid_sample_step(ora_grid* g, long m, int nthreads) {
     g->dt = ora_grid_stable_dt(g, nthreads);
     g->have_prediction = 1;
   }
-  predict_cells(g, 0, m);
+  predict_cells(g, 0, m, g->dt);
 #pragma omp parallel for schedule(static)
   for (long c = 0; c < m; c++) {
     riemann_face(g, c, (c + 1) % m, 0);

@@ -151,7 +151,18 @@ void ora_adjust_entropy_wave(const ora_solver* s, const double* x, double t, dou
  * boundary face (non-periodic), adjust = the entropy-wave adjustment after
  * every update at t + dt */
 void ora_grid_set_user(ora_grid* g, int user_boundary, int adjust);
-/* adjustSolution(t, 0) on every cell (the initial condition at t = 0) */
+/* the same with any hooks (NULL: none); a boundary hook makes the boundary
+ * faces non-periodic */
+void ora_grid_set_hooks(ora_grid* g, ora_boundary_values_fn bv, ora_adjust_fn adj);
+/* fused time stepping (fuse-algorithmic-steps on, factor weight in (0, 1]);
+ * the next step starts with a new initial dt and prediction */
+void ora_grid_set_fused(ora_grid* g, int on, double weight);
+/* predictor re-runs after a violated stability condition (fused form) */
+long ora_grid_predictor_reruns(const ora_grid* g);
+/* standard scheme with prescribed dt per step (no prediction left) */
+void ora_grid_step_dts(ora_grid* g, int nsteps, const double* dts, int nthreads);
+/* adjustSolution(t, 0) on every cell with the grid's adjust hook (the
+ * entropy wave without one): the initial condition at t = 0 */
 void ora_grid_adjust(ora_grid* g);
 
 /* ---- limiter path (limiter.c) ---- */

@@ -73,6 +73,11 @@ def lib():
         L.ora_grid_sample_step.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int]
         L.ora_grid_set_user.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.ora_grid_adjust.argtypes = [ctypes.c_void_p]
+        L.ora_grid_set_hooks.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.ora_grid_set_fused.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.ora_grid_predictor_reruns.argtypes = [ctypes.c_void_p]
+        L.ora_grid_predictor_reruns.restype = ctypes.c_long
+        L.ora_grid_step_dts.argtypes = [ctypes.c_void_p, ctypes.c_int, P, ctypes.c_int]
         L.ora_boundary_conditions.argtypes = [S, ctypes.c_void_p, P, P, P, P, P, P, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.ora_solution_adjustment.argtypes = [S, ctypes.c_void_p, P, P, P, ctypes.c_double, ctypes.c_double]
@@ -304,6 +309,24 @@ class Grid:
     def adjust(self):
         """adjustSolution(t, 0) on every cell (the initial condition at t = 0)."""
         lib().ora_grid_adjust(self._h)
+
+    def set_hooks(self, bv=None, adj=None):
+        """Hooks as BV_FN / ADJ_FN callbacks (kept alive by the grid)."""
+        self._hooks = (bv, adj)
+        lib().ora_grid_set_hooks(self._h, ctypes.cast(bv, ctypes.c_void_p) if bv else None,
+                                 ctypes.cast(adj, ctypes.c_void_p) if adj else None)
+
+    def set_fused(self, on=True, weight=0.9):
+        """Fused time stepping (fuse-algorithmic-steps, factor weight)."""
+        lib().ora_grid_set_fused(self._h, int(on), float(weight))
+
+    def predictor_reruns(self):
+        return lib().ora_grid_predictor_reruns(self._h)
+
+    def step_dts(self, dts, nthreads=0):
+        """The standard scheme with a prescribed dt per step."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        lib().ora_grid_step_dts(self._h, len(dts), _p(dts), nthreads)
 
 
 # ---------------------------------------------------------------------------

@@ -44,6 +44,21 @@ def predictor_flops(B, V, D, picard_total, cells):
     return per_iter * picard_total + fixed * cells
 
 
+ELASTIC_NCP_FLOPS = 42  # flops of one elastic non-conservative product B(Q) grad Q (3 directions)
+
+
+def predictor_flops_ncp(B, V, D, picard_total, cells, c_s=ELASTIC_NCP_FLOPS):
+    """Algorithmic FLOPs of the predictor + volume integral for a PDE with
+    only the non-conservative product (the elastic workload): per Picard
+    iteration and cell 6 V B^5 (gradients along 3 directions) + 2 V B^5
+    (iK1) + (c_S + 3 V) B^4 (product, rhs, residual); per cell once
+    2 (V + D) B^4 (time averages) + 12 (D + V) B^3 (face extrapolation) +
+    2 V B^3 (volume integral of the averaged product)."""
+    per_iter = 8 * V * B ** 5 + (c_s + 3 * V) * B ** 4
+    fixed = 2 * (V + D) * B ** 4 + 12 * (D + V) * B ** 3 + 2 * V * B ** 3
+    return per_iter * picard_total + fixed * cells
+
+
 def predictor_executed_flops_fp32(B, V, picard_total, cells):
     """FLOPs the default FP32-bulk predictor executes for a flux-only PDE (the
     algorithmic count above counts every Picard iteration in full): the first
@@ -163,37 +178,121 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4}), flush=True)
 
 
-def cpu_baseline_limiter(n, seconds):
-    """The oracle's limiting driver, single thread, on a small blast grid
-    (rate per DOF-update, an intensive figure)."""
+def cpu_baseline_limiter(n, seconds, threads, gpu_fv_fraction, warmup, steps):
+    """The oracle's limiting driver (OpenMP over cells) on the octant of the
+    n^3 blast that holds the blast centre: the (n+1)/2 cells per direction
+    from the centre cell to the +x,+y,+z walls, at the workload's resolution,
+    so that the FV-active fraction of the sample follows the whole grid's;
+    the GPU run's warmup steps untimed, then its timed steps (fewer when the
+    time budget runs out), so that the sample covers the same stretch of the
+    blast's evolution (rate per DOF-update, an intensive figure)."""
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refutil import blast_cells
-    order, m = 3, 12
+    order = 3
+    i0 = n // 2
+    m = n - i0
     s = O.make_solver(order)
     nodes = np.array([O.table(O.T_NODES, order, i) for i in range(order + 1)])
-    g = O.LimitingGrid(s, m, periodic=False)
-    g.set_solution(blast_cells(order, m, nodes=nodes, radius=0.2),
-                   blast_cells(order, m, subcells=True, radius=0.2))
-    steps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
+    g = O.LimitingGrid(s, m, lo=(i0 / n,) * 3, hi=(1.0,) * 3, periodic=False)
+    g.set_solution(blast_cells(order, n, nodes=nodes, radius=0.2, box=(i0, m)),
+                   blast_cells(order, n, subcells=True, radius=0.2, box=(i0, m)))
+    if warmup:
+        g.step(warmup)
+    st0 = g.status()
+    done, t0 = 0, time.perf_counter()
+    while done < 1 or (done < steps and time.perf_counter() - t0 < seconds):
         g.step(1)
-        steps += 1
+        done += 1
     el = time.perf_counter() - t0
     st = g.status()
-    return dict(value=m ** 3 * (order + 1) ** 3 * steps / el, unit="DOF-updates/s", cores=1, kind="port",
-                sample=f"{steps} steps of the limiting driver (oracle) on a {m}^3 N=3 blast (r=0.2, transmissive), "
-                       f"{(st >= 2).mean():.2f} of the cells FV-active at the end, 1 thread, {el:.1f} s")
+    frac = 0.5 * (float((st0 >= 2).mean()) + float((st >= 2).mean()))
+    return dict(value=m ** 3 * (order + 1) ** 3 * done / el, unit="DOF-updates/s", cores=threads, kind="port",
+                fv_active_fraction=frac,
+                sample=f"steps {warmup + 1}..{warmup + done} of the limiting driver (oracle, OpenMP {threads} threads) "
+                       f"on the {m}^3-cell octant of the {n}^3 N=3 blast holding its centre (transmissive walls), "
+                       f"{el:.1f} s; FV-active fraction {frac:.4f} (mean of before and after) against "
+                       f"{gpu_fv_fraction:.4f} of the GPU grid over the same steps")
+
+
+LIM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "limiter_status_blast_n3_81cube_pmc_traffic.txt")
+
+
+def limiter_measured_traffic(n):
+    """HBM bytes per status-kernel launch on the limiter workload (rocprofv3
+    --pmc, copy-calibrated), as recorded for this grid; None otherwise."""
+    if not os.path.exists(LIM_TRAFFIC_FILE):
+        return None
+    key = f"traffic_bytes_per_launch lim_update_status N=3 {n}^3"
+    for line in open(LIM_TRAFFIC_FILE):
+        if line.startswith(key + " "):
+            return int(line.split()[-1])
+    return None
+
+
+def limiter_rooflines(stages, n, B, V, fv_frac, troubled_frac, picard):
+    """Rooflines of the limiter step's three largest kernels.
+    Status kernel (one launch per step over every cell; DG surface integral
+    and update, stable dt, DMP, status): per DG cell it reads luh, lduh (B^3 V
+    each), lFhbnd (6 B^2 V) and the merged extrema (2 x 6 V), writes luh and
+    luh_prev (B^3 V each), the local extrema (2 V), the cell dt and two face
+    blocks of extrema (4 V); per FV-active cell it reads luh, the extrema and
+    the patch extrema (2 V) and writes the local extrema, the dt and all six
+    face blocks (12 V); about 76 bytes of statuses, counters and neighbour
+    indices per cell either way.
+    FV kernel (one launch per step over the FV-active cells): per cell it
+    reads the patch with its ghost layers (S^3 V, S = 2N+1+4) and writes the
+    new patch interior (L^3 V, L = 2N+1), its projection to DG (B^3 V) and
+    its extrema (2 V).
+    Predictor (one launch per step; FP64 Picard + volume integral on the
+    cells below troubled): the algorithmic FLOPs as for the precision
+    workload's FP64 form."""
+    cells = n ** 3
+    B3, B2 = B ** 3, B * B
+    L = 2 * (B - 1) + 1
+    S = L + 4
+    dg = 8 * (4 * B3 * V + 6 * B2 * V + 12 * V + 2 * V + 1 + 4 * V) + 76
+    fv = 8 * (B3 * V + 12 * V + 2 * V + 2 * V + 1 + 12 * V) + 76
+    status_bytes = cells * ((1 - fv_frac) * dg + fv_frac * fv)
+    fv_bytes = cells * fv_frac * 8 * (S ** 3 * V + L ** 3 * V + B3 * V + 2 * V)
+    out = []
+    ms, nl = stages["update"]
+    avg = ms / max(nl, 1)
+    a = status_bytes / (avg * 1e-3) / 1e9
+    out.append(dict(bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(a / HBM_PEAK_GBS, 4), traffic=limiter_measured_traffic(n),
+                    kernel="lim_update_status_kernel<3> (DG update, stable dt, DMP, status)",
+                    avg_launch_ms=round(avg, 4), algorithmic_bytes_per_launch=int(status_bytes)))
+    ms, nl = stages["predictor"]
+    avg = ms / max(nl, 1)
+    it, pc = picard
+    run_cells = pc * (1 - troubled_frac)
+    flops = predictor_flops(B, V, V, it, run_cells) / max(nl, 1)
+    a = flops / (avg * 1e-3) / 1e12
+    out.append(dict(bound="mfma", achieved=round(a, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(a / FP64_PEAK_TFLOPS, 4), traffic=None, compute_unit="VALU",
+                    kernel="predictor_kernel<4,Euler,flux,double> (Picard + volume integral)",
+                    avg_launch_ms=round(avg, 4), flops_per_launch=flops,
+                    picard_mean=round(it / max(run_cells, 1), 4)))
+    ms, nl = stages["fv"]
+    avg = ms / max(nl, 1)
+    a = fv_bytes / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+    out.append(dict(bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(a / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="lim_fv_update_kernel<3> (MUSCL-Hancock patch update + projection)",
+                    avg_launch_ms=round(avg, 4), algorithmic_bytes_per_launch=int(fv_bytes)))
+    return out
 
 
 def bench_limiter(capi, args):
     """Limiter workload: Euler blast, N=3, 81^3, transmissive, FP64."""
     order, n = 3, args.cells
-    B = order + 1
+    B, V = order + 1, 5
     g = capi.LimitingGrid(capi.euler_solver(order), n)
     g.init_blast(seed=args.seed, radius=0.2)
     g.step(args.warmup)
     g.synchronize()
+    st0, _ = g.status()
     g.set_profiling(True)
     g.stage_times()
     t0 = time.perf_counter()
@@ -201,11 +300,17 @@ def bench_limiter(capi, args):
     g.synchronize()
     wall = time.perf_counter() - t0
     stages = g.stage_times()
+    picard = g.picard_totals()
     st, irr = g.status()
     t_now, dt_now, nsteps = g.time()
     if not np.isfinite(dt_now) or dt_now <= 0:
         raise RuntimeError("non-finite time step")
     dof = n ** 3 * B ** 3
+    # statuses over the timed steps: the mean of the fractions before and after
+    fv_frac = 0.5 * (float((st0 >= 2).mean()) + float((st >= 2).mean()))
+    tr_frac = 0.5 * (float((st0 == 3).mean()) + float((st == 3).mean()))
+    roofs = limiter_rooflines(stages, n, B, V, fv_frac, tr_frac, picard)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     out = {
         "metric": "DOF-updates/s, 3D Euler blast N=3 81^3 with a-posteriori subcell limiter (FP64)",
         "value": dof * args.steps / wall, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -217,9 +322,11 @@ def bench_limiter(capi, args):
                    "dof_per_step": dof, "seed": args.seed},
         "stage_ms_per_step": {k: round(v[0] / max(args.steps, 1), 4) for k, v in stages.items()},
         "limited": {"fv_active_fraction": float((st >= 2).mean()), "troubled_fraction": float((st == 3).mean()),
-                    "irregular_steps_total": int(irr)},
-        "roofline": None,
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_limiter(n, args.cpu_seconds),
+                    "fv_active_fraction_mean_timed": fv_frac, "irregular_steps_total": int(irr)},
+        "roofline": roofs[0],
+        "rooflines_other_kernels": roofs[1:],
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_limiter(n, args.cpu_seconds, threads,
+                                                                               fv_frac, args.warmup, args.steps),
     }
     print(json.dumps(out), flush=True)
     g.close()
@@ -609,6 +716,23 @@ def linear_roofline(stages, n, B):
                 note="latency-bound on LDS (neither HBM nor ALU saturated)")
 
 
+def elastic_roofline(r, B, peak, valu_peak, kernel):
+    """Compute roofline of the elastic predictor (the nonlinear form's
+    dominant kernel), FLOPs from predictor_flops_ncp and the counted Picard
+    iterations; launch time from the grid's HIP events."""
+    ms, nl = r["stages"]["predictor"]
+    avg = ms / max(nl, 1)
+    flops = predictor_flops_ncp(B, 9, 12, r["picard_total"], r["pred_cells"]) / max(nl, 1)
+    a = flops / (avg * 1e-3) / 1e12
+    out = dict(bound="mfma", achieved=round(a, 3), peak=peak, unit="TFLOP/s", frac=round(a / peak, 4),
+               traffic=None, kernel=kernel, compute_unit="VALU", avg_launch_ms=round(avg, 4),
+               flops_per_launch=flops, picard_mean=round(r["picard_total"] / max(r["pred_cells"], 1), 4))
+    if valu_peak:
+        out["valu_peak"] = valu_peak
+        out["valu_frac"] = round(a / valu_peak, 4)
+    return out
+
+
 def bench_elastic(capi, args):
     """Elastic workload: velocity-stress NCP form, N=4, 81^3, Gaussian pulse
     of width 0.05 in a seeded two-layer material; FP32-bulk vs FP64, and the
@@ -634,7 +758,9 @@ def bench_elastic(capi, args):
         t_now, dt_now, _ = g.time()
         if not np.isfinite(dt_now) or dt_now <= 0:
             raise RuntimeError(f"{form}: non-finite time step")
-        res[form] = dict(wall=wall, stages=stages, t=t_now, sol=g.solution().reshape(-1, 12)[:, :9])
+        it, pc = g.picard_totals() if form != "linear" else (0, 0)
+        res[form] = dict(wall=wall, stages=stages, t=t_now, sol=g.solution().reshape(-1, 12)[:, :9],
+                         picard_total=it, pred_cells=pc)
         print(f"elastic {form}: {wall / max(args.steps, 1) * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
         g.close()
     dof = n ** 3 * B ** 3
@@ -671,7 +797,10 @@ def bench_elastic(capi, args):
                         "l2_rel_diff_vs_nonlinear_fp64": float(np.linalg.norm(res["linear"]["sol"] - b)
                                                                / np.linalg.norm(b)),
                         "roofline": linear_roofline(res["linear"]["stages"], n, B)},
-        "roofline": None,
+        "roofline": elastic_roofline(m, B, FP32_PEAK_TFLOPS, FP32_VALU_PEAK_TFLOPS,
+                                     "predictor_kernel<5,Elastic,ncp,float> (Picard + volume integral)"),
+        "fp64_roofline": elastic_roofline(m64, B, FP64_PEAK_TFLOPS, None,
+                                          "predictor_kernel<5,Elastic,ncp,double> (Picard + volume integral)"),
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_elastic(args.cpu_seconds),
     }
     print(json.dumps(out), flush=True)
@@ -717,8 +846,7 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from exahype_amd import capi

@@ -43,6 +43,7 @@ EXPORTED = (
     "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
+    "exa_lgrid_picard_totals",
     "exa_grid_set_predictor_variant", "exa_grid_set_fused_corrector", "exa_grid_create_user",
     "exa_grid_set_fused_time_stepping", "exa_grid_predictor_reruns",
     "exa_grid_set_adjust_solution", "exa_grid_adjust_solution", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
@@ -210,6 +211,7 @@ def lib():
     L.exa_lgrid_min_max.argtypes = [V, _D, _D]
     L.exa_lgrid_set_profiling.argtypes = [V, i]
     L.exa_lgrid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
+    L.exa_lgrid_picard_totals.argtypes = [V, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
     L.exa_patchset_create.argtypes = [ctypes.POINTER(PatchSpec), i, i, d_, ctypes.c_long, ctypes.POINTER(V)]
     L.exa_patchset_run.argtypes = [V, i, i, i, i]
     L.exa_patchset_stats.argtypes = [V, _D, _D, _I]
@@ -847,11 +849,18 @@ class LimitingGrid:
         check(lib().exa_lgrid_set_profiling(self._h, int(on)), "set_profiling")
 
     def stage_times(self):
-        ms = np.zeros(5)
-        n = (ctypes.c_long * 5)()
+        """ms and launches per stage: predictor, riemann, update (the status
+        kernel: DG update, DMP, status), dt, limiter, fv (FV patch updates)."""
+        names = ("predictor", "riemann", "update", "dt", "limiter", "fv")
+        ms = np.zeros(len(names))
+        n = (ctypes.c_long * len(names))()
         check(lib().exa_lgrid_stage_times(self._h, _dp(ms), n), "stage_times")
-        names = ("predictor", "riemann", "update", "dt", "limiter")
-        return {names[i]: (float(ms[i]), int(n[i])) for i in range(5)}
+        return {names[i]: (float(ms[i]), int(n[i])) for i in range(len(names))}
+
+    def picard_totals(self):
+        it, nc = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(lib().exa_lgrid_picard_totals(self._h, ctypes.byref(it), ctypes.byref(nc)), "picard totals")
+        return it.value, nc.value
 
 
 class PatchSet:

@@ -91,8 +91,8 @@ struct exa_lgrid {
   long ws_cells = 0;
   bool profiling = false;
   std::vector<LProf> prof;
-  double prof_ms[EXA_NUM_LSTAGES] = {0, 0, 0, 0, 0};
-  long prof_n[EXA_NUM_LSTAGES] = {0, 0, 0, 0, 0};
+  double prof_ms[EXA_NUM_LSTAGES] = {0, 0, 0, 0, 0, 0};
+  long prof_n[EXA_NUM_LSTAGES] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -175,7 +175,10 @@ hipError_t lpredict(exa_lgrid* g) {
   a.ws_cells = g->ws_cells;
   a.status = g->st;  // Prediction.cpp:177: cells with status < troubled
   a.status_limit = 2 * g->h + 1;
-  return lstaged(g, EXA_STAGE_PREDICTOR, [&] { return exa::launch_predictor(g->sd, false, a, g->ncells, g->stream); });
+  hipError_t e =
+      lstaged(g, EXA_STAGE_PREDICTOR, [&] { return exa::launch_predictor(g->sd, false, a, g->ncells, g->stream); });
+  if (e == hipSuccess && g->profiling) e = exa::launch_count_iterations(g->iters, g->ncells, g->state, g->stream);
+  return e;
 }
 
 // First time step size and prediction of an initial state.
@@ -219,12 +222,8 @@ hipError_t lstep(exa_lgrid* g) {
       if (r == hipSuccess) r = exa::launch_lim_ghosts(sd, a, false, s);
       return r;
     });
-  if (e == hipSuccess)
-    e = lstaged(g, EXA_STAGE_UPDATE, [&] {
-      hipError_t r = exa::launch_lim_fv_update(sd, a, false, s);
-      if (r == hipSuccess) r = exa::launch_lim_update_status(sd, a, s);
-      return r;
-    });
+  if (e == hipSuccess) e = lstaged(g, EXA_STAGE_FV, [&] { return exa::launch_lim_fv_update(sd, a, false, s); });
+  if (e == hipSuccess) e = lstaged(g, EXA_STAGE_UPDATE, [&] { return exa::launch_lim_update_status(sd, a, s); });
   if (e == hipSuccess)
     e = lstaged(g, EXA_STAGE_LIMITER, [&] {
       hipError_t r = exa::launch_lim_spread_rollback(sd, a, s);
@@ -469,7 +468,22 @@ extern "C" int exa_lgrid_min_max(exa_lgrid* g, double* mn, double* mx) {
 
 extern "C" int exa_lgrid_set_profiling(exa_lgrid* g, int on) {
   if (!g) return EXA_E_INVALID;
+  EXA_TRY(hipStreamSynchronize(g->stream));
   g->profiling = on != 0;
+  if (on) {  // Picard totals count from here
+    const unsigned long long zero[2] = {0, 0};
+    EXA_TRY(hipMemcpy(&g->state->picard_total, zero, sizeof(zero), hipMemcpyHostToDevice));
+  }
+  return EXA_OK;
+}
+
+extern "C" int exa_lgrid_picard_totals(exa_lgrid* g, unsigned long long* iterations, unsigned long long* cells) {
+  if (!g) return EXA_E_INVALID;
+  exa::StepState st;
+  EXA_TRY(hipMemcpyAsync(&st, g->state, sizeof(st), hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  if (iterations) *iterations = st.picard_total;
+  if (cells) *cells = st.predictor_cells;
   return EXA_OK;
 }
 

@@ -529,12 +529,18 @@ int exa_lgrid_status(exa_lgrid* g, int* status, long* irregular_steps);
 int exa_lgrid_limiter_solution(exa_lgrid* g, long cell, double* lim);
 /* Solution min/max face blocks [cells][6][V]. */
 int exa_lgrid_min_max(exa_lgrid* g, double* mn, double* mx);
-/* Stage timing: EXA_STAGE_PREDICTOR, _RIEMANN, _UPDATE (DG update, FV update,
- * DMP and status), _DT, and EXA_STAGE_LIMITER (merging of min/max and ghost
- * layers, spreading, rollback, recomputation). */
-enum { EXA_STAGE_LIMITER = 4, EXA_NUM_LSTAGES = 5 };
+/* Stage timing: EXA_STAGE_PREDICTOR, _RIEMANN, _UPDATE (the status kernel:
+ * DG surface integral and update, DMP, admissibility and status), _DT,
+ * EXA_STAGE_LIMITER (merging of min/max and ghost layers, spreading,
+ * rollback, recomputation) and EXA_STAGE_FV (the FV update of the
+ * FV-active cells' patches). */
+enum { EXA_STAGE_LIMITER = 4, EXA_STAGE_FV = 5, EXA_NUM_LSTAGES = 6 };
 int exa_lgrid_set_profiling(exa_lgrid* g, int on);
 int exa_lgrid_stage_times(exa_lgrid* g, double* ms, long* launches);
+/* Picard iterations summed over the predictor launches since profiling was
+ * switched on, and the cells those launches covered (FV-active cells, which
+ * the predictor skips, count 0 iterations). */
+int exa_lgrid_picard_totals(exa_lgrid* g, unsigned long long* iterations, unsigned long long* cells);
 
 /* ---------------------------------------------------------------------------
  * Patch set: many independent periodic Euler grids ("patches") of uneven size

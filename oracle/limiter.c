@@ -816,7 +816,9 @@ static void lg_merge_min_max(ora_lgrid* g, long cl, long cr, int d) {
 
 static void lg_merging(ora_lgrid* g, int recomputation) {
   const int A = g->A, D = g->s.nvar + g->s.npar;
+  /* within one direction the faces write disjoint face slots and ghost slabs */
   for (int d = 0; d < 3; d++)
+#pragma omp parallel for schedule(static)
     for (long c = 0; c < g->ncells; c++) {
       const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
       const long r = lg_cell(g, i + (d == 0), j + (d == 1), k + (d == 2));
@@ -837,6 +839,7 @@ static void lg_merging(ora_lgrid* g, int recomputation) {
     }
   if (g->periodic) return;
   /* boundary faces */
+#pragma omp parallel for schedule(static)
   for (long c = 0; c < g->ncells; c++) {
     const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
     const int at[6] = {i == 0, i == g->nx - 1, j == 0, j == g->ny - 1, k == 0, k == g->nz - 1};
@@ -857,38 +860,44 @@ static void lg_predict(ora_lgrid* g) {
   const ora_solver* s = &g->s;
   const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
   const long B3 = B * B * B, B4 = B3 * B;
-  double* lQi = (double*)malloc(sizeof(double) * B4 * D);
-  double* rhs = (double*)malloc(sizeof(double) * B4 * D);
-  double* lFi = (double*)malloc(sizeof(double) * B4 * 4 * V);
-  double* gradQ = (double*)malloc(sizeof(double) * B4 * 3 * V);
-  double* lQhi = (double*)malloc(sizeof(double) * B3 * D);
-  double* lFhi = (double*)malloc(sizeof(double) * B3 * 4 * V);
-  for (long c = 0; c < g->ncells; c++) {
-    if (g->st[c] >= g->T) {
-      g->iters[c] = 0;
-      continue;
+#pragma omp parallel
+  {
+    double* lQi = (double*)malloc(sizeof(double) * B4 * D);
+    double* rhs = (double*)malloc(sizeof(double) * B4 * D);
+    double* lFi = (double*)malloc(sizeof(double) * B4 * 4 * V);
+    double* gradQ = (double*)malloc(sizeof(double) * B4 * 3 * V);
+    double* lQhi = (double*)malloc(sizeof(double) * B3 * D);
+    double* lFhi = (double*)malloc(sizeof(double) * B3 * 4 * V);
+#pragma omp for schedule(dynamic, 16)
+    for (long c = 0; c < g->ncells; c++) {
+      if (g->st[c] >= g->T) {
+        g->iters[c] = 0;
+        continue;
+      }
+      g->iters[c] = ora_space_time_predictor_nonlinear(s, g->lQhbnd + c * g->n_qb, g->lFhbnd + c * g->n_fb, lQi, rhs,
+                                                       lFi, gradQ, lQhi, lFhi, LG_LUH(g, c), g->dx, g->dt);
+      ora_volume_integral_nonlinear(s, g->lduh + c * g->n_du, lFi, lFhi, g->dx);
     }
-    g->iters[c] = ora_space_time_predictor_nonlinear(s, g->lQhbnd + c * g->n_qb, g->lFhbnd + c * g->n_fb, lQi, rhs,
-                                                     lFi, gradQ, lQhi, lFhi, LG_LUH(g, c), g->dx, g->dt);
-    ora_volume_integral_nonlinear(s, g->lduh + c * g->n_du, lFi, lFhi, g->dx);
+    free(lQi);
+    free(rhs);
+    free(lFi);
+    free(gradQ);
+    free(lQhi);
+    free(lFhi);
   }
-  free(lQi);
-  free(rhs);
-  free(lFi);
-  free(gradQ);
-  free(lQhi);
-  free(lFhi);
 }
 
 static void lg_spread(ora_lgrid* g) {
   for (int sweep = 0; sweep < g->T; sweep++) {
     for (int d = 0; d < 3; d++)
+#pragma omp parallel for schedule(static)
       for (long c = 0; c < g->ncells; c++) {
         const int i = (int)(c % g->nx), j = (int)((c / g->nx) % g->ny), k = (int)(c / ((long)g->nx * g->ny));
         const long r = lg_cell(g, i + (d == 0), j + (d == 1), k + (d == 2));
         if (r >= 0) lg_merge_status(g, c, r, d);
       }
     /* updateLimiterStatusDuringLimiterStatusSpreading, LS:392-405 */
+#pragma omp parallel for schedule(static)
     for (long c = 0; c < g->ncells; c++) {
       if (g->st[c] < g->T) {
         int m = 0;
@@ -918,6 +927,7 @@ void ora_lgrid_set_solution(ora_lgrid* g, const double* luh, const double* lim_i
   memset(g->fs, 0, sizeof(int) * g->ncells * 6);
   memset(g->has_patch, 0, sizeof(int) * g->ncells);
   int any = 0;
+#pragma omp parallel for schedule(static) reduction(| : any)
   for (long c = 0; c < g->ncells; c++) {
     lg_solver_min_max(g, c);
     g->itc[c] = 0;
@@ -930,6 +940,7 @@ void ora_lgrid_set_solution(ora_lgrid* g, const double* luh, const double* lim_i
   if (any) {
     lg_spread(g);
     const int L = g->L, G = g->g, S = g->S, D = g->s.nvar + g->s.npar;
+#pragma omp parallel for schedule(static)
     for (long c = 0; c < g->ncells; c++)
       if (g->st[c] > 0) {
         lg_ensure_patch(g, c);
@@ -947,6 +958,7 @@ void ora_lgrid_set_solution(ora_lgrid* g, const double* luh, const double* lim_i
   memcpy(g->pst, g->st, sizeof(int) * g->ncells);
   g->t = 0.0;
   double dt = DBL_MAX;
+#pragma omp parallel for schedule(static) reduction(min : dt)
   for (long c = 0; c < g->ncells; c++) dt = std_min(dt, lg_dg_dt(g, c));
   g->dt = dt;
   lg_predict(g);
@@ -960,6 +972,8 @@ static int lg_step(ora_lgrid* g) {
   lg_merging(g, 0);
   /* 2. solution update */
   int irregular = 0;
+  /* per cell: reads its own face statuses, writes only its own data */
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : irregular)
   for (long c = 0; c < g->ncells; c++) {
     g->pst[c] = g->st[c];
     const int s = g->st[c];
@@ -1004,6 +1018,7 @@ static int lg_step(ora_lgrid* g) {
   /* 3. irregular change: spreading, rollback, local recomputation */
   if (irregular) {
     lg_spread(g);
+#pragma omp parallel for schedule(static)
     for (long c = 0; c < g->ncells; c++) { /* reinitialiseSolversLocally */
       lg_ensure_patch(g, c);
       if (g->st[c] > 0) {
@@ -1019,6 +1034,7 @@ static int lg_step(ora_lgrid* g) {
       lg_release_patch(g, c);
     }
     lg_merging(g, 1);
+#pragma omp parallel for schedule(dynamic, 16)
     for (long c = 0; c < g->ncells; c++) { /* recomputeSolutionLocally */
       const int s = g->st[c];
       if (s > 0) {
@@ -1043,6 +1059,7 @@ static int lg_step(ora_lgrid* g) {
   /* 4. new time step size and prediction */
   g->t += g->dt;
   double dt = DBL_MAX;
+#pragma omp parallel for schedule(static) reduction(min : dt)
   for (long c = 0; c < g->ncells; c++) dt = std_min(dt, g->dtc[c]);
   g->dt = dt;
   lg_predict(g);

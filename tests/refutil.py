@@ -145,22 +145,34 @@ def splitmix64_uniform(counter, seed):
     return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def blast_cells(order, n, seed=7, nodes=None, radius=0.2, p_in=10.0, p_out=0.1, noise=0.01, subcells=False):
+def blast_cells(order, n, seed=7, nodes=None, radius=0.2, p_in=10.0, p_out=0.1, noise=0.01, subcells=False,
+                box=None):
     """Spherical blast on the unit cube: rho = 1 + noise*(2u-1) with u from
     splitmix64 keyed by the global node index (layout independent), u = 0,
     p = p_in inside |x - 1/2| < radius, p_out outside.  [ncells*B^3*5].
     subcells=True evaluates it at the 2N+1 FV subcell centres per direction
-    instead (noise keyed by the global subcell index, stream seed + 1)."""
+    instead (noise keyed by the global subcell index, stream seed + 1).
+    box=(i0, m): only the m^3 cells i0..i0+m-1 (per direction) of the n^3
+    grid, with the full grid's coordinates and noise."""
     if subcells:
         L = 2 * order + 1
         nodes = (2.0 * np.arange(L) + 1.0) / (2.0 * L)
         order, seed = L - 1, seed + 1
     B = order + 1
     n3 = (n, n, n) if isinstance(n, int) else n
-    X = node_coords(order, n3, nodes=nodes)  # [c, z, y, x, 3]
+    if box is None:
+        X = node_coords(order, n3, nodes=nodes)  # [c, z, y, x, 3]
+        off, m3 = (0, 0, 0), n3
+    else:
+        i0, m = box
+        off, m3 = (i0, i0, i0), (m, m, m)
+        X = node_coords(order, m3, lo=tuple(i0 / n3[d] for d in range(3)),
+                        hi=tuple((i0 + m) / n3[d] for d in range(3)), nodes=nodes)
     nc = X.shape[0]
     c = np.arange(nc)
-    ci, cj, ck = c % n3[0], (c // n3[0]) % n3[1], c // (n3[0] * n3[1])
+    ci = c % m3[0] + off[0]
+    cj = (c // m3[0]) % m3[1] + off[1]
+    ck = c // (m3[0] * m3[1]) + off[2]
     z, y, x = np.meshgrid(np.arange(B), np.arange(B), np.arange(B), indexing="ij")
     gx = ci[:, None, None, None] * B + x[None]
     gy = cj[:, None, None, None] * B + y[None]

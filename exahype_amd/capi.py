@@ -47,7 +47,8 @@ EXPORTED = (
     "exa_grid_set_predictor_variant", "exa_grid_set_fused_corrector", "exa_grid_create_user",
     "exa_grid_set_fused_time_stepping", "exa_grid_predictor_reruns",
     "exa_grid_set_adjust_solution", "exa_grid_adjust_solution", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
-    "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy",
+    "exa_patchset_solution", "exa_patchset_time", "exa_patchset_destroy", "exa_patchset_group_variant",
+    "exa_patchset_set_fused_corrector", "exa_patchset_set_profiling", "exa_patchset_group_times",
     "exa_godunov_solution_update", "exa_fvgrid_create", "exa_fvgrid_destroy", "exa_fvgrid_num_cells",
     "exa_fvgrid_upload", "exa_fvgrid_download", "exa_fvgrid_init_euler", "exa_fvgrid_step",
     "exa_fvgrid_synchronize", "exa_fvgrid_time", "exa_fvgrid_set_profiling", "exa_fvgrid_stage_times",
@@ -99,8 +100,24 @@ class EulerIC(ctypes.Structure):
 
 
 class PatchSpec(ctypes.Structure):
-    """exa_patch_spec: one periodic unit-cube Euler patch with an entropy wave."""
-    _fields_ = [("n", ctypes.c_int), ("order", ctypes.c_int), ("k", ctypes.c_double * 3), ("amp", ctypes.c_double)]
+    """exa_patch_spec: one periodic unit-cube patch: Euler with an entropy
+    wave, or elasticity with a seeded material and a velocity pulse."""
+    _fields_ = [("n", ctypes.c_int), ("order", ctypes.c_int), ("k", ctypes.c_double * 3), ("amp", ctypes.c_double),
+                ("pde", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("width", ctypes.c_double)]
+
+
+def elastic_patch(n, order, seed, width=0.05, amp=1.0):
+    """An elastic patch spec for PatchSet (Euler patches are (n, order, k, amp))."""
+    return {"pde": PDE_ELASTIC, "n": n, "order": order, "seed": seed, "width": width, "amp": amp}
+
+
+def _patch_fields(spec):
+    """(pde, n, order, k, amp, seed, width) of an Euler tuple or an elastic dict."""
+    if isinstance(spec, dict):
+        return (spec["pde"], spec["n"], spec["order"], (0.0, 0.0, 0.0), spec.get("amp", 1.0), spec.get("seed", 0),
+                spec.get("width", 0.05))
+    n, order, k, amp = spec
+    return (PDE_EULER, n, order, k, amp, 0, 0.0)
 
 
 def euler_solver(order, npar=0, terms=TERM_FLUX, cfl=0.9, precision=PRECISION_FP64, linear=False):
@@ -216,6 +233,11 @@ def lib():
     L.exa_patchset_run.argtypes = [V, i, i, i, i]
     L.exa_patchset_stats.argtypes = [V, _D, _D, _I]
     L.exa_patchset_predictor_variant.argtypes = [V, i]
+    L.exa_patchset_group_variant.argtypes = [V, i, i]
+    L.exa_patchset_set_fused_corrector.argtypes = [V, i]
+    L.exa_patchset_set_profiling.argtypes = [V, i]
+    L.exa_patchset_group_times.argtypes = [V, i, i, _D, _D, ctypes.POINTER(ctypes.c_long),
+                                           ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
     L.exa_patchset_solution.argtypes = [V, i, _D]
     L.exa_patchset_time.argtypes = [V, i, _D, _D]
     L.exa_patchset_destroy.argtypes = [V]
@@ -869,8 +891,10 @@ class PatchSet:
 
     def __init__(self, specs, precision=PRECISION_FP64, hbm_budget_gb=200.0, keep_max_cells=0):
         arr = (PatchSpec * len(specs))()
-        for j, (n, order, k, amp) in enumerate(specs):
+        for j, spec in enumerate(specs):
+            pde, n, order, k, amp, seed, width = _patch_fields(spec)
             arr[j].n, arr[j].order, arr[j].amp = int(n), int(order), float(amp)
+            arr[j].pde, arr[j].seed, arr[j].width = int(pde), int(seed), float(width)
             for d in range(3):
                 arr[j].k[d] = float(k[d])
         self.specs = list(specs)
@@ -898,12 +922,28 @@ class PatchSet:
         check(lib().exa_patchset_stats(self._h, ctypes.byref(sec), ctypes.byref(dof), ctypes.byref(nw)), "stats")
         return sec.value, dof.value, nw.value
 
-    def predictor_variant(self, order):
-        return lib().exa_patchset_predictor_variant(self._h, order)
+    def predictor_variant(self, order, pde=PDE_EULER):
+        return lib().exa_patchset_group_variant(self._h, pde, order)
+
+    def set_fused_corrector(self, on):
+        check(lib().exa_patchset_set_fused_corrector(self._h, int(on)), "fused corrector")
+
+    def set_profiling(self, on):
+        check(lib().exa_patchset_set_profiling(self._h, int(on)), "profiling")
+
+    def group_times(self, pde, order):
+        """(predictor ms, corrector ms, predictor launches, Picard iterations,
+        predicted cells) of one group over the last profiled run."""
+        p, c, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
+        it, cells = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(lib().exa_patchset_group_times(self._h, pde, order, ctypes.byref(p), ctypes.byref(c), ctypes.byref(n),
+                                             ctypes.byref(it), ctypes.byref(cells)), "group times")
+        return p.value, c.value, n.value, it.value, cells.value
 
     def solution(self, j):
-        n, order = self.specs[j][0], self.specs[j][1]
-        out = np.zeros(n ** 3 * (order + 1) ** 3 * 5)
+        pde, n, order = _patch_fields(self.specs[j])[:3]
+        D = 12 if pde == PDE_ELASTIC else 5
+        out = np.zeros(n ** 3 * (order + 1) ** 3 * D)
         check(lib().exa_patchset_solution(self._h, j, _dp(out)), "solution")
         return out
 

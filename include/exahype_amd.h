@@ -543,24 +543,31 @@ int exa_lgrid_stage_times(exa_lgrid* g, double* ms, long* launches);
 int exa_lgrid_picard_totals(exa_lgrid* g, unsigned long long* iterations, unsigned long long* cells);
 
 /* ---------------------------------------------------------------------------
- * Patch set: many independent periodic Euler grids ("patches") of uneven size
- * and order advanced together, each with its own time step size.  Every patch
- * is the unit cube with n^3 cells and the entropy wave rho = 1 + amp
- * sin(2 pi k.x), u = v = w = 1, p = 1.  Each of them follows the non-fused
- * global time step of exa_grid (ADERDGSolver::updateSolution,
- * performPredictionAndVolumeIntegral, solveRiemannProblemAtInterface,
- * ADERDGSolver.cpp:1881-2290; Runner.cpp:1116-1162) with its own dt.  Patches
- * of one order share every launch (cell -> patch map); the patches run in
- * waves whose state fits hbm_budget_gb; each wave's time step is captured
- * once in a hipGraph and replayed.  The reference has no batching of
+ * Patch set: many independent periodic grids ("patches") of uneven size, order
+ * and PDE advanced together, each with its own time step size.  Every patch
+ * is the unit cube with n^3 cells: an Euler patch starts from the entropy wave
+ * rho = 1 + amp sin(2 pi k.x), u = v = w = 1, p = 1; an elastic patch from the
+ * seeded material and velocity pulse of exa_grid_init_elastic.  Each of them
+ * follows the non-fused global time step of exa_grid
+ * (ADERDGSolver::updateSolution, performPredictionAndVolumeIntegral,
+ * solveRiemannProblemAtInterface, ADERDGSolver.cpp:1881-2290; Runner.cpp:
+ * 1116-1162) with its own dt.  Patches of one PDE and order share every
+ * launch (cell -> patch map); the patches run in waves whose state fits
+ * hbm_budget_gb; each wave's time step is captured once in a hipGraph and
+ * replayed.  The reference has no batching of
  * independent grids: this scheduler replaces running one ExaHyPE process per
  * patch.
  * ------------------------------------------------------------------------- */
 typedef struct exa_patch_spec {
   int n;       /* cells per side */
-  int order;   /* polynomial order 2..7 */
-  double k[3]; /* wave vector of the entropy wave */
-  double amp;  /* density amplitude of the entropy wave */
+  int order;   /* polynomial order: Euler 2..7, elastic 2..5 */
+  double k[3]; /* Euler: wave vector of the entropy wave */
+  double amp;  /* Euler: density amplitude of the entropy wave; elastic: pulse amplitude */
+  int pde;     /* EXA_PDE_EULER (0) or EXA_PDE_ELASTIC: velocity-stress elasticity,
+                  seeded two-layer material and a Gaussian velocity pulse
+                  (as exa_grid_init_elastic) */
+  unsigned long long seed; /* elastic: material seed */
+  double width;            /* elastic: pulse width */
 } exa_patch_spec;
 typedef struct exa_patchset exa_patchset;
 /* precision: EXA_PRECISION_*; the solution of every patch with at most
@@ -577,9 +584,26 @@ int exa_patchset_run(exa_patchset* b, int warmup_steps, int nsteps, int use_grap
 /* Device seconds of the timed steps (HIP events around each wave's timed steps),
  * degrees of freedom updated in them, and the number of waves. */
 int exa_patchset_stats(exa_patchset* b, double* seconds, double* dof_updates, int* nwaves);
-/* Predictor variant the autotuner chose for an order (EXA_PREDICTOR_*; -1 none). */
+/* Predictor variant the autotuner chose for the Euler patches of an order
+ * (EXA_PREDICTOR_*; -1 none); exa_patchset_group_variant: for the patches of
+ * one PDE (EXA_PDE_*) and order. */
 int exa_patchset_predictor_variant(exa_patchset* b, int order);
-/* Solution [n^3][B^3][5] of one patch after the run (kept patches only). */
+int exa_patchset_group_variant(exa_patchset* b, int pde, int order);
+/* Corrector of the patch set's steps: on (the default), the one-kernel
+ * corrector (each cell solves its six faces' Riemann problems inside its
+ * patch, then surface integral, update and dt); off, the face pass and the
+ * update pass.  Applies to later exa_patchset_run calls. */
+int exa_patchset_set_fused_corrector(exa_patchset* b, int on);
+/* Profiling of later exa_patchset_run calls: the timed steps run eagerly
+ * (no graph) with HIP events around every corrector and predictor launch and
+ * the Picard iterations counted per group; exa_patchset_group_times returns
+ * one group's totals over the timed steps of all waves (predictor and
+ * corrector ms, predictor launches, Picard iterations, predicted cells). */
+int exa_patchset_set_profiling(exa_patchset* b, int on);
+int exa_patchset_group_times(exa_patchset* b, int pde, int order, double* predictor_ms, double* corrector_ms,
+                             long* launches, unsigned long long* picard_iterations,
+                             unsigned long long* predictor_cells);
+/* Solution [n^3][B^3][V+P] of one patch after the run (kept patches only). */
 int exa_patchset_solution(exa_patchset* b, int patch, double* luh);
 /* Time stamp and the dt of the next step of one patch after the run. */
 int exa_patchset_time(exa_patchset* b, int patch, double* t, double* dt);

@@ -154,6 +154,14 @@ struct CorrectorArgs {
   int nx, ny, nz;
   int boundary;         // kBoundary*
   int adjust;           // apply the PDE's adjust_point_solution after the update
+  // optional batch of independent periodic patches (cubes): cell -> patch p,
+  // whose cells are patch_first[p] .. patch_first[p] + patch_n[p]^3 - 1; dx,
+  // dt and t per patch (dx[3 p], dt[p], t[p]); slots_per_patch dt slots per
+  // patch.  nx = the number of cells, ny = nz = 1.
+  const int* cell_patch;
+  const long* patch_first;
+  const int* patch_n;
+  int slots_per_patch;
 };
 hipError_t launch_corrector(const SolverDesc& sd, const CorrectorArgs& a, hipStream_t s);
 // Whether the library's instantiation for this solver supports a boundary

@@ -289,18 +289,27 @@ __global__ __launch_bounds__(node_threads(B)) void corrector_kernel(CorrectorArg
   const long cell = xcd_ordered_cell(ncells);
   if (cell >= ncells) return;  // whole workgroup
   const int tid = threadIdx.x;
-  const int ci = (int)(cell % a.nx), cj = (int)((cell / a.nx) % a.ny), ck = (int)(cell / ((long)a.nx * a.ny));
+  // the cell's grid (the whole grid, or its patch of a batch)
+  int nx = a.nx, ny = a.ny, nz = a.nz, patch = 0;
+  long base = 0;
+  if (a.cell_patch) {
+    patch = a.cell_patch[cell];
+    base = a.patch_first[patch];
+    nx = ny = nz = a.patch_n[patch];
+  }
+  const long local = cell - base;
+  const int ci = (int)(local % nx), cj = (int)((local / nx) % ny), ck = (int)(local / ((long)nx * ny));
   // neighbour across face f (-x, +x, -y, +y, -z, +z); -1 at a transmissive boundary
   auto neighbour = [&](int f) -> long {
     int i = ci, j = cj, k = ck;
     const int s = (f & 1) ? 1 : -1;
     if (f < 2) i += s; else if (f < 4) j += s; else k += s;
-    if (a.boundary != kBoundaryPeriodic && (i < 0 || i >= a.nx || j < 0 || j >= a.ny || k < 0 || k >= a.nz))
+    if (a.boundary != kBoundaryPeriodic && (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz))
       return -1;
-    i = (i + a.nx) % a.nx;
-    j = (j + a.ny) % a.ny;
-    k = (k + a.nz) % a.nz;
-    return ((long)k * a.ny + j) * a.nx + i;
+    i = (i + nx) % nx;
+    j = (j + ny) % ny;
+    k = (k + nz) % nz;
+    return base + ((long)k * ny + j) * nx + i;
   };
   const TF* const Qb = static_cast<const TF*>(a.lQhbnd);
   const TF* const Fb = static_cast<const TF*>(a.lFhbnd);
@@ -422,7 +431,9 @@ __global__ __launch_bounds__(node_threads(B)) void corrector_kernel(CorrectorArg
   const bool active = tid < B3;
   const int node = active ? tid : 0;
   const int z = node / B2, y = (node / B) % B, x = node % B;
-  const double invDx[3] = {1.0 / a.dx[0], 1.0 / a.dx[1], 1.0 / a.dx[2]};
+  const double* const dxp = a.dx + 3 * patch;
+  const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
+  const double dt = a.dt[patch];
   double* luh = a.luh + (size_t)cell * (B3 * D) + node * D;
   double q[D];
 #pragma unroll
@@ -453,7 +464,7 @@ __global__ __launch_bounds__(node_threads(B)) void corrector_kernel(CorrectorArg
     for (int m = 0; m < V; ++m)
       du[m] -= ((double)sFlux[(5 * B2 + y * B + x) * V + m] * fr - (double)sFlux[(4 * B2 + y * B + x) * V + m] * fl) * us;
   }
-  const double updateSize = a.dt[0] / (wz * wy * wx);
+  const double updateSize = dt / (wz * wy * wx);
 #pragma unroll
   for (int m = 0; m < V; ++m) q[m] += du[m] * updateSize;
   if constexpr (HasAdjustPointSolution<PDE>) {
@@ -477,7 +488,10 @@ __global__ __launch_bounds__(node_threads(B)) void corrector_kernel(CorrectorArg
   double dtn = active ? node_stable_dt<PDE>(q, invDx, a.cfl_pnpm) : DBL_MAX;
   dtn = dtn < DBL_MAX ? dtn : DBL_MAX;  // a NaN or inf candidate never wins (std::min order)
   dtn = block_min<NT>(dtn, sRed);
-  if (tid == 0) atomic_min_positive(a.dt_slots + cell % kDtSlots, dtn);
+  if (tid == 0)
+    atomic_min_positive(a.dt_slots + (a.cell_patch ? (size_t)patch * a.slots_per_patch + cell % a.slots_per_patch
+                                                   : (size_t)(cell % kDtSlots)),
+                        dtn);
 }
 
 // solutionAdjustment (solutionAdjustment.cpph:17-47) over a grid: one

@@ -108,3 +108,107 @@ def test_full_workload_96_patches_fp32(exa):
         t, dt = ps.time(j)
         assert np.isfinite(t) and t > 0 and np.isfinite(dt) and dt > 0
     ps.close()
+
+
+# elastic patches (velocity-stress NCP form) beside Euler ones: orders 2..5
+ELASTIC = [(3, 2, 11, 0.15), (3, 3, 12, 0.2), (2, 4, 13, 0.25), (2, 5, 14, 0.2)]
+
+
+def _mixed(exa):
+    return SMALL[:3] + [exa.elastic_patch(n, o, seed, width=w) for n, o, seed, w in ELASTIC]
+
+
+def _rel_var(a, b, nv):
+    a, b = a.reshape(-1, nv), b.reshape(-1, nv)
+    return max(np.abs(a[:, v] - b[:, v]).max() / max(np.abs(b[:, v]).max(), 1e-300) for v in range(nv))
+
+
+def test_elastic_patches_vs_oracle(exa, oracle):
+    """Elastic patches in the Euler patches' scheduler (groups of one PDE and
+    order): initial state against the host generator, then every patch
+    against the oracle's grid after 3 steps (each variable to 1e-12 of its
+    scale, time and next dt to 1e-13)."""
+    from refutil import elastic_cells
+    steps = 3
+    specs = _mixed(exa)
+    print(f"case: patch set with {len(ELASTIC)} elastic patches vs oracle grids, {steps} steps", flush=True)
+    ic, _, _ = _run(exa, specs, 0, use_graph=False, autotune=False)
+    sols, times, (_, dof, _) = _run(exa, specs, steps)
+    assert dof == sum(s[0] ** 3 * (s[1] + 1) ** 3 for s in SMALL[:3]) * steps + \
+        sum(n ** 3 * (o + 1) ** 3 for n, o, _, _ in ELASTIC) * steps
+    for j, spec in enumerate(specs):
+        if isinstance(spec, dict):
+            n, order = spec["n"], spec["order"]
+            nodes = np.array([oracle.table(oracle.T_NODES, order, i) for i in range(order + 1)])
+            host = elastic_cells(order, n, seed=spec["seed"], nodes=nodes, width=spec["width"])
+            assert _rel_var(ic[j], host, 12) < 1e-13, spec
+            o = oracle.Grid(oracle.elastic_solver(order), n)
+            nv = 12
+        else:
+            n, order = spec[0], spec[1]
+            o = oracle.Grid(oracle.make_solver(order, 5, 0), n)
+            nv = 5
+        o.set_solution(ic[j])
+        odts, _ = o.step(steps, nthreads=0)
+        err = _rel_var(sols[j], o.solution(), nv)
+        t, dt = times[j]
+        print(f"  patch {j} N={order} {n}^3: {err:.2e}", flush=True)
+        assert err < 1e-12, (spec, err)
+        assert abs(t - o.time()) <= 1e-13 * o.time()
+        assert abs(dt - o.stable_dt()) <= 1e-13 * dt, spec
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_patch_corrector_forms_agree(exa, precision):
+    """The one-kernel corrector inside patches against the face and update
+    passes: FP64 to 1e-13 of each variable's scale, FP32-bulk L2 1e-6."""
+    steps = 3
+    specs = _mixed(exa)
+    print(f"case: patch set corrector forms, precision {precision}", flush=True)
+    out = []
+    for fused in (True, False):
+        ps = exa.PatchSet(specs, precision=precision, hbm_budget_gb=100.0, keep_max_cells=1 << 30)
+        ps.set_fused_corrector(fused)
+        ps.run(steps, use_graph=True, autotune=False)
+        out.append([ps.solution(j) for j in range(len(specs))])
+        ps.close()
+    for spec, a, b in zip(specs, *out):
+        nv = 12 if isinstance(spec, dict) else 5
+        if precision == 0:
+            assert _rel_var(a, b, nv) < 1e-13, spec
+        else:
+            assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6, spec
+
+
+def test_elastic_patches_fp32_bulk_vs_fp64(exa):
+    steps = 4
+    specs = [exa.elastic_patch(n, o, seed, width=w) for n, o, seed, w in ELASTIC]
+    print("case: elastic patches FP32-bulk vs FP64", flush=True)
+    ref, tref, _ = _run(exa, specs, steps)
+    sols, times, _ = _run(exa, specs, steps, precision=exa.PRECISION_FP32_BULK)
+    for spec, a, b in zip(specs, ref, sols):
+        a, b = a.reshape(-1, 12)[:, :9], b.reshape(-1, 12)[:, :9]
+        l2 = np.linalg.norm(a - b) / np.linalg.norm(a)
+        assert l2 <= 1e-6, (spec, l2)
+
+
+def test_patch_profiling(exa):
+    """Profiled run: per-group launches, Picard counts and times; the same
+    solutions as the graph run (profiling only adds events)."""
+    steps = 2
+    specs = _mixed(exa)
+    print("case: patch set profiling", flush=True)
+    ref, _, _ = _run(exa, specs, steps, autotune=False)
+    ps = exa.PatchSet(specs, hbm_budget_gb=100.0, keep_max_cells=1 << 30)
+    ps.set_profiling(True)
+    ps.run(steps, autotune=False)
+    for j, spec in enumerate(specs):
+        assert np.array_equal(ps.solution(j), ref[j])
+    groups = {(exa.PDE_EULER, s[1]) for s in SMALL[:3]} | {(exa.PDE_ELASTIC, o) for _, o, _, _ in ELASTIC}
+    for pde, order in groups:
+        pred_ms, corr_ms, launches, picard, cells = ps.group_times(pde, order)
+        ncells = sum(n ** 3 for n, o, *_ in (SMALL[:3] if pde == exa.PDE_EULER else ELASTIC) if o == order)
+        assert launches == steps and cells == ncells * steps, (pde, order)
+        assert picard >= cells and pred_ms > 0 and corr_ms > 0
+    assert ps.group_times(exa.PDE_EULER, 7)[2] == 0
+    ps.close()

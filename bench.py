@@ -602,6 +602,50 @@ def cpu_baseline_batch(specs, threads, seconds):
                        "weighted by the patch set's DOF mix")
 
 
+def batch_profile(capi, specs, precision, steps, hbm_budget):
+    """A profiled run of the patch set (timed steps eager, events around every
+    corrector and predictor launch): per group (PDE, order) the predictor's
+    roofline from the counted Picard iterations, the corrector time, and the
+    share of the step the launches leave idle (what one persistent launch
+    over all groups could at most recover)."""
+    ps = capi.PatchSet(specs, precision=precision, hbm_budget_gb=hbm_budget)
+    ps.set_profiling(True)
+    ps.run(steps, use_graph=False, autotune=True, warmup=1)
+    sec, _, _ = ps.stats()
+    peak = FP32_PEAK_TFLOPS if precision == capi.PRECISION_FP32_BULK else FP64_PEAK_TFLOPS
+    groups, busy = [], 0.0
+    for pde in (capi.PDE_EULER, capi.PDE_ELASTIC):
+        for order in range(2, 8):
+            pred_ms, corr_ms, launches, it, cells = ps.group_times(pde, order)
+            if launches == 0:
+                continue
+            B = order + 1
+            if pde == capi.PDE_EULER:
+                flops = predictor_flops(B, 5, 5, it, cells)
+            else:
+                flops = predictor_flops_ncp(B, 9, 12, it, cells)
+            avg = pred_ms / launches
+            a = flops / launches / (avg * 1e-3) / 1e12
+            busy += pred_ms + corr_ms
+            groups.append(dict(pde="euler" if pde == capi.PDE_EULER else "elastic", order=order,
+                               predictor_ms_per_step=round(pred_ms / steps, 4),
+                               corrector_ms_per_step=round(corr_ms / steps, 4), avg_launch_ms=round(avg, 4),
+                               flops_per_launch=flops / launches, achieved_tflops=round(a, 3),
+                               frac=round(a / peak, 4), picard_mean=round(it / max(cells, 1), 4),
+                               variant=ps.predictor_variant(order, pde)))
+    ps.close()
+    dom = max(groups, key=lambda g: g["predictor_ms_per_step"])
+    roof = dict(bound="mfma", achieved=dom["achieved_tflops"], peak=peak, unit="TFLOP/s", frac=dom["frac"],
+                traffic=None, compute_unit="VALU",
+                kernel=f"predictor of the {dom['pde']} N={dom['order']} group (the largest predictor time)",
+                avg_launch_ms=dom["avg_launch_ms"], flops_per_launch=dom["flops_per_launch"],
+                picard_mean=dom["picard_mean"])
+    idle = max(0.0, 1.0 - busy * 1e-3 / sec) if sec > 0 else None
+    return roof, groups, dict(profiled_ms_per_step=round(sec / steps * 1e3, 4),
+                              kernel_ms_per_step=round(busy / steps, 4),
+                              idle_fraction=round(idle, 4) if idle is not None else None)
+
+
 def bench_batch(capi, args):
     """Patch-set workload: 96 seeded periodic Euler patches, sides in
     {9,15,27,45,81}, orders 2..7, entropy waves, per-patch dt; FP32-bulk vs FP64."""
@@ -627,6 +671,20 @@ def bench_batch(capi, args):
     num = sum(float(np.sum((res["fp32"]["sols"][j] - res["fp64"]["sols"][j]) ** 2)) for j in res["fp64"]["sols"])
     den = sum(float(np.sum(res["fp64"]["sols"][j] ** 2)) for j in res["fp64"]["sols"])
     m, m64 = res["fp32"], res["fp64"]
+    # the two-pass corrector (face pass + update pass) on the FP32-bulk form
+    ps = capi.PatchSet(specs, precision=capi.PRECISION_FP32_BULK, hbm_budget_gb=args.hbm_budget)
+    ps.set_fused_corrector(False)
+    ps.run(args.steps, use_graph=True, autotune=True, warmup=args.warmup)
+    sec2, dof2, _ = ps.stats()
+    ps.close()
+    roof, groups, launch = batch_profile(capi, specs, capi.PRECISION_FP32_BULK, 2, args.hbm_budget)
+    graph_ms = m["sec"] / args.steps * 1e3
+    launch["graph_ms_per_step"] = round(graph_ms, 4)
+    # the share of the graph step not covered by corrector and predictor
+    # launches (dt reductions and gaps): the most one persistent launch over
+    # all groups could recover
+    launch["idle_fraction_vs_graph"] = round(max(0.0, 1.0 - launch["kernel_ms_per_step"] / graph_ms), 4)
+    print(f"batch profile: {launch}", file=sys.stderr, flush=True)
     cells = sum(n ** 3 for n, *_ in specs)
     dof_step = sum(n ** 3 * (o + 1) ** 3 for n, o, _, _ in specs)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
@@ -640,7 +698,11 @@ def bench_batch(capi, args):
                                f"{cells} cells, {dof_step} DOF per step, periodic CFL0.9",
                    "seed": args.seed, "hbm_budget_gb": args.hbm_budget, "waves": m["nwaves"],
                    "launches": "one per stage and order group, captured once per wave in a hipGraph",
+                   "corrector": "one kernel per group (Riemann + surface + update + dt)",
                    "predictor_variant_by_order": m["variants"]},
+        "two_pass_corrector_fp32": {"value": dof2 / sec2, "ms_per_step": sec2 / args.steps * 1e3},
+        "groups_fp32": groups,
+        "launch_overhead_fp32": launch,
         "wall_s_including_setup": round(m["wall"], 3),
         "fp64": {"value": m64["dof"] / m64["sec"], "ms_per_step": m64["sec"] / args.steps * 1e3,
                  "waves": m64["nwaves"], "predictor_variant_by_order": m64["variants"],
@@ -649,7 +711,7 @@ def bench_batch(capi, args):
                          "l2_rel_err": (num / den) ** 0.5 if den > 0 else None,
                          "l2_over": f"the {len(m64['sols'])} patches with at most {keep} cells",
                          "max_rel_time_diff": max(abs(a[0] - b[0]) / b[0] for a, b in zip(m["times"], m64["times"]))},
-        "roofline": None,
+        "roofline": roof,
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_batch(specs, threads, args.cpu_seconds),
     }
     print(json.dumps(out), flush=True)
@@ -774,6 +836,20 @@ def bench_elastic(capi, args):
     a, b = res["fp32"]["sol"], res["fp64"]["sol"]
     l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     m, m64 = res["fp32"], res["fp64"]
+    # the same problem as one patch of the batching scheduler (exa_patchset)
+    patch = {}
+    for form in ("fp64", "fp32"):
+        prec = capi.PRECISION_FP32_BULK if form == "fp32" else capi.PRECISION_FP64
+        ps = capi.PatchSet([capi.elastic_patch(n, order, args.seed, width=0.05)], precision=prec,
+                           hbm_budget_gb=args.hbm_budget, keep_max_cells=n ** 3)
+        ps.run(args.steps, use_graph=True, autotune=True, warmup=args.warmup)
+        sec, pdof, _ = ps.stats()
+        sol = ps.solution(0).reshape(-1, 12)[:, :9]
+        ps.close()
+        ref = res[form]["sol"]
+        patch[form] = {"value": pdof / sec, "ms_per_step": sec / args.steps * 1e3,
+                       "l2_rel_diff_vs_grid": float(np.linalg.norm(sol - ref) / np.linalg.norm(ref))}
+        print(f"elastic patch set {form}: {patch[form]}", file=sys.stderr, flush=True)
     out = {
         "metric": "DOF-updates/s, 3D linear elasticity (NCP form) N=4 81^3: FP32-bulk vs FP64",
         "value": dof * args.steps / m["wall"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -787,6 +863,7 @@ def bench_elastic(capi, args):
                  "stage_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in m64["stages"].items()}},
         "fp32_vs_fp64": {"speedup": m64["wall"] / m["wall"], "l2_rel_err": l2,
                          "steps_compared": args.warmup + args.steps},
+        "patchset": patch,
         # the linear kernels on the same problem: a different scheme in its
         # face states (face-averaged coefficient matrix), so the difference to
         # the nonlinear FP64 run is a discretisation difference, not an error

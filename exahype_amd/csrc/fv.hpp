@@ -58,6 +58,16 @@ struct FvGridArgs {
 // One Godunov update u -> un with dt = state->dt, fused with the new
 // solution's eigenvalue maxima (atomic maxima into state->lam_max).
 hipError_t launch_fv_godunov_step(const FvGridArgs& a, hipStream_t s);
+// The MUSCL-Hancock grid's patches [cells][2][S^3][5] (S = L + 2G, current
+// buffer cur[cell]): the stable dt of the current interiors (begin: the
+// first dt, t = 0; else the end of a step, t += dt); cell-major interiors
+// [cells][L^3][5] to or from the current buffers; the next dt into a step
+// state.
+hipError_t launch_fv_patch_dt(const double* lim, const uint8_t* cur, long ncells, int L, int G, FvState* st,
+                              bool begin, hipStream_t s);
+hipError_t launch_fv_patch_io(double* cells, double* lim, const uint8_t* cur, long ncells, int L, int G, bool to_patch,
+                              hipStream_t s);
+hipError_t launch_fv_dt_to_step(const FvState* fs, StepState* ss, hipStream_t s);
 // t += dt; dt = min_d ch[d] / lam_max[d]; lam_max = 0; steps += 1.
 hipError_t launch_fv_finalize(FvState* st, hipStream_t s);
 // Eigenvalue maxima of u, then t = 0 and dt as above.

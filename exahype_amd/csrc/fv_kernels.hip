@@ -437,6 +437,63 @@ __global__ __launch_bounds__(256) void fv_euler_ic_kernel(ICArgs a, double* plan
   }
 }
 
+// The MUSCL-Hancock grid keeps one patch per cell with its ghost layers,
+// [cells][2 buffers][S^3][5] (S = L + 2G, the current buffer per cell in
+// cur): the eigenvalue maxima over the current interiors, for the grid's
+// stable dt as above.
+__global__ __launch_bounds__(256) void fv_patch_lambda_kernel(const double* lim, const uint8_t* cur, long ncells,
+                                                             int L, int G, FvState* st) {
+  const int S = L + 2 * G;
+  const long L3 = (long)L * L * L;
+  const size_t PATCH = (size_t)S * S * S * 5;
+  const size_t n = (size_t)ncells * L3;
+  double lm[3] = {0.0, 0.0, 0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = i / L3;
+    const int j = (int)(i % L3);
+    const int z = j / (L * L), y = (j / L) % L, x = j % L;
+    const double* p = lim + (c * 2 + cur[c]) * PATCH + ((size_t)((z + G) * S + (y + G)) * S + (x + G)) * 5;
+    double q[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = p[v];
+    const CellBase b = cell_base(q);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) lm[d] = dmax(lm[d], cell_lambda(q, b, d));
+  }
+  reduce_lambda(st, 0, lm[0]);
+  reduce_lambda(st, 1, lm[1]);
+  reduce_lambda(st, 2, lm[2]);
+}
+
+// cells [cells][L^3][5] (the reference's patch interiors, cell-major) <->
+// the interiors of the current patch buffers
+template <bool TO_PATCH>
+__global__ __launch_bounds__(256) void fv_patch_io_kernel(double* cells, double* lim, const uint8_t* cur, long ncells,
+                                                         int L, int G) {
+  const int S = L + 2 * G;
+  const long L3 = (long)L * L * L;
+  const size_t PATCH = (size_t)S * S * S * 5;
+  const size_t n = (size_t)ncells * L3;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = i / L3;
+    const int j = (int)(i % L3);
+    const int z = j / (L * L), y = (j / L) % L, x = j % L;
+    double* p = lim + (c * 2 + cur[c]) * PATCH + ((size_t)((z + G) * S + (y + G)) * S + (x + G)) * 5;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      if constexpr (TO_PATCH)
+        p[v] = cells[i * 5 + v];
+      else
+        cells[i * 5 + v] = p[v];
+    }
+  }
+}
+
+// the dt of the next update into the step state the patch kernels read
+__global__ void fv_dt_to_step_kernel(const FvState* fs, StepState* ss) {
+  if (threadIdx.x == 0) ss->dt = fs->dt;
+}
+
 unsigned stride_grid(size_t n) {
   const size_t b = (n + 255) / 256;
   return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
@@ -466,6 +523,32 @@ hipError_t launch_fv_initial_dt(const FvGridArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(fv_reset_lambda_kernel, dim3(1), dim3(64), 0, s, a.state);
   hipLaunchKernelGGL(fv_lambda_kernel, dim3(stride_grid((size_t)a.X * a.Y * a.Z)), dim3(256), 0, s, a);
   hipLaunchKernelGGL(fv_begin_kernel, dim3(1), dim3(64), 0, s, a.state);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_patch_dt(const double* lim, const uint8_t* cur, long ncells, int L, int G, FvState* st,
+                              bool begin, hipStream_t s) {
+  hipLaunchKernelGGL(fv_patch_lambda_kernel, dim3(stride_grid((size_t)ncells * L * L * L)), dim3(256), 0, s, lim, cur,
+                     ncells, L, G, st);
+  if (begin)
+    hipLaunchKernelGGL(fv_begin_kernel, dim3(1), dim3(64), 0, s, st);
+  else
+    hipLaunchKernelGGL(fv_finalize_kernel, dim3(1), dim3(64), 0, s, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_patch_io(double* cells, double* lim, const uint8_t* cur, long ncells, int L, int G, bool to_patch,
+                              hipStream_t s) {
+  const size_t n = (size_t)ncells * L * L * L;
+  if (to_patch)
+    hipLaunchKernelGGL(fv_patch_io_kernel<true>, dim3(stride_grid(n)), dim3(256), 0, s, cells, lim, cur, ncells, L, G);
+  else
+    hipLaunchKernelGGL(fv_patch_io_kernel<false>, dim3(stride_grid(n)), dim3(256), 0, s, cells, lim, cur, ncells, L, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_fv_dt_to_step(const FvState* fs, StepState* ss, hipStream_t s) {
+  hipLaunchKernelGGL(fv_dt_to_step_kernel, dim3(1), dim3(64), 0, s, fs, ss);
   return hipGetLastError();
 }
 

@@ -457,16 +457,20 @@ int exa_godunov_solution_update(const exa_solver* s, double* luh_new, const doub
  * neighbour merging (ghostLayerFilling, FVS:767-803; transmissive boundary
  * FVS:805-846), updateSolution with the scheme's solutionUpdate (FVS:662-715),
  * startNewTimeStep's stableTimeStepSize of the new solution with the solver's
- * CFL (FVS:553-574), dt = minimum over cells.  Euler (flux only), FP64,
- * scheme EXA_FV_GODUNOV.  Subcells live in dense planes without ghost layers:
+ * CFL (FVS:553-574), dt = minimum over cells.  Euler (flux only), FP64.
+ * EXA_FV_GODUNOV: subcells live in dense planes without ghost layers:
  * neighbour reads wrap (periodic) or clamp to the boundary subcell
  * (transmissive, the reference's mirror for ghost width 1), which gives the
- * values the reference's ghost layers hold.
+ * values the reference's ghost layers hold.  EXA_FV_MUSCLHANCOCK: every cell
+ * keeps its patch with two ghost layers (its results depend on the patches:
+ * the face-ghost subcells' transverse slopes read the never-filled, zero edge
+ * ghosts, as in the reference); patch sizes 5 and 7.
  * ------------------------------------------------------------------------- */
 typedef struct exa_fvgrid exa_fvgrid;
 /* nx*ny*nz cells on [lo, hi], cell c = (k*ny + j)*nx + i, patch_size L;
  * boundary EXA_BC_PERIODIC or EXA_BC_TRANSMISSIVE (boundaryValues returning
- * the inside state).  EXA_FV_MUSCLHANCOCK returns EXA_E_UNSUPPORTED. */
+ * the inside state).  EXA_E_UNSUPPORTED for MUSCL-Hancock with L other than
+ * 5 or 7. */
 int exa_fvgrid_create(const exa_solver* s, int scheme, int patch_size, int nx, int ny, int nz,
                       const double* lo, const double* hi, int boundary, exa_fvgrid** out);
 int exa_fvgrid_destroy(exa_fvgrid* g);

@@ -69,3 +69,78 @@ def test_elastic_fp32_bulk_vs_fp64(exa):
     l2 = np.linalg.norm(sols[1] - sols[0]) / np.linalg.norm(sols[0])
     print(f"  L2 rel {l2:.3e}", flush=True)
     assert l2 <= 1e-5
+
+
+def _elastic_neighbourhood_update(oracle, s, luh, n, c, dx, dt):
+    """One reference step of cell c from its own and its 6 neighbours' states
+    (periodic), for the elastic PDE: predictor of all seven cells, the six
+    Riemann problems of c with the neighbours' opposite face blocks (left
+    state on the lower side), surface integral, update."""
+    B, V, D = s.order + 1, s.nvar, s.nvar + s.npar
+    nb3 = B ** 3 * D
+    fq, ff = B * B * D, B * B * V
+    i, j, k = c % n, (c // n) % n, c // (n * n)
+
+    def cid(a, b, cc):
+        return ((cc % n) * n + (b % n)) * n + (a % n)
+
+    nbrs = {"c": c, "x-": cid(i - 1, j, k), "x+": cid(i + 1, j, k), "y-": cid(i, j - 1, k),
+            "y+": cid(i, j + 1, k), "z-": cid(i, j, k - 1), "z+": cid(i, j, k + 1)}
+    pred = {key: oracle.space_time_predictor(s, luh[v * nb3:(v + 1) * nb3], dx, dt) for key, v in nbrs.items()}
+    lduh = oracle.volume_integral(s, pred["c"]["lFhi"], dx)
+    lF = pred["c"]["lFhbnd"].copy().reshape(6, ff)
+    Qc = pred["c"]["lQhbnd"].reshape(6, fq)
+    for d, (lo, hi) in enumerate((("x-", "x+"), ("y-", "y+"), ("z-", "z+"))):
+        _, FR = oracle.riemann(s, pred[lo]["lFhbnd"].reshape(6, ff)[2 * d + 1], lF[2 * d],
+                               pred[lo]["lQhbnd"].reshape(6, fq)[2 * d + 1], Qc[2 * d], dt, d)
+        lF[2 * d] = FR
+        FL, _ = oracle.riemann(s, lF[2 * d + 1], pred[hi]["lFhbnd"].reshape(6, ff)[2 * d], Qc[2 * d + 1],
+                               pred[hi]["lQhbnd"].reshape(6, fq)[2 * d], dt, d)
+        lF[2 * d + 1] = FL
+    lduh = oracle.surface_integral(s, lduh, lF.reshape(-1), dx)
+    return oracle.solution_update(s, luh[c * nb3:(c + 1) * nb3], lduh, dt)
+
+
+def test_elastic_workload_full_size_81cube(exa, oracle):
+    """The elastic workload at its real size (81^3 cells, N=4, pulse width
+    0.05, seeded material): one step of 24 sampled cells around the pulse and
+    8 elsewhere recomputed by the oracle from their neighbourhoods (each
+    variable to 1e-12 of its scale over the sample), the first dt recomputed
+    over all cells, and FP32-bulk against FP64 after 3 steps (L2 < 1e-6)."""
+    order, n = 4, 81
+    B, D = order + 1, 12
+    nb3 = B ** 3 * D
+    print("case: elastic full-size 81^3 N=4 sampled parity", flush=True)
+    g = exa.Grid(exa.elastic_solver(order), n)
+    g.init_elastic(seed=2024, width=0.05)
+    luh0 = g.solution()
+    g.step(1)
+    dts = g.dt_history(1)
+    luh1 = g.solution()
+    s = oracle.elastic_solver(order)
+    og = oracle.Grid(s, n)
+    og.set_solution(luh0)
+    dt0 = og.stable_dt()
+    del og
+    assert abs(dts[0] - dt0) <= 1e-13 * dt0
+    dx = np.full(3, 1.0 / n)
+    rng = np.random.default_rng(1)
+    near = [((40 + a) * n + (40 + b)) * n + (40 + c) for a, b, c in rng.integers(-3, 4, size=(24, 3))]
+    cells = sorted(set(near) | set(int(x) for x in rng.choice(n ** 3, 8, replace=False)))
+    got = np.stack([luh1[c * nb3:(c + 1) * nb3] for c in cells]).reshape(-1, D)
+    ref = np.stack([_elastic_neighbourhood_update(oracle, s, luh0, n, c, dx, dts[0]) for c in cells]).reshape(-1, D)
+    err = max(np.abs(got[:, v] - ref[:, v]).max() / max(np.abs(ref[:, v]).max(), 1e-300) for v in range(D))
+    print(f"  {len(cells)} cells, max rel error per variable {err:.2e}", flush=True)
+    assert err <= 1e-12
+    assert np.abs(got[:, 6:9]).max() > 1e-3  # the sample holds the pulse
+    g.step(2)
+    a64 = g.solution().reshape(-1, D)[:, :9]
+    g.close()
+    g32 = exa.Grid(exa.elastic_solver(order, precision=exa.PRECISION_FP32_BULK), n)
+    g32.init_elastic(seed=2024, width=0.05)
+    g32.step(3)
+    a32 = g32.solution().reshape(-1, D)[:, :9]
+    g32.close()
+    l2 = np.linalg.norm(a32 - a64) / np.linalg.norm(a64)
+    print(f"  FP32-bulk vs FP64 after 3 steps: L2 {l2:.2e}", flush=True)
+    assert l2 < 1e-6

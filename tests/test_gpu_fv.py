@@ -154,3 +154,48 @@ def test_fvgrid_full_size_conservation(exa):
         assert abs(tot2[v] - tot0[v]) <= 1e-11 * scale, f"total of variable {v}"
     assert np.abs(u2 - u0).max() > 1e-6  # it evolved
     g.close()
+
+
+@pytest.mark.parametrize("n,L,periodic", [((2, 3, 2), 5, True), ((3, 2, 2), 7, False), ((3, 3, 3), 5, True),
+                                          ((4, 1, 2), 7, True)])
+def test_fvgrid_musclhancock_vs_oracle(exa, oracle, n, L, periodic):
+    """The stand-alone MUSCL-Hancock FV solver (FiniteVolumesSolver with
+    musclhancock.cpph:49): per-cell patches with two ghost layers on the
+    device against the oracle's patch-and-ghost-layer driver after every
+    step (dt 1e-13, time stamp, solution 1e-12 of each variable's scale)."""
+    steps = 4
+    print(f"case: MUSCL-Hancock FV grid vs oracle, cells {n} L={L} periodic={periodic}, {steps} steps", flush=True)
+    _, cells = fv_grid_state(n, L, fv_smooth)
+    s = exa.euler_solver(3)
+    g = exa.FVGrid(s, L, n, boundary=exa.BC_PERIODIC if periodic else exa.BC_TRANSMISSIVE,
+                   scheme=exa.FV_MUSCLHANCOCK)
+    g.upload(cells)
+    o = oracle.FVGrid(oracle.make_solver(3), oracle.FV_MUSCLHANCOCK, L, n, periodic=periodic)
+    o.set_solution(cells)
+    assert np.array_equal(g.solution(), cells.ravel())
+    for k in range(steps):
+        t, dt, nst = g.time()
+        assert nst == k
+        assert abs(dt - o.dt()) <= 1e-13 * o.dt(), f"dt of step {k}"
+        assert abs(t - o.time()) <= 1e-13 * max(o.time(), 1e-300)
+        g.step(1)
+        o.step(1)
+        err = _var_err(g.solution(), o.solution())
+        assert err <= 1e-12, f"solution after step {k}: {err:.3e}"
+    g.close()
+
+
+def test_fvgrid_musclhancock_init_and_refusals(exa):
+    """The device initial condition through the patches; patch sizes other
+    than 5 and 7 are refused."""
+    n, L = (2, 2, 3), 5
+    g = exa.FVGrid(exa.euler_solver(3), L, n, scheme=exa.FV_MUSCLHANCOCK)
+    g.init_euler(rho0=1.0, amp=0.2, k=(1, 2, 1), vel=(0.5, -0.25, 0.1), p=1.3)
+    h = exa.FVGrid(exa.euler_solver(3), L, n)
+    h.init_euler(rho0=1.0, amp=0.2, k=(1, 2, 1), vel=(0.5, -0.25, 0.1), p=1.3)
+    assert np.array_equal(g.solution(), h.solution())
+    assert g.time()[1] == h.time()[1]
+    g.close()
+    h.close()
+    with pytest.raises(exa.ExaError, match="status 2"):
+        exa.FVGrid(exa.euler_solver(3), 6, 2, scheme=exa.FV_MUSCLHANCOCK)

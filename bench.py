@@ -143,6 +143,32 @@ def measured_traffic(form, order, n):
     return None
 
 
+CORR_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "corrector_euler_n5_81cube_pmc_traffic.txt")
+
+
+def corrector_roofline(r, form, order, n, B, V, D):
+    """HBM roofline of the one-kernel corrector (the step's second kernel):
+    per cell it reads luh (double, B^3 D), lduh (B^3 V) and the cell's face
+    blocks (6 B^2 (D + V)) in the bulk type and writes luh (B^3 V doubles);
+    the neighbours' face blocks it also reads are other cells' own blocks, so
+    the minimal traffic counts each block once (`traffic`, when recorded,
+    shows the re-reads).  Launch time from the grid's HIP events."""
+    ms, nl = r["stages"]["update"]
+    avg = ms / max(nl, 1)
+    bulk = 4 if form == "fp32" else 8
+    alg = n ** 3 * (8 * B ** 3 * D + bulk * (B ** 3 * V + 6 * B * B * (D + V)) + 8 * B ** 3 * V)
+    a = alg / (avg * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(CORR_TRAFFIC_FILE):
+        key = f"traffic_bytes_per_launch corrector {form} N={order} {n}^3"
+        for line in open(CORR_TRAFFIC_FILE):
+            if line.startswith(key + " "):
+                traffic = int(line.split()[-1])
+    return dict(bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(a / HBM_PEAK_GBS, 4),
+                traffic=traffic, kernel=f"corrector_kernel<6,Euler,{'float' if form == 'fp32' else 'double'}>",
+                avg_launch_ms=round(avg, 4), algorithmic_bytes_per_launch=alg)
+
+
 def roofline_for(r, B, V, D, peak, kernel, compute_unit, valu_peak=None, executed=None):
     """Compute roofline of the predictor.  `bound` is the contract's category
     (compute, i.e. "mfma", not "hbm"); `compute_unit` names the unit that
@@ -1018,6 +1044,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if args.corrector == "fused":
+        out["corrector_roofline"] = corrector_roofline(m, main_form, order, n, B, V, D)
     if "fp64" in res:
         r64 = res["fp64"]
         out["fp64"] = {"value": dof * args.steps / r64["wall"], "ms_per_step": r64["wall"] / args.steps * 1e3,

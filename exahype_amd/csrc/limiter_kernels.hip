@@ -539,7 +539,7 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
   __shared__ double sWave[kStatusCells * PER];
   load_tables<N>(sUh2lim, sUh2lob, nullptr);
   __syncthreads();  // the only workgroup barrier: before any wavefront can leave
-  const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
   const int c = blockIdx.x * kStatusCells + wid;
   if (c >= a.ncells) return;
   double* const sQ = sWave + wid * PER;
@@ -547,6 +547,16 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
   double* const t2 = t1 + T1;
   double* const sSub = t2 + T2;
   double* const sLob = sSub + L3;
+  // loads whose latency the DG update hides: the merged face extrema (one
+  // lane per face-block entry) and the neighbours' previous statuses (one
+  // lane per face)
+  const double bnd = lane < 6 * V ? a.mn[(size_t)c * 6 * V + lane]
+                                  : (lane >= 32 && lane < 32 + 6 * V ? a.mx[(size_t)c * 6 * V + lane - 32] : 0.0);
+  const int nbf = lane < 6 ? a.nbr[c * 6 + lane] : -1;
+  const int onb = nbf >= 0 ? a.pst[nbf] : 0;
+  const int itc0 = a.itc[c];
+  const bool had_patch = a.has_patch[c] != 0;
+  const int cur0 = a.cur[c];
   const int s = a.pst[c];
   const bool active = lane < B3;
   const int node = active ? lane : 0;
@@ -599,15 +609,18 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
     for (int m = 0; m < V; ++m) sQ[m * B3 + node] = q[m];
   }
   // ensureNoUnrequiredLimiterPatchIsAllocatedOnComputeCell (status before the update)
-  bool patch = a.has_patch[c] != 0;
+  bool patch = had_patch;
   if (s == 0) patch = false;
-  // startNewTimeStep: stableTimeStepSize of the new solution
+  // startNewTimeStep: stableTimeStepSize of the new solution.  The slot's
+  // returning atomic is waited for at the end of the kernel (its result is
+  // consumed there), so its latency overlaps the min/max work.
+  unsigned long long dt_old = 0;
   {
     double dtn = active ? node_stable_dt<Euler<0>>(q, invDx, a.cfl_pnpm) : DBL_MAX;
     dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
     dtn = wave_min(dtn);
     if (lane == 0) {
-      atomic_min_positive(&a.st_state->dt_slots[c % kDtSlots], dtn);
+      dt_old = atomicMin(&a.st_state->dt_slots[c % kDtSlots], (unsigned long long)__double_as_longlong(dtn));
       a.dt_cell[c] = dtn;
     }
   }
@@ -647,18 +660,22 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
       a.loc_mm[(size_t)c * 2 * V + V + v] = mx[v];
     }
   }
-  // DMP (LIM:114-165) against the merged boundary extrema
+  // DMP (LIM:114-165) against the merged boundary extrema, staged in LDS
+  // (sSub is free until the helper projection below)
   double* bmn = a.mn + (size_t)c * 6 * V;
   double* bmx = a.mx + (size_t)c * 6 * V;
+  if (lane < 6 * V) sSub[lane] = bnd;
+  if (lane >= 32 && lane < 32 + 6 * V) sSub[6 * V + lane - 32] = bnd;
+  wave_lds_sync();
   bool ok = true;
   double bMin[V], bMax[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    double lo = bmn[v], hi = bmx[v];
+    double lo = sSub[v], hi = sSub[6 * V + v];
 #pragma unroll
     for (int f = 1; f < 6; ++f) {
-      lo = dmin(lo, bmn[f * V + v]);
-      hi = dmax(hi, bmx[f * V + v]);
+      lo = dmin(lo, sSub[f * V + v]);
+      hi = dmax(hi, sSub[6 * V + f * V + v]);
     }
     double sd = (hi - lo) * a.scaling;
     sd = dmax(sd, a.relax);
@@ -687,7 +704,7 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
   // determineLimiterStatusAfterSolutionUpdate (LS:955-994); the face-wise
   // statuses come from mergeWithLimiterStatus (AS:2547-2563) of the merge phase
   int ns;
-  int itc = a.itc[c];
+  int itc = itc0;
   if (!ok) {
     itc = a.cure + 1;
     ns = a.T;
@@ -700,21 +717,17 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
       itc = a.cure + 1;
     }
   } else {
-    ns = 0;
+    // the six face statuses (mergeWithLimiterStatus) from the lanes that
+    // loaded them, their maximum over the wavefront
     const int own = s < a.T ? s : a.T;
-    for (int f = 0; f < 6; ++f) {
-      const int nb = a.nbr[c * 6 + f];
-      if (nb < 0) continue;
-      int o = a.pst[nb];
-      o = o < a.T ? o : a.T;
-      const int fs = (own > o ? own : o) - 1;
-      ns = fs > ns ? fs : ns;
-    }
+    const int o = onb < a.T ? onb : a.T;
+    const int fs = nbf >= 0 ? (own > o ? own : o) - 1 : 0;
+    ns = (int)(-wave_min(-(double)(fs > 0 ? fs : 0)));
   }
   // helper cells: swap + project (LS:861-873); new patches: project (LS:940-953)
   const bool helper = s > 0 && s < a.A;
   const bool fresh = !patch && ns > 0;
-  const int buf = a.cur[c] ^ (helper ? 1 : 0);
+  const int buf = cur0 ^ (helper ? 1 : 0);
   if (helper || (fresh && ns < a.A)) {
     double* const dst = patch_ptr(a, c, buf, PATCH);
     constexpr int G = Sh::G, S = Sh::S;
@@ -734,6 +747,7 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
     a.has_patch[c] = (uint8_t)((patch || fresh) ? 1 : 0);
     a.cur[c] = (uint8_t)buf;
   }
+  asm volatile("" : : "v"(dt_old));  // the dt atomic has been performed
 }
 
 // ---------------------------------------------------------------------------

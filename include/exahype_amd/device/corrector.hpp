@@ -254,6 +254,28 @@ __device__ inline long xcd_ordered_cell(long ncells) {
   return (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
+// A walk of an nx x ny x nz grid that keeps face neighbours close in time:
+// blocks of T planes, in them blocks of T rows, in them the x columns one
+// after the other, each column's T x T cells together.  A cell's x, y and z
+// neighbours inside its block are then 16, 1 and 4 positions away (instead
+// of 1, nx and nx ny), so the neighbour's face blocks are still in the L2.
+// Partial blocks at the grid's high ends keep the walk a permutation.
+__device__ inline long blocked_cell(long idx, int nx, int ny, int nz) {
+  constexpr int T = 4;
+  const long plane_block = (long)T * ny * nx;
+  const int kb = (int)(idx / plane_block);
+  long r = idx - kb * plane_block;
+  const int kdim = nz - kb * T < T ? nz - kb * T : T;
+  const long row_block = (long)T * nx * kdim;
+  const int jb = (int)(r / row_block);
+  r -= jb * row_block;
+  const int jdim = ny - jb * T < T ? ny - jb * T : T;
+  const int i = (int)(r / (jdim * kdim));
+  const int rest = (int)(r - (long)i * jdim * kdim);
+  const int kk = rest / jdim, jj = rest - kk * jdim;
+  return ((long)(kb * T + kk) * ny + (jb * T + jj)) * nx + i;
+}
+
 // boundaryValues at face node fn of face f of cell (ci, cj, ck), normal
 // direction DN (boundaryConditions.cpph:66-83): x on the face, the two free
 // coordinates from the node (i, j) along (d2, d1); Fout/Qout hold the inside
@@ -286,8 +308,9 @@ __global__ __launch_bounds__(node_threads(B)) void corrector_kernel(CorrectorArg
   __shared__ double sRed[NT / kWave];
 
   const long ncells = (long)a.nx * a.ny * a.nz;
-  const long cell = xcd_ordered_cell(ncells);
-  if (cell >= ncells) return;  // whole workgroup
+  const long slot = xcd_ordered_cell(ncells);
+  if (slot >= ncells) return;  // whole workgroup
+  const long cell = a.cell_patch ? slot : blocked_cell(slot, a.nx, a.ny, a.nz);
   const int tid = threadIdx.x;
   // the cell's grid (the whole grid, or its patch of a batch)
   int nx = a.nx, ny = a.ny, nz = a.nz, patch = 0;
@@ -313,6 +336,24 @@ __global__ __launch_bounds__(node_threads(B)) void corrector_kernel(CorrectorArg
   };
   const TF* const Qb = static_cast<const TF*>(a.lQhbnd);
   const TF* const Fb = static_cast<const TF*>(a.lFhbnd);
+  // the node's solution and volume update: a large node state (elastic, 12
+  // quantities) is loaded before the face phase, so that its latency overlaps
+  // that phase (5.67 -> 4.97 ms per launch at 81^3, N=5); a small one (Euler,
+  // 5) after it, where the early load's registers would cost an occupancy
+  // step for no gain (3.69 -> 3.75 ms)
+  constexpr bool kEarlyNodeLoads = D > 8;
+  const bool active = tid < B3;
+  const int node = active ? tid : 0;
+  double* luh = a.luh + (size_t)cell * (B3 * D) + node * D;
+  const TF* lduhp = static_cast<const TF*>(a.lduh) + (size_t)cell * (B3 * V) + node * V;
+  double q[D], du[V];
+  auto load_node = [&] {
+#pragma unroll
+    for (int m = 0; m < D; ++m) q[m] = luh[m];
+#pragma unroll
+    for (int m = 0; m < V; ++m) du[m] = lduhp[m];
+  };
+  if constexpr (kEarlyNodeLoads) load_node();
 
   // ---- face nodes: load both sides, stage the weighted states ----
   TF qL[R][D], qR[R][D], fL[R][V], fR[R][V];
@@ -428,21 +469,12 @@ __global__ __launch_bounds__(node_threads(B)) void corrector_kernel(CorrectorArg
   }
   __syncthreads();
   // ---- surface integral + update + stable dt (as update_kernel) ----
-  const bool active = tid < B3;
-  const int node = active ? tid : 0;
   const int z = node / B2, y = (node / B) % B, x = node % B;
   const double* const dxp = a.dx + 3 * patch;
   const double invDx[3] = {1.0 / dxp[0], 1.0 / dxp[1], 1.0 / dxp[2]};
   const double dt = a.dt[patch];
-  double* luh = a.luh + (size_t)cell * (B3 * D) + node * D;
-  double q[D];
-#pragma unroll
-  for (int m = 0; m < D; ++m) q[m] = luh[m];
+  if constexpr (!kEarlyNodeLoads) load_node();
   const double wx = exa_gl_weights[N][x], wy = exa_gl_weights[N][y], wz = exa_gl_weights[N][z];
-  const TF* lduhp = static_cast<const TF*>(a.lduh) + (size_t)cell * (B3 * V) + node * V;
-  double du[V];
-#pragma unroll
-  for (int m = 0; m < V; ++m) du[m] = lduhp[m];
   {  // x faces, (z,y)
     const double us = wz * wy * invDx[0];
     const double fr = exa_fr_coeff[N][x], fl = exa_fl_coeff[N][x];

@@ -43,7 +43,7 @@ EXPORTED = (
     "exa_lgrid_create", "exa_lgrid_destroy", "exa_lgrid_num_cells", "exa_lgrid_upload", "exa_lgrid_init_blast",
     "exa_lgrid_step", "exa_lgrid_synchronize", "exa_lgrid_time", "exa_lgrid_download", "exa_lgrid_status",
     "exa_lgrid_limiter_solution", "exa_lgrid_min_max", "exa_lgrid_set_profiling", "exa_lgrid_stage_times",
-    "exa_lgrid_picard_totals",
+    "exa_lgrid_picard_totals", "exa_lgrid_set_fused_time_stepping", "exa_lgrid_predictor_reruns",
     "exa_grid_set_predictor_variant", "exa_grid_set_fused_corrector", "exa_grid_create_user",
     "exa_grid_set_fused_time_stepping", "exa_grid_predictor_reruns",
     "exa_grid_set_adjust_solution", "exa_grid_adjust_solution", "exa_patchset_create", "exa_patchset_run", "exa_patchset_stats", "exa_patchset_predictor_variant",
@@ -229,6 +229,8 @@ def lib():
     L.exa_lgrid_set_profiling.argtypes = [V, i]
     L.exa_lgrid_stage_times.argtypes = [V, _D, ctypes.POINTER(ctypes.c_long)]
     L.exa_lgrid_picard_totals.argtypes = [V, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
+    L.exa_lgrid_set_fused_time_stepping.argtypes = [V, i, d_]
+    L.exa_lgrid_predictor_reruns.argtypes = [V, ctypes.POINTER(ctypes.c_long)]
     L.exa_patchset_create.argtypes = [ctypes.POINTER(PatchSpec), i, i, d_, ctypes.c_long, ctypes.POINTER(V)]
     L.exa_patchset_run.argtypes = [V, i, i, i, i]
     L.exa_patchset_stats.argtypes = [V, _D, _D, _I]
@@ -836,6 +838,16 @@ class LimitingGrid:
 
     def step(self, n=1):
         check(lib().exa_lgrid_step(self._h, n), "step")
+
+    def set_fused_time_stepping(self, on=True, weight=0.9):
+        """fuse-algorithmic-steps of the limiting solver; set after upload or
+        init and before the first step for the fused start."""
+        check(lib().exa_lgrid_set_fused_time_stepping(self._h, int(on), float(weight)), "fused time stepping")
+
+    def predictor_reruns(self):
+        n = ctypes.c_long()
+        check(lib().exa_lgrid_predictor_reruns(self._h, ctypes.byref(n)), "predictor reruns")
+        return n.value
 
     def synchronize(self):
         check(lib().exa_lgrid_synchronize(self._h), "synchronize")

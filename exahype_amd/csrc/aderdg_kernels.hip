@@ -86,6 +86,72 @@ hipError_t launch_dt_finalize_fused(StepState* st, double weight, bool init, hip
   return hipGetLastError();
 }
 
+// Fused time stepping of the limiting solver, in a one-process run.  Phase 1
+// is the SolutionUpdate endIteration (SolutionUpdate.cpp:305-318): without an
+// irregular limiter-domain change the check and startNewTimeStep of the plain
+// solver; with one, the rollback (Reinitialisation.cpp:105-120) takes the
+// time stamps back again, so only the checked predictor dt survives (the
+// "used" dt of the repeated check).  Phase 2, after the local recomputation,
+// repeats the check with the recomputed cells' candidates
+// (LocalRecomputation.cpp:168-222) and then the adapter's
+// TimeStepSizeComputation sets corrector and predictor dt to the new stable
+// dt (updateTimeStepSizesFused, ADERDGSolver.cpp:594-612).
+__global__ __launch_bounds__(256) void dt_finalize_fused_limiting_kernel(StepState* st, double weight,
+                                                                          const int* irregular, int phase) {
+#pragma clang fp contract(off)
+  __shared__ double sRed[256 / kWave];
+  const bool irr = *irregular != 0;
+  if (phase == 2 && !irr) return;
+  double m = 1.0e308;
+  for (int i = threadIdx.x; i < kDtSlots; i += 256) {
+    const double v = __longlong_as_double((long long)atomic_take(&st->dt_slots[i], kDtSlotEmpty));
+    m = v < m ? v : m;
+  }
+  m = block_min<256>(m, sRed);
+  if (threadIdx.x != 0) return;
+  const double stable = m;
+  if (phase == 1) {
+    double next = stable;
+    double used = st->dtp;
+    if (used == 0.0) used = stable;
+    const int violated = used > stable ? 1 : 0;
+    double checked = st->dtp;
+    if (violated) {
+      next = fmin(next, weight * stable);
+      checked = weight * stable;
+    } else {
+      next = fmin(next, 0.5 * (used + weight * stable));
+    }
+    if (irr) {
+      st->dtp_checked = checked;
+      return;
+    }
+    st->t += st->dt;
+    st->steps += 1;
+    st->dt = checked;
+    st->dtp = next;
+    st->violated = violated;
+    st->reruns += violated;
+  } else {
+    double used = st->dtp_checked;
+    if (used == 0.0) used = stable;
+    const int violated = used > stable ? 1 : 0;
+    st->t += st->dt;
+    st->steps += 1;
+    st->dt = stable;
+    st->dtp = stable;
+    st->violated = violated;
+    st->reruns += violated;
+  }
+  if (st->dt_hist && st->steps < st->dt_hist_len) st->dt_hist[st->steps] = st->dt;
+}
+
+hipError_t launch_dt_finalize_fused_limiting(StepState* st, double weight, const int* irregular, int phase,
+                                             hipStream_t s) {
+  hipLaunchKernelGGL(dt_finalize_fused_limiting_kernel, dim3(1), dim3(256), 0, s, st, weight, irregular, phase);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Stand-alone per-cell kernels for the drop-in single-cell / batch API (FP64).
 // ---------------------------------------------------------------------------

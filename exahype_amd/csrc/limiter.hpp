@@ -88,6 +88,9 @@ hipError_t launch_lim_update_status(const SolverDesc& sd, const LimArgs& a, hipS
 hipError_t launch_lim_spread_rollback(const SolverDesc& sd, const LimArgs& a, hipStream_t s);
 // irregular change: recompute helpers, new dt and min/max of every cell
 hipError_t launch_lim_recompute_finish(const SolverDesc& sd, const LimArgs& a, hipStream_t s);
+// irregular change in fused time stepping: out[c] = 0 for the cells whose
+// prediction is recomputed, T for the others (the predictor's status gate)
+hipError_t launch_lim_repredict_mask(const LimArgs& a, int* out, hipStream_t s);
 // initial state: DMP min/max, admissibility, spreading, FV initial condition
 // into the patches of cells with status > 0 (lim_ic [cells][L^3][V] or null)
 hipError_t launch_lim_init(const SolverDesc& sd, const LimArgs& a, const double* lim_ic, hipStream_t s);

@@ -90,6 +90,8 @@ struct exa_lgrid {
   void* ws = nullptr;
   long ws_cells = 0;
   bool profiling = false;
+  bool fused_ts = false;      // fused time stepping (fuse-algorithmic-steps)
+  double fused_weight = 0.9;  // fuse-algorithmic-steps-factor
   std::vector<LProf> prof;
   double prof_ms[EXA_NUM_LSTAGES] = {0, 0, 0, 0, 0, 0};
   long prof_n[EXA_NUM_LSTAGES] = {0, 0, 0, 0, 0, 0};
@@ -160,7 +162,9 @@ void lfree(exa_lgrid* g) {
   delete g;
 }
 
-hipError_t lpredict(exa_lgrid* g) {
+// Prediction of the cells with status[c] < troubled (the limiter status, or
+// the re-prediction mask) with the device dt; with run_if, only if *run_if.
+hipError_t lpredict(exa_lgrid* g, const double* dt, const int* status, const int* run_if = nullptr) {
   exa::PredictorArgs a{};
   a.luh = g->luh;
   a.lduh = g->lduh;
@@ -168,16 +172,20 @@ hipError_t lpredict(exa_lgrid* g) {
   a.lFhbnd = g->lFhbnd;
   a.dx = g->d_dx;
   a.dx_stride = 0;
-  a.dt = &g->state->dt;
-  a.iters = g->iters;
+  a.dt = dt;
+  // the gated launches skip cells without writing their iteration count
+  const bool gated = run_if != nullptr;
+  a.iters = gated ? nullptr : g->iters;
   a.cell0 = 0;
   a.ws = g->ws;
   a.ws_cells = g->ws_cells;
-  a.status = g->st;  // Prediction.cpp:177: cells with status < troubled
+  a.status = status;  // Prediction.cpp:177: cells with status < troubled
   a.status_limit = 2 * g->h + 1;
+  a.run_if = run_if;
   hipError_t e =
       lstaged(g, EXA_STAGE_PREDICTOR, [&] { return exa::launch_predictor(g->sd, false, a, g->ncells, g->stream); });
-  if (e == hipSuccess && g->profiling) e = exa::launch_count_iterations(g->iters, g->ncells, g->state, g->stream);
+  if (e == hipSuccess && g->profiling && !gated)
+    e = exa::launch_count_iterations(g->iters, g->ncells, g->state, g->stream);
   return e;
 }
 
@@ -197,12 +205,17 @@ hipError_t lstart(exa_lgrid* g) {
   u.cfl_pnpm = g->sd.cfl * kPNPM[g->sd.order];
   u.dt_slots = g->state->dt_slots;
   e = exa::launch_update(g->sd, 0, u, g->ncells, g->stream);
-  if (e == hipSuccess) e = exa::launch_dt_finalize(g->state, false, g->stream);
-  if (e == hipSuccess) e = lpredict(g);
+  if (e == hipSuccess)
+    e = g->fused_ts ? exa::launch_dt_finalize_fused(g->state, g->fused_weight, true, g->stream)
+                    : exa::launch_dt_finalize(g->state, false, g->stream);
+  if (e == hipSuccess) e = lpredict(g, &g->state->dt, g->st);
   return e;
 }
 
-// One non-fused time step of the limiting solver.
+// One time step of the limiting solver: non-fused, or fused (fusedTimeStep,
+// LS:806-836, in the SolutionUpdate traversal; updateMeshAndSubdomains,
+// Runner.cpp:866-946, after an irregular change; the re-run of the
+// prediction after a failed stability check, Runner.cpp:1101-1107).
 hipError_t lstep(exa_lgrid* g) {
   const exa::LimArgs a = largs(g);
   const SolverDesc& sd = g->sd;
@@ -224,6 +237,17 @@ hipError_t lstep(exa_lgrid* g) {
     });
   if (e == hipSuccess) e = lstaged(g, EXA_STAGE_FV, [&] { return exa::launch_lim_fv_update(sd, a, false, s); });
   if (e == hipSuccess) e = lstaged(g, EXA_STAGE_UPDATE, [&] { return exa::launch_lim_update_status(sd, a, s); });
+  const int* irregular = g->flags + exa::kLimFlagIrregular;
+  if (g->fused_ts) {
+    // the cells' prediction with the predictor dt right after their update
+    // and status (the status before spreading; spreading keeps the set of
+    // cells below troubled), then the stability check
+    if (e == hipSuccess) e = lpredict(g, &g->state->dtp, g->st);
+    if (e == hipSuccess)
+      e = lstaged(g, EXA_STAGE_DT, [&] {
+        return exa::launch_dt_finalize_fused_limiting(g->state, g->fused_weight, irregular, 1, s);
+      });
+  }
   if (e == hipSuccess)
     e = lstaged(g, EXA_STAGE_LIMITER, [&] {
       hipError_t r = exa::launch_lim_spread_rollback(sd, a, s);
@@ -234,11 +258,24 @@ hipError_t lstep(exa_lgrid* g) {
         hipLaunchKernelGGL(lim_count_irregular_kernel, dim3(1), dim3(64), 0, s, g->flags);
         r = hipGetLastError();
       }
+      if (r == hipSuccess && g->fused_ts) r = exa::launch_lim_repredict_mask(a, g->st2, s);
       return r;
     });
+  if (!g->fused_ts) {
+    if (e == hipSuccess)
+      e = lstaged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->state, true, s); });
+    if (e == hipSuccess) e = lpredict(g, &g->state->dt, g->st);
+    return e;
+  }
+  // irregular change: the recomputed cells' prediction with the predictor dt
+  // they were rolled back to (recomputePredictorLocally), the repeated check
+  if (e == hipSuccess) e = lpredict(g, &g->state->dtp, g->st2, irregular);
   if (e == hipSuccess)
-    e = lstaged(g, EXA_STAGE_DT, [&] { return exa::launch_dt_finalize(g->state, true, s); });
-  if (e == hipSuccess) e = lpredict(g);
+    e = lstaged(g, EXA_STAGE_DT, [&] {
+      return exa::launch_dt_finalize_fused_limiting(g->state, g->fused_weight, irregular, 2, s);
+    });
+  // a failed check: the prediction again with the new dt
+  if (e == hipSuccess) e = lpredict(g, &g->state->dt, g->st, &g->state->violated);
   return e;
 }
 
@@ -408,6 +445,27 @@ extern "C" int exa_lgrid_init_blast(exa_lgrid* g, unsigned long long seed, doubl
 extern "C" int exa_lgrid_step(exa_lgrid* g, int nsteps) {
   if (!g || nsteps < 0) return EXA_E_INVALID;
   for (int n = 0; n < nsteps; ++n) EXA_TRY(lstep(g));
+  return EXA_OK;
+}
+
+extern "C" int exa_lgrid_set_fused_time_stepping(exa_lgrid* g, int on, double weight) {
+  if (!g) return EXA_E_INVALID;
+  if (on && !(weight > 0.0 && weight <= 1.0)) return EXA_E_INVALID;
+  g->fused_ts = on != 0;
+  if (on) g->fused_weight = weight;
+  // the predictor dt starts as the current dt: set before the first step,
+  // this is the fused start (first dt unweighted, TimeStepSizeComputation.cpp:226-237)
+  EXA_TRY(hipMemcpyAsync(&g->state->dtp, &g->state->dt, sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  return EXA_OK;
+}
+
+extern "C" int exa_lgrid_predictor_reruns(exa_lgrid* g, long* reruns) {
+  if (!g || !reruns) return EXA_E_INVALID;
+  long long r = 0;
+  EXA_TRY(hipMemcpyAsync(&r, &g->state->reruns, sizeof(long long), hipMemcpyDeviceToHost, g->stream));
+  EXA_TRY(hipStreamSynchronize(g->stream));
+  *reruns = (long)r;
   return EXA_OK;
 }
 

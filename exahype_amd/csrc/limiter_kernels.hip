@@ -782,6 +782,19 @@ __global__ __launch_bounds__(256) void lim_spread_kernel(LimArgs a, const int* i
   if (ns > 0) a.has_patch[c] = 1;
 }
 
+// recomputePredictorLocally's selection (LS:1438-1464) after an irregular
+// change in fused time stepping, as a status array for the predictor's gate:
+// 0 for the cells to predict again (status < troubled and the solution
+// recomputed from an FV patch: FV-active neighbours, cells troubled in the
+// previous step), troubled for the rest.
+__global__ __launch_bounds__(256) void lim_repredict_mask_kernel(LimArgs a, int* out) {
+  if (!a.flags[kLimFlagIrregular]) return;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= a.ncells) return;
+  const int s = a.st[c];
+  out[c] = (s < a.T && (s >= a.A || a.pst[c] >= a.T)) ? 0 : a.T;
+}
+
 __global__ void lim_reset_fv_count_kernel(int* flags, int only_if_irregular) {
   if (only_if_irregular && !flags[kLimFlagIrregular]) return;
   if (threadIdx.x == 0) {
@@ -1322,6 +1335,11 @@ hipError_t launch_lim_spread_rollback(const SolverDesc& sd, const LimArgs& a, hi
     hipLaunchKernelGGL(lim_rollback_kernel<N>, dim3((unsigned)(g < a.ncells ? g : a.ncells)), dim3(kLimNT), 0, s, a);
     return hipGetLastError();
   });
+}
+
+hipError_t launch_lim_repredict_mask(const LimArgs& a, int* out, hipStream_t s) {
+  hipLaunchKernelGGL(lim_repredict_mask_kernel, dim3(blocks256(a.ncells)), dim3(256), 0, s, a, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_lim_recompute_finish(const SolverDesc& sd, const LimArgs& a, hipStream_t s) {

@@ -520,6 +520,26 @@ int exa_lgrid_upload(exa_lgrid* g, const double* luh, const double* lim_ic);
 int exa_lgrid_init_blast(exa_lgrid* g, unsigned long long seed, double radius, double p_in,
                          double p_out, double noise);
 int exa_lgrid_step(exa_lgrid* g, int nsteps);
+/* Fused time stepping of the limiting solver (fuse-algorithmic-steps with
+ * weight in (0, 1]; LimitingADERDGSolver::fusedTimeStep, LS:806-836, in
+ * Runner::runOneTimeStepWithFusedAlgorithmicSteps, Runner.cpp:1053-1110).
+ * Each step merges, updates and sets the limiter status as the non-fused
+ * step, then predicts the cells below troubled with the predictor dt and
+ * checks that dt as exa_grid_set_fused_time_stepping does.  After an
+ * irregular limiter-domain change the spreading, rollback and local
+ * recomputation of the non-fused step follow (updateMeshAndSubdomains,
+ * Runner.cpp:866-946); the cells whose solution came from an FV patch are
+ * predicted again with the step's predictor dt (recomputePredictorLocally,
+ * LS:1438-1464), the check is repeated with the recomputed solution's stable
+ * dt s (LocalRecomputation.cpp:168-222), and corrector and predictor dt both
+ * become s (updateTimeStepSizesFused of the adapter's TimeStepSizeComputation,
+ * ADERDGSolver.cpp:594-612).  A failed last check re-runs the prediction with
+ * the new dt.  Switching on takes the current dt as the predictor dt: set
+ * before the first step (after upload or init) this is the fused start, the
+ * unweighted stable dt of the initial state. */
+int exa_lgrid_set_fused_time_stepping(exa_lgrid* g, int on, double weight);
+/* Predictions re-run after a failed stability check. */
+int exa_lgrid_predictor_reruns(exa_lgrid* g, long* reruns);
 int exa_lgrid_synchronize(exa_lgrid* g);
 int exa_lgrid_time(exa_lgrid* g, double* t, double* dt, long* steps);
 int exa_lgrid_download(exa_lgrid* g, double* luh);

@@ -62,6 +62,7 @@ struct StepState {
   double dtp;
   int violated;
   long long reruns;
+  double dtp_checked;  // limiting solver: the predictor dt after the first check
 };
 
 // Arrays marked (bulk) hold double in the kFP64 form and float in kFP32Bulk.
@@ -257,6 +258,14 @@ hipError_t launch_dt_finalize(StepState* st, bool advance_time, hipStream_t s);
 // fused time stepping: the stability check, next predictor dt and
 // startNewTimeStep (init: dt = dtp = the stable dt)
 hipError_t launch_dt_finalize_fused(StepState* st, double weight, bool init, hipStream_t s);
+// fused time stepping of the limiting solver.  Phase 1 is the check after the
+// update; without an irregular limiter-domain change (*irregular == 0) it is
+// the check above, with one it only keeps the checked predictor dt
+// (dtp_checked) for the repeat.  Phase 2 (a no-op without an irregular
+// change) repeats the check with the recomputed dt candidates and sets the
+// corrector and predictor dt to the new stable dt.
+hipError_t launch_dt_finalize_fused_limiting(StepState* st, double weight, const int* irregular, int phase,
+                                             hipStream_t s);
 // Per-patch time steps of a batch: dt[p] = min of the patch's slots (then
 // reset); with advance_time, t[p] += the old dt[p] first.
 hipError_t launch_patch_dt_finalize(double* dt, double* t, unsigned long long* slots, int npatches,

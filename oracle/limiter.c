@@ -578,6 +578,11 @@ struct ora_lgrid {
   double uh2lim[19 * 10], uh2lob[100], lim2uh[19 * 10];
   int have_prediction;
   int last_irregular;
+  /* fused time stepping: dt is the corrector time step size, dtp the
+   * predictor's; violated = the last stability check failed */
+  int fused, violated;
+  double weight, dtp;
+  long reruns;
 };
 
 #define LG_LUH(g, c) ((g)->luh + (c) * (g)->n_luh)
@@ -856,7 +861,12 @@ static void lg_merging(ora_lgrid* g, int recomputation) {
 
 static double lg_dg_dt(ora_lgrid* g, long c) { return ora_stable_time_step_size(&g->s, LG_LUH(g, c), g->dx); }
 
-static void lg_predict(ora_lgrid* g) {
+/* Prediction (Prediction.cpp:177) of the cells with status < troubled
+ * (select 0), or recomputePredictorLocally (LS:1438-1464) after an irregular
+ * change in fused time stepping (select 1): the cells with status < troubled
+ * whose solution was recomputed from an FV patch -- FV-active neighbours
+ * (status >= A) and cells troubled in the previous step. */
+static void lg_predict(ora_lgrid* g, double dt, int select) {
   const ora_solver* s = &g->s;
   const long B = s->order + 1, D = s->nvar + s->npar, V = s->nvar;
   const long B3 = B * B * B, B4 = B3 * B;
@@ -870,12 +880,13 @@ static void lg_predict(ora_lgrid* g) {
     double* lFhi = (double*)malloc(sizeof(double) * B3 * 4 * V);
 #pragma omp for schedule(dynamic, 16)
     for (long c = 0; c < g->ncells; c++) {
+      if (select == 1 && !(g->st[c] < g->T && (g->st[c] >= g->A || g->pst[c] >= g->T) && dt > 0)) continue;
       if (g->st[c] >= g->T) {
         g->iters[c] = 0;
         continue;
       }
       g->iters[c] = ora_space_time_predictor_nonlinear(s, g->lQhbnd + c * g->n_qb, g->lFhbnd + c * g->n_fb, lQi, rhs,
-                                                       lFi, gradQ, lQhi, lFhi, LG_LUH(g, c), g->dx, g->dt);
+                                                       lFi, gradQ, lQhi, lFhi, LG_LUH(g, c), g->dx, dt);
       ora_volume_integral_nonlinear(s, g->lduh + c * g->n_du, lFi, lFhi, g->dx);
     }
     free(lQi);
@@ -961,16 +972,17 @@ void ora_lgrid_set_solution(ora_lgrid* g, const double* luh, const double* lim_i
 #pragma omp parallel for schedule(static) reduction(min : dt)
   for (long c = 0; c < g->ncells; c++) dt = std_min(dt, lg_dg_dt(g, c));
   g->dt = dt;
-  lg_predict(g);
+  g->dtp = dt; /* fused: the first predictor and corrector dt, unweighted */
+  g->violated = 0;
+  g->reruns = 0;
+  lg_predict(g, g->dt, 0);
   g->have_prediction = 1;
 }
 
-/* One time step; returns 1 when it had an irregular limiter-domain change. */
-static int lg_step(ora_lgrid* g) {
+/* Solution update and limiter status of every cell (SolutionUpdate with
+ * LS:843-937); returns 1 on an irregular limiter-domain change. */
+static int lg_update_status(ora_lgrid* g) {
   const int T = g->T, A = g->A, nobs = g->nobs;
-  /* 1. merging */
-  lg_merging(g, 0);
-  /* 2. solution update */
   int irregular = 0;
   /* per cell: reads its own face statuses, writes only its own data */
 #pragma omp parallel for schedule(dynamic, 16) reduction(| : irregular)
@@ -1015,55 +1027,143 @@ static int lg_step(ora_lgrid* g) {
     if (lg_ensure_patch(g, c) && g->st[c] < A) lg_project_fv(g, c);
     for (int f = 0; f < 6; f++) g->fs[c * 6 + f] = 0;
   }
-  /* 3. irregular change: spreading, rollback, local recomputation */
-  if (irregular) {
-    lg_spread(g);
+  return irregular;
+}
+
+/* Irregular change (Runner.cpp:866-946): status spreading, rollback
+ * (reinitialiseSolversLocally), merging and local recomputation with the
+ * new dt candidates of every cell. */
+static void lg_irregular(ora_lgrid* g) {
+  const int T = g->T, A = g->A;
+  lg_spread(g);
 #pragma omp parallel for schedule(static)
-    for (long c = 0; c < g->ncells; c++) { /* reinitialiseSolversLocally */
-      lg_ensure_patch(g, c);
-      if (g->st[c] > 0) {
-        if (g->pst[c] >= A) {
-          lg_swap(LG_LIM(g, c), g->lim_prev + c * g->n_lim, g->n_lim);
-          lg_project_dg(g, c);
-        } else {
-          lg_swap(LG_LUH(g, c), g->luh_prev + c * g->n_luh, g->n_luh);
-          lg_project_fv(g, c);
-        }
-        if (g->st[c] >= T) g->itc[c] = g->cure + 1;
+  for (long c = 0; c < g->ncells; c++) { /* reinitialiseSolversLocally */
+    lg_ensure_patch(g, c);
+    if (g->st[c] > 0) {
+      if (g->pst[c] >= A) {
+        lg_swap(LG_LIM(g, c), g->lim_prev + c * g->n_lim, g->n_lim);
+        lg_project_dg(g, c);
+      } else {
+        lg_swap(LG_LUH(g, c), g->luh_prev + c * g->n_luh, g->n_luh);
+        lg_project_fv(g, c);
       }
-      lg_release_patch(g, c);
+      if (g->st[c] >= T) g->itc[c] = g->cure + 1;
     }
-    lg_merging(g, 1);
-#pragma omp parallel for schedule(dynamic, 16)
-    for (long c = 0; c < g->ncells; c++) { /* recomputeSolutionLocally */
-      const int s = g->st[c];
-      if (s > 0) {
-        if (s >= A) {
-          lg_fv_update(g, c);
-        } else if (g->pst[c] < A) {
-          lg_swap(LG_LUH(g, c), g->luh_prev + c * g->n_luh, g->n_luh);
-          lg_project_fv(g, c);
-        } else {
-          lg_swap(LG_LIM(g, c), g->lim_prev + c * g->n_lim, g->n_lim);
-          lg_project_dg(g, c);
-        }
-      }
-      g->dtc[c] = lg_dg_dt(g, c);
-      if (s < A)
-        lg_solver_min_max(g, c);
-      else
-        lg_limiter_min_max(g, c);
-    }
+    lg_release_patch(g, c);
   }
-  g->last_irregular = irregular;
-  /* 4. new time step size and prediction */
-  g->t += g->dt;
+  lg_merging(g, 1);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long c = 0; c < g->ncells; c++) { /* recomputeSolutionLocally */
+    const int s = g->st[c];
+    if (s > 0) {
+      if (s >= A) {
+        lg_fv_update(g, c);
+      } else if (g->pst[c] < A) {
+        lg_swap(LG_LUH(g, c), g->luh_prev + c * g->n_luh, g->n_luh);
+        lg_project_fv(g, c);
+      } else {
+        lg_swap(LG_LIM(g, c), g->lim_prev + c * g->n_lim, g->n_lim);
+        lg_project_dg(g, c);
+      }
+    }
+    g->dtc[c] = lg_dg_dt(g, c);
+    if (s < A)
+      lg_solver_min_max(g, c);
+    else
+      lg_limiter_min_max(g, c);
+  }
+}
+
+static double lg_min_dt(const ora_lgrid* g) {
   double dt = DBL_MAX;
 #pragma omp parallel for schedule(static) reduction(min : dt)
   for (long c = 0; c < g->ncells; c++) dt = std_min(dt, g->dtc[c]);
-  g->dt = dt;
-  lg_predict(g);
+  return dt;
+}
+
+/* One time step; returns 1 when it had an irregular limiter-domain change. */
+static int lg_step(ora_lgrid* g) {
+  /* 1. merging */
+  lg_merging(g, 0);
+  /* 2. solution update */
+  const int irregular = lg_update_status(g);
+  /* 3. irregular change: spreading, rollback, local recomputation */
+  if (irregular) lg_irregular(g);
+  g->last_irregular = irregular;
+  /* 4. new time step size and prediction */
+  g->t += g->dt;
+  g->dt = lg_min_dt(g);
+  lg_predict(g, g->dt, 0);
   return irregular;
+}
+
+/* One fused time step of the limiting solver (LS:806-836 per cell, the
+ * SolutionUpdate endIteration's stability check TimeStepSizeComputation.cpp:
+ * 167-204, Runner.cpp:1053-1110), in a one-process run: merging, update and
+ * status, prediction of the cells with status < troubled with the predictor
+ * dt, the stability check of that dt against the new stable dt and
+ * startNewTimeStep.  After an irregular change the spreading, rollback
+ * (which takes the time stamps back one step) and local recomputation run
+ * (updateMeshAndSubdomains, Runner.cpp:866-946); the recomputation re-predicts
+ * the cells whose solution came from an FV patch with the predictor dt the
+ * cells were rolled back to (the one this step predicted with), the check is
+ * repeated with the recomputed dt candidates (LocalRecomputation.cpp:168-222),
+ * and the adapter's TimeStepSizeComputation (updateTimeStepSizesFused,
+ * ADERDGSolver.cpp:594-612) then sets the corrector and predictor dt to the
+ * new stable dt.  A failed (last) check re-runs the prediction with the new
+ * dt.  Returns 1 on an irregular change. */
+static int lg_step_fused(ora_lgrid* g) {
+  lg_merging(g, 0);
+  const int irregular = lg_update_status(g);
+  lg_predict(g, g->dtp, 0);
+  const double stable = lg_min_dt(g);
+  double next = stable;
+  double used = g->dtp;
+  if (used == 0.0) used = stable;
+  int violated = used > stable;
+  double dtp_checked = g->dtp;
+  if (violated) {
+    next = fmin(next, g->weight * stable);
+    dtp_checked = g->weight * stable;
+  } else {
+    next = fmin(next, 0.5 * (used + g->weight * stable));
+  }
+  if (!irregular) {
+    g->t += g->dt;
+    g->dt = dtp_checked;
+    g->dtp = next;
+  } else {
+    lg_irregular(g);
+    lg_predict(g, g->dtp, 1);
+    const double stable2 = lg_min_dt(g);
+    double used2 = dtp_checked;
+    if (used2 == 0.0) used2 = stable2;
+    violated = used2 > stable2;
+    g->t += g->dt;
+    g->dt = stable2;
+    g->dtp = stable2;
+  }
+  g->last_irregular = irregular;
+  g->violated = violated;
+  if (violated) {
+    lg_predict(g, g->dt, 0);
+    g->reruns++;
+  }
+  return irregular;
+}
+
+void ora_lgrid_set_fused(ora_lgrid* g, int on, double weight) {
+  g->fused = on;
+  g->weight = weight;
+}
+
+long ora_lgrid_predictor_reruns(const ora_lgrid* g) { return g->reruns; }
+
+/* The cell's current prediction: its face blocks lQhbnd [6][B^2][D] and
+ * lFhbnd [6][B^2][V] (test access). */
+void ora_lgrid_get_prediction(const ora_lgrid* g, long c, double* lQhbnd, double* lFhbnd) {
+  memcpy(lQhbnd, g->lQhbnd + c * g->n_qb, sizeof(double) * g->n_qb);
+  memcpy(lFhbnd, g->lFhbnd + c * g->n_fb, sizeof(double) * g->n_fb);
 }
 
 /* Advance nsteps; status_out[nsteps*ncells] (may be NULL) receives the
@@ -1072,7 +1172,7 @@ static int lg_step(ora_lgrid* g) {
 void ora_lgrid_step(ora_lgrid* g, int nsteps, int* status_out, double* dt_out, int* irregular_out) {
   for (int n = 0; n < nsteps; n++) {
     if (dt_out) dt_out[n] = g->dt;
-    const int irr = lg_step(g);
+    const int irr = g->fused ? lg_step_fused(g) : lg_step(g);
     if (irregular_out) irregular_out[n] = irr;
     if (status_out) memcpy(status_out + (long)n * g->ncells, g->st, sizeof(int) * g->ncells);
   }

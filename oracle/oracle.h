@@ -224,6 +224,11 @@ void ora_lgrid_get_min_max(const ora_lgrid* g, double* mn, double* mx);
 double ora_lgrid_time(const ora_lgrid* g);
 double ora_lgrid_dt(const ora_lgrid* g);
 int ora_lgrid_last_irregular(const ora_lgrid* g);
+/* fused time stepping (fuse-algorithmic-steps) of the limiting solver, weight
+ * = timeStepSizeWeightForPredictionRerun; call before set_solution */
+void ora_lgrid_set_fused(ora_lgrid* g, int on, double weight);
+long ora_lgrid_predictor_reruns(const ora_lgrid* g);
+void ora_lgrid_get_prediction(const ora_lgrid* g, long c, double* lQhbnd, double* lFhbnd);
 
 #ifdef __cplusplus
 }

@@ -153,6 +153,10 @@ def lib():
         L.ora_lgrid_time.restype = D_
         L.ora_lgrid_dt.argtypes = [ctypes.c_void_p]
         L.ora_lgrid_dt.restype = D_
+        L.ora_lgrid_set_fused.argtypes = [ctypes.c_void_p, I, D_]
+        L.ora_lgrid_predictor_reruns.argtypes = [ctypes.c_void_p]
+        L.ora_lgrid_predictor_reruns.restype = ctypes.c_long
+        L.ora_lgrid_get_prediction.argtypes = [ctypes.c_void_p, ctypes.c_long, P, P]
         _lib = L
     return _lib
 
@@ -557,7 +561,8 @@ def fv_boundary_transmissive(L, g, D, patch, face):
 
 
 class LimitingGrid:
-    """Uniform grid with the restated limiting driver (non-fused time stepping)."""
+    """Uniform grid with the restated limiting driver (non-fused time
+    stepping, or fused after set_fused)."""
 
     def __init__(self, s, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), periodic=False, helper_layers=1,
                  steps_till_cured=0, relax=1e-4, scaling=1e-3, fv_scheme=FV_MUSCLHANCOCK):
@@ -615,6 +620,20 @@ class LimitingGrid:
 
     def dt(self):
         return lib().ora_lgrid_dt(self._h)
+
+    def set_fused(self, on, weight=0.99):
+        lib().ora_lgrid_set_fused(self._h, int(on), float(weight))
+
+    def predictor_reruns(self):
+        return lib().ora_lgrid_predictor_reruns(self._h)
+
+    def prediction(self, c):
+        """The cell's face blocks (lQhbnd, lFhbnd) of its current prediction."""
+        B = self.s.order + 1
+        qb = np.zeros(6 * B * B * (self.s.nvar + self.s.npar))
+        fb = np.zeros(6 * B * B * self.s.nvar)
+        lib().ora_lgrid_get_prediction(self._h, c, _p(qb), _p(fb))
+        return qb, fb
 
 
 # ---------------------------------------------------------------- AMR routines

@@ -1,0 +1,100 @@
+"""Fused time stepping of the limiting grid on the device
+(exa_lgrid_set_fused_time_stepping: LimitingADERDGSolver::fusedTimeStep
+LS:806-836 in Runner::runOneTimeStepWithFusedAlgorithmicSteps, Runner.cpp:
+1053-1110) against the oracle's restatement (oracle/limiter.c lg_step_fused).
+
+Bars as for the non-fused limiting grid: the limiter status of every cell
+identical after every step, the same steps with an irregular change, the same
+number of predictor re-runs, dt and time to 1e-13, the solution to 1e-12 of
+each variable's scale.  The blast cases have irregular steps (rollback, local
+recomputation, re-prediction) and regular steps with re-runs.
+"""
+import numpy as np
+import pytest
+
+from refutil import blast_cells, rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(exa, oracle, order, n, periodic, radius, weight):
+    nodes = np.array([oracle.table(oracle.T_NODES, order, i) for i in range(order + 1)])
+    luh = blast_cells(order, n, nodes=nodes, radius=radius)
+    lim_ic = blast_cells(order, n, subcells=True, radius=radius)
+    g = exa.LimitingGrid(exa.euler_solver(order), n,
+                         boundary=exa.BC_PERIODIC if periodic else exa.BC_TRANSMISSIVE)
+    g.upload(luh, lim_ic)
+    g.set_fused_time_stepping(True, weight)
+    o = oracle.LimitingGrid(oracle.make_solver(order), n, periodic=periodic)
+    o.set_fused(True, weight)
+    o.set_solution(luh, lim_ic)
+    return g, o
+
+
+@pytest.mark.parametrize("n,periodic,radius,weight,steps",
+                         [(6, True, 0.3, 1.0, 9), (6, True, 0.3, 0.9, 8), (6, False, 0.2, 0.95, 7)])
+def test_fused_limiting_grid_vs_oracle(exa, oracle, n, periodic, radius, weight, steps):
+    order = 3
+    print(f"case: fused limiting grid vs oracle, N={order} {n}^3 periodic={periodic} r={radius} "
+          f"weight={weight} {steps} steps", flush=True)
+    g, o = _pair(exa, oracle, order, n, periodic, radius, weight)
+    assert np.array_equal(g.status()[0], o.status())
+    irr_prev = 0
+    for k in range(steps):
+        t, dt, _ = g.time()
+        assert abs(dt - o.dt()) <= 1e-13 * o.dt(), f"dt of step {k}"
+        assert abs(t - o.time()) <= 1e-13 * max(o.time(), o.dt()), f"time before step {k}"
+        g.step(1)
+        ost, _, oirr = o.step(1)
+        st, irr = g.status()
+        assert np.array_equal(st, ost[0]), f"limiter status after step {k}: {np.nonzero(st != ost[0])[0][:10]}"
+        assert irr - irr_prev == int(oirr[0]), f"irregular change in step {k}"
+        irr_prev = irr
+        assert g.predictor_reruns() == o.predictor_reruns(), f"re-runs after step {k}"
+        sol, osol = g.solution().reshape(-1, 5), o.solution().reshape(-1, 5)
+        for v in range(5):
+            err = np.abs(sol[:, v] - osol[:, v]).max() / max(np.abs(osol[:, v]).max(), 1e-300)
+            assert err <= 1e-12 or np.abs(sol[:, v] - osol[:, v]).max() <= 1e-14, f"var {v} step {k}: {err:.3e}"
+    print(f"  irregular steps {irr_prev}, re-runs {o.predictor_reruns()}", flush=True)
+    assert irr_prev > 0
+    if weight == 1.0:
+        assert o.predictor_reruns() > 0
+    S = 2 * order + 1 + 4
+    inner = (slice(2, S - 2),) * 3
+    for c in np.nonzero(st >= 2)[0][:8]:
+        a = g.limiter_solution(int(c)).reshape(S, S, S, 5)[inner]
+        b = o.limiter_solution(int(c)).reshape(S, S, S, 5)[inner]
+        assert rel_max(a, b) <= 1e-11
+    g.close()
+
+
+def test_fused_limiting_without_troubled_cells_equals_fused_plain_grid(exa):
+    """A smooth flow with a loose DMP never flags a cell: the fused limiting
+    grid then equals the fused plain grid (same dt sequence and solution)."""
+    order, n, steps = 3, 5, 6
+    print(f"case: fused limiting grid without troubled cells vs fused plain grid N={order} {n}^3", flush=True)
+    lg = exa.LimitingGrid(exa.euler_solver(order), n, boundary=exa.BC_PERIODIC, relax=1.0, scaling=1.0)
+    pg = exa.Grid(exa.euler_solver(order), n)
+    pg.init_euler(nmodes=6, mode_amp=0.03, seed=5)
+    luh = pg.solution()
+    lg.upload(luh)
+    lg.set_fused_time_stepping(True, 0.9)
+    pg.set_fused_time_stepping(True, 0.9)
+    lg.step(steps)
+    pg.step(steps)
+    st, irr = lg.status()
+    assert (st == 0).all() and irr == 0
+    assert abs(lg.time()[0] - pg.time()[0]) <= 1e-13 * pg.time()[0]
+    assert rel_max(lg.solution(), pg.solution()) <= 1e-12
+    assert lg.predictor_reruns() == pg.predictor_reruns()
+    lg.close()
+    pg.close()
+
+
+def test_fused_limiting_refusals(exa):
+    L = exa.lib()
+    g = exa.LimitingGrid(exa.euler_solver(3), 3)
+    assert L.exa_lgrid_set_fused_time_stepping(g._h, 1, 0.0) == exa.EXA_E_INVALID
+    assert L.exa_lgrid_set_fused_time_stepping(g._h, 1, 1.01) == exa.EXA_E_INVALID
+    assert L.exa_lgrid_set_fused_time_stepping(None, 1, 0.9) == exa.EXA_E_INVALID
+    g.close()

@@ -1,4 +1,5 @@
-"""Legacy VTK output of ADER-DG grids: the Cartesian vertex plotter.
+"""Legacy VTK output of ADER-DG grids: the Cartesian and Legendre vertex
+plotters.
 
 Mirrors the reference's ``vtk::Cartesian::vertices`` plotters
 (``exahype/plotters/VTK/ADERDG2CartesianVTK.cpp:288-336``, ``:398-420``):
@@ -8,6 +9,11 @@ vertices ``offset + i h / N``, carrying the interpolated values as vertex data
 stamp.  The interpolation runs on the device (``exa_plot_vertices`` /
 ``exa_grid_plot_vertices``); this module only writes the file.
 
+The Legendre vertex plotter (``vtk::Legendre::vertices``,
+``exahype/plotters/VTK/ADERDG2LegendreVTK.cpp:278-385``) writes the stored
+nodal values themselves at the Gauss-Legendre nodes ``offset + x_i h``, with
+the same N^3 hexahedra per cell between neighbouring nodes.
+
 Each patch owns its vertices (vertices on cell faces appear once per cell),
 as the reference's patch writer does.
 """
@@ -16,10 +22,11 @@ import numpy as np
 VTK_HEXAHEDRON = 12
 
 
-def patch_geometry(order, n, lo, hi):
+def patch_geometry(order, n, lo, hi, nodes=None):
     """Points [ncells * (N+1)^3, 3] and hexahedra [ncells * N^3, 8] of a uniform
     n = (nx, ny, nz) grid on [lo, hi], cells in grid order (x fastest) and each
-    cell's vertices (i2, i1, i0) with x fastest."""
+    cell's vertices (i2, i1, i0) with x fastest.  The vertices sit at the
+    reference-cell positions i/N, or at nodes[i] when given."""
     B = order + 1
     n = np.asarray(n)
     lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
@@ -27,7 +34,8 @@ def patch_geometry(order, n, lo, hi):
     c = np.arange(int(np.prod(n)))
     cell_lo = lo + h * np.stack([c % n[0], (c // n[0]) % n[1], c // (n[0] * n[1])], -1)
     i = np.arange(B ** 3)
-    frac = np.stack([i % B, (i // B) % B, i // (B * B)], -1) / order
+    idx = np.stack([i % B, (i // B) % B, i // (B * B)], -1)
+    frac = idx / order if nodes is None else np.asarray(nodes, dtype=np.float64)[idx]
     points = (cell_lo[:, None, :] + frac[None, :, :] * h[None, None, :]).reshape(-1, 3)
     # hexahedra of one patch: sub-cell (a, b, c) -> its 8 corners in VTK order
     s = np.arange(order)
@@ -47,7 +55,24 @@ def write_cartesian_vertices(path, vertices, order, n, lo=(0.0, 0.0, 0.0), hi=(1
     as exa_plot_vertices returns them."""
     if order < 1:
         raise ValueError("the Cartesian vertex plotter needs order >= 1")
-    points, cells = patch_geometry(order, n, lo, hi)
+    return _write(path, vertices, order, n, lo, hi, time, binary, title, None)
+
+
+def write_legendre_vertices(path, luh, order, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), time=0.0,
+                            binary=True, title="exahype_amd ADER-DG Legendre vertices", nodes=None):
+    """Write the stored solution luh [ncells * (N+1)^3 * D] (as exa_grid
+    downloads it) at the Gauss-Legendre nodes.  nodes defaults to the
+    library's quadrature nodes (exa_quadrature_node)."""
+    if order < 1:
+        raise ValueError("the Legendre vertex plotter needs order >= 1")
+    if nodes is None:
+        from . import capi
+        nodes = [capi.lib().exa_quadrature_node(order, i) for i in range(order + 1)]
+    return _write(path, luh, order, n, lo, hi, time, binary, title, nodes)
+
+
+def _write(path, vertices, order, n, lo, hi, time, binary, title, nodes):
+    points, cells = patch_geometry(order, n, lo, hi, nodes)
     npts = len(points)
     q = np.asarray(vertices, dtype=np.float64).reshape(npts, -1)
     D = q.shape[1]

@@ -71,3 +71,31 @@ def test_vtk_writer_round_trip(tmp_path, oracle):
     assert np.allclose(pts[B ** 3], np.array(lo) + np.array([0.5, 0.0, 0.0]))
     binp = vtk.write_cartesian_vertices(str(tmp_path / "b.vtk"), q, order, n, lo, hi, binary=True)
     assert open(binp, "rb").read().startswith(b"# vtk DataFile Version 3.0")
+
+
+def test_legendre_vertices_writer(tmp_path):
+    """The Legendre vertex plotter (ADERDG2LegendreVTK.cpp:278-385): the
+    stored nodal values themselves at offset + x_i h, x_i the Gauss-Legendre
+    nodes (numpy's rule, independent of the library's table), N^3 hexahedra
+    per cell between neighbouring nodes."""
+    from refutil import gauss_legendre
+    order, n, lo, hi = 3, (2, 3, 1), (0.0, -1.0, 0.5), (1.0, 0.5, 2.0)
+    B, D = order + 1, 5
+    ncells = 6
+    luh = np.random.default_rng(2).standard_normal(ncells * B ** 3 * D)
+    path = vtk.write_legendre_vertices(str(tmp_path / "l.vtk"), luh, order, n, lo, hi, time=0.5, binary=False)
+    pts, cells, q = vtk.read_ascii(path)
+    assert np.array_equal(q, luh.reshape(-1, D))
+    x, _ = gauss_legendre(B)
+    h = (np.array(hi) - np.array(lo)) / np.array(n)
+    for c in (0, 1, 5):
+        ci = np.array([c % n[0], (c // n[0]) % n[1], c // (n[0] * n[1])])
+        for node in (0, 1, B, B * B + 2, B ** 3 - 1):
+            idx = np.array([node % B, (node // B) % B, node // (B * B)])
+            assert np.allclose(pts[c * B ** 3 + node], np.array(lo) + (ci + x[idx]) * h, rtol=0, atol=1e-14)
+    assert cells.shape == (ncells * order ** 3, 8)
+    for hexa in cells[:order ** 3]:
+        c = pts[hexa]
+        assert (c[6] > c[0]).all()                     # opposite corners
+        assert np.allclose(c[1, 1:], c[0, 1:]) and c[1, 0] > c[0, 0]
+        assert np.allclose(c[4, :2], c[0, :2]) and c[4, 2] > c[0, 2]

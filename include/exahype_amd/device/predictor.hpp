@@ -115,8 +115,18 @@ struct PredictorLayout {
 //     into the next iterate through iK1 (the reference's n order), so rhs is
 //     never stored for all t.
 // ---------------------------------------------------------------------------
+// Waves per SIMD the register allocation must leave room for: the float
+// flux-only form at N=5 accumulates its time-averaged fluxes during the last
+// iteration (3V more live registers) within the four waves that its LDS
+// allows anyway; elsewhere no constraint.
+template <int B, int TERMS, typename T>
+__host__ __device__ constexpr int predictor_min_waves() {
+  return (sizeof(T) == sizeof(float) && TERMS == kFlux && B == 6) ? 4 : 1;
+}
+
 template <int B, class PDE, int TERMS, bool EXPORT, typename T, int LDSK = 144>
-__global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArgs a) {
+__global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>())) void predictor_kernel(
+    PredictorArgs a) {
   constexpr int V = PDE::V, P = PDE::P, D = V + P;
   constexpr int N = B - 1;
   constexpr int B2 = B * B, B3 = B2 * B;
@@ -215,10 +225,12 @@ __global__ __launch_bounds__(node_threads(B)) void predictor_kernel(PredictorArg
   // and the time-averaged fluxes accumulated while the iterations run (see
   // below); the double form keeps the reference's order of operations.
   constexpr bool FLOAT_FLUX_ONLY = !IS_FP64 && USE_FLUX && !USE_SRC_NCP;
-  // accumulating the averages needs 3V more registers: only at B = 5, where
-  // they fit the budget of four wavefronts per SIMD (measured register counts:
-  // B = 3, 4 and 6 would drop a wavefront per SIMD, B = 7 would spill)
-  constexpr bool ACC_FH = FLOAT_FLUX_ONLY && B == 5;
+  // accumulating the averages needs 3V more registers: only at B = 5 and 6,
+  // where they fit the budget of four wavefronts per SIMD (B = 6 with the
+  // register cap of predictor_min_waves: 124 registers; the epilogue's B flux
+  // evaluations go, 4% of the predictor at 81^3; B = 3 and 4 would drop a
+  // wavefront per SIMD, B = 7 would spill)
+  constexpr bool ACC_FH = FLOAT_FLUX_ONLY && (B == 5 || B == 6);
   T fh[3][V];
   const T iweight = T(1.0) / w3;
   __syncthreads();

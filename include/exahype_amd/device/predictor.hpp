@@ -232,6 +232,13 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
   // wavefront per SIMD, B = 7 would spill)
   constexpr bool ACC_FH = FLOAT_FLUX_ONLY && (B == 5 || B == 6);
   T fh[3][V];
+  // Float form of a PDE without fluxes (source and/or NCP only: the elastic
+  // PDE): the time-averaged source term likewise accumulates during the
+  // iterations, from the S each slice's phase 3 evaluates anyway, so the
+  // epilogue re-evaluates no slice (V more registers; the elastic kernels keep
+  // their occupancy).
+  constexpr bool ACC_SH = USE_SRC_NCP && !USE_FLUX;
+  T sh[V];
   const T iweight = T(1.0) / w3;
   __syncthreads();
 
@@ -310,6 +317,10 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
 #pragma unroll
         for (int m = 0; m < V; ++m) fh[d][m] = T(0);
     }
+    if constexpr (ACC_SH) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) sh[m] = T(0);
+    }
     // Rolled (large sizes): table entries come from constant memory with a
     // uniform index; unrolled: they are immediates.
     // One time slice; with SPLIT (rolled loops over slices that live partly
@@ -387,6 +398,11 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
           const T us = wt * dt;
 #pragma unroll
           for (int m = 0; m < V; ++m) ir[m] += us * S[m];
+          if constexpr (ACC_SH) {
+            // time average of the slice sources of this iterate (:367-396)
+#pragma unroll
+            for (int m = 0; m < V; ++m) sh[m] += S[m] * wt;
+          }
         }
 #pragma unroll
         for (int m = 0; m < V; ++m)
@@ -411,6 +427,10 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
         const T us = wt * wz * wy * wx * dt;
 #pragma unroll
         for (int m = 0; m < V; ++m) r[m] += us * S[m];
+        if constexpr (ACC_SH) {
+#pragma unroll
+          for (int m = 0; m < V; ++m) sh[m] += S[m] * wt;
+        }
         if (EXPORT) {
           double* lFi = a.lFi + cell * (size_t)(4 * B3 * B * V) + (((size_t)t * B3 + node) * 4 + 3) * V;
 #pragma unroll
@@ -579,7 +599,6 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
   // Time-averaged fluxes of the last flux evaluation, i.e. of lQi^k = qi (the
   // reference computes lFi before the final lQi update, :96-122 vs :306):
   // lFhi = sum_t w_t F(lQi^k(t)), lShi likewise (:367-396).
-  T sh[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) {
     if constexpr (!ACC_FH) {
@@ -587,7 +606,7 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
       fh[1][m] = T(0);
       fh[2][m] = T(0);
     }
-    sh[m] = T(0);
+    if constexpr (!ACC_SH) sh[m] = T(0);
   }
   auto average_slice = [&](const int t) __device__ __attribute__((always_inline)) {
     T Q[D];
@@ -617,8 +636,9 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
       if (USE_NCP) __syncthreads();
     }
   };
-  if constexpr (ACC_FH) {
-    // fh holds the time average of the last iteration's fluxes already
+  if constexpr (ACC_FH || ACC_SH) {
+    // fh (flux-only) or sh (no flux) holds the time average of the last
+    // iteration's fluxes or sources already
   } else if constexpr (SPLIT) {
 #pragma unroll 1
     for (int t = 0; t < NQL; ++t) average_slice(t);

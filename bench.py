@@ -65,13 +65,14 @@ def predictor_executed_flops_fp32(B, V, picard_total, cells):
     iteration evaluates one flux and one slice's Kxi contractions per node
     (c_F + 6 V B) and forms lQi^1 in closed form and its residual (8 V B + 5 V);
     later iterations are full; the epilogue adds the once-per-cell terms and,
-    except at B = 5 and 6 where the flux averages are accumulated during the
-    iterations, re-evaluates the B slice fluxes of the last iterate (c_F B^4)."""
+    except at B = 5, 6 and 8 where the flux averages are accumulated during
+    the iterations, re-evaluates the B slice fluxes of the last iterate
+    (c_F B^4)."""
     c_f = EULER_FLUX_FLOPS
     first = B ** 3 * (c_f + 6 * V * B + 8 * V * B + 5 * V)
     per_iter = 8 * V * B ** 5 + (c_f + 4 * V) * B ** 4
     fixed = 2 * (3 * V + V) * B ** 4 + 12 * (V + V) * B ** 3 + 6 * V * B ** 4
-    recompute = 0 if B in (5, 6) else c_f * B ** 4
+    recompute = 0 if B in (5, 6, 8) else c_f * B ** 4
     return cells * first + (picard_total - cells) * per_iter + cells * (fixed + recompute)
 
 

@@ -252,10 +252,15 @@ void predictor_mfma_kernel(PredictorArgs a) {
     for (int m = 0; m < V; ++m) sQi[(t * V + m) * B3 + node] = q0[m];
   v4f qn[V][G];
   int iters = MAX_ITER;
+  // From B = 7 (two or three wavefronts per SIMD, set by LDS) the registers
+  // have room to accumulate the time-averaged fluxes of each iteration as its
+  // slices are evaluated, so the epilogue evaluates no slice again.
+  constexpr bool ACC_FH = B >= 7;
+  float fh[3][V];
   __syncthreads();
 
-  // phase 1: fluxes of Q -> sF[d][var][node]
-  auto stage_flux = [&](const float* Qv) __device__ __attribute__((always_inline)) {
+  // phase 1: fluxes of Q -> sF[d][var][node]; with ACC_FH also fh += wt F
+  auto stage_flux = [&](const float* Qv, float wt) __device__ __attribute__((always_inline)) {
     float Q[D];
 #pragma unroll
     for (int m = 0; m < V; ++m) Q[m] = Qv[m];
@@ -266,7 +271,10 @@ void predictor_mfma_kernel(PredictorArgs a) {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int m = 0; m < V; ++m) sF[(d * V + m) * B3 + node] = F[d][m];
+      for (int m = 0; m < V; ++m) {
+        sF[(d * V + m) * B3 + node] = F[d][m];
+        if constexpr (ACC_FH) fh[d][m] += F[d][m] * wt;
+      }
   };
 
 #pragma unroll 1
@@ -280,13 +288,24 @@ void predictor_mfma_kernel(PredictorArgs a) {
       q0i[m] = q0[m];
       asm volatile("" : "+v"(q0i[m]));
     }
+    if constexpr (ACC_FH) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < V; ++m) fh[d][m] = 0.f;
+    }
     if (iter == 0) {
       // First iteration: every slice of the initial iterate equals luh, so the
       // slices share one contraction S = sum_d C'_d and
       // lQi^1(l) = (sum_t iK1[l][t] FLCoeff[t]) luh - (sum_t iK1[l][t] w_t) S,
       // coefficients formed in double from the tables (packed: slot g holds
       // rows 4g.. of the first sum, slot G + g of the second, negated).
-      stage_flux(q0i);
+      {
+        // every slice's flux: the average is sum_t w_t F(luh)
+        double sw = 0.0;
+        for (int t = 0; t < B; ++t) sw += exa_gl_weights[N][t];
+        stage_flux(q0i, (float)sw);
+      }
       __syncthreads();
       contract_lines_mfma<B, V, NW, 1>(sF, cK[0], cK[1], cK[2], lane, wave);
       __syncthreads();
@@ -318,7 +337,7 @@ void predictor_mfma_kernel(PredictorArgs a) {
         float Q[V];
 #pragma unroll
         for (int m = 0; m < V; ++m) Q[m] = sQi[(t * V + m) * B3 + node];
-        stage_flux(Q);
+        stage_flux(Q, (float)exa_gl_weights[N][t]);
         __syncthreads();
         contract_lines_mfma<B, V, NW, 1>(sF, cK[0], cK[1], cK[2], lane, wave);
         __syncthreads();
@@ -399,11 +418,12 @@ void predictor_mfma_kernel(PredictorArgs a) {
   // Time-averaged fluxes of the last flux evaluation, i.e. of lQi^k (the
   // reference computes lFi before the final lQi update, :96-122 vs :306):
   // lFhi = sum_t w_t F(lQi^k(t)) (:367-396)
-  float fh[3][V];
+  if constexpr (!ACC_FH) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-    for (int m = 0; m < V; ++m) fh[d][m] = 0.f;
+      for (int m = 0; m < V; ++m) fh[d][m] = 0.f;
+  }
   auto average_slice = [&](const int t) __device__ __attribute__((always_inline)) {
     float Q[D];
 #pragma unroll
@@ -418,7 +438,9 @@ void predictor_mfma_kernel(PredictorArgs a) {
 #pragma unroll
       for (int m = 0; m < V; ++m) fh[d][m] += F[d][m] * wt;
   };
-  if constexpr (ROLLED) {
+  if constexpr (ACC_FH) {
+    // fh holds the time average of the last iteration's fluxes already
+  } else if constexpr (ROLLED) {
 #pragma unroll 1
     for (int t = 0; t < B; ++t) average_slice(t);
   } else {

@@ -116,12 +116,13 @@ struct PredictorLayout {
 //     never stored for all t.
 // ---------------------------------------------------------------------------
 // Waves per SIMD the register allocation must leave room for: the float
-// flux-only form at N=5 accumulates its time-averaged fluxes during the last
-// iteration (3V more live registers) within the four waves that its LDS
-// allows anyway; elsewhere no constraint.
+// flux-only form at N=5 and N=7 accumulates its time-averaged fluxes during
+// the last iteration (3V more live registers) within the four (N=5) or three
+// (N=7, 64 KB layout) waves that its LDS allows anyway; elsewhere no
+// constraint.
 template <int B, int TERMS, typename T>
 __host__ __device__ constexpr int predictor_min_waves() {
-  return (sizeof(T) == sizeof(float) && TERMS == kFlux && B == 6) ? 4 : 1;
+  return (sizeof(T) == sizeof(float) && TERMS == kFlux) ? (B == 6 ? 4 : (B == 8 ? 3 : 1)) : 1;
 }
 
 template <int B, class PDE, int TERMS, bool EXPORT, typename T, int LDSK = 144>
@@ -225,12 +226,12 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
   // and the time-averaged fluxes accumulated while the iterations run (see
   // below); the double form keeps the reference's order of operations.
   constexpr bool FLOAT_FLUX_ONLY = !IS_FP64 && USE_FLUX && !USE_SRC_NCP;
-  // accumulating the averages needs 3V more registers: only at B = 5 and 6,
-  // where they fit the budget of four wavefronts per SIMD (B = 6 with the
-  // register cap of predictor_min_waves: 124 registers; the epilogue's B flux
-  // evaluations go, 4% of the predictor at 81^3; B = 3 and 4 would drop a
-  // wavefront per SIMD, B = 7 would spill)
-  constexpr bool ACC_FH = FLOAT_FLUX_ONLY && (B == 5 || B == 6);
+  // accumulating the averages needs 3V more registers: only at B = 5, 6 and
+  // 8, where they fit the wavefronts per SIMD that LDS allows (B = 6 and 8
+  // with the register cap of predictor_min_waves: 124 and 166 registers; the
+  // epilogue's B flux evaluations go, 4% of the predictor at 81^3 N=5; B = 3
+  // and 4 would drop a wavefront per SIMD, B = 7 spills under the cap)
+  constexpr bool ACC_FH = FLOAT_FLUX_ONLY && (B == 5 || B == 6 || B == 8);
   T fh[3][V];
   // Float form of a PDE without fluxes (source and/or NCP only: the elastic
   // PDE): the time-averaged source term likewise accumulates during the

@@ -98,3 +98,37 @@ def test_fused_limiting_refusals(exa):
     assert L.exa_lgrid_set_fused_time_stepping(g._h, 1, 1.01) == exa.EXA_E_INVALID
     assert L.exa_lgrid_set_fused_time_stepping(None, 1, 0.9) == exa.EXA_E_INVALID
     g.close()
+
+
+def test_fused_limiting_full_size_invariants(exa):
+    """The limiter workload at its real shape (N=3, 81^3, transmissive blast)
+    with fused time stepping: finite and positive, neighbouring statuses
+    within one, the time stamp the sum of the corrector dts, and the same
+    limited region as the non-fused run after the same steps to within a few
+    cells (the dts differ slightly)."""
+    print("case: fused limiting 81^3 N=3, 5 steps", flush=True)
+    n, steps = 81, 5
+    runs = []
+    for fused in (False, True):
+        g = exa.LimitingGrid(exa.euler_solver(3), n)
+        g.init_blast(seed=7, radius=0.2)
+        if fused:
+            g.set_fused_time_stepping(True, 0.9)
+        total = 0.0
+        for _ in range(steps):
+            total += g.time()[1]
+            g.step(1)
+        t, _, nsteps = g.time()
+        st, irr = g.status()
+        sol = g.solution().reshape(-1, 5)
+        assert nsteps == steps and abs(t - total) <= 1e-13 * total
+        assert np.isfinite(sol).all() and sol[:, 0].min() > 0
+        S3 = st.reshape(n, n, n)
+        for ax in range(3):
+            assert np.abs(np.diff(S3, axis=ax)).max() <= 1
+        runs.append(((st >= 2).mean(), irr, g.predictor_reruns()))
+        g.close()
+    print(f"  FV-active fraction non-fused {runs[0][0]:.4f} fused {runs[1][0]:.4f}, irregular steps "
+          f"{runs[0][1]} / {runs[1][1]}, re-runs {runs[1][2]}", flush=True)
+    assert 0.0 < runs[1][0] < 0.5
+    assert abs(runs[1][0] - runs[0][0]) < 0.01

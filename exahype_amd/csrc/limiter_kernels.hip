@@ -1276,16 +1276,17 @@ hipError_t launch_lim_minmax_merge(const SolverDesc& sd, const LimArgs& a, const
   return hipGetLastError();
 }
 
-static unsigned persistent_grid(long ncells) {
+static unsigned persistent_grid(long ncells, long per_cu = 2) {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const long g = 2L * cus;
+  const long g = per_cu * cus;
   return (unsigned)(g < ncells ? g : (ncells > 0 ? ncells : 1));
 }
 
 hipError_t launch_lim_ghosts(const SolverDesc& sd, const LimArgs& a, bool only_if_irregular, hipStream_t s) {
   return dispatch_lim(sd.order, [&]<int N>() -> hipError_t {
-    hipLaunchKernelGGL(lim_ghost_kernel<N>, dim3(persistent_grid(a.ncells)), dim3(kLimNT), 0, s, a,
+    // a pure gather (no LDS): eight workgroups per CU keep more of it in flight
+    hipLaunchKernelGGL(lim_ghost_kernel<N>, dim3(persistent_grid(a.ncells, 8)), dim3(kLimNT), 0, s, a,
                        only_if_irregular ? 1 : 0);
     return hipGetLastError();
   });

@@ -132,3 +132,32 @@ def test_fused_limiting_full_size_invariants(exa):
           f"{runs[0][1]} / {runs[1][1]}, re-runs {runs[1][2]}", flush=True)
     assert 0.0 < runs[1][0] < 0.5
     assert abs(runs[1][0] - runs[0][0]) < 0.01
+
+
+@pytest.mark.timeout(600)
+def test_fused_limiting_grid_vs_oracle_mid_size(exa, oracle):
+    """Statuses bit-exact against the oracle's fused driver on a 16^3
+    transmissive blast over 20 fused steps (N=3), with the dt of every step,
+    the irregular steps and re-runs, and the solution at the end."""
+    order, n, steps, weight = 3, 16, 20, 0.9
+    print(f"case: fused limiting grid vs oracle, N={order} {n}^3 blast {steps} steps", flush=True)
+    g, o = _pair(exa, oracle, order, n, False, 0.1, weight)
+    assert np.array_equal(g.status()[0], o.status())
+    irr_prev = 0
+    for k in range(steps):
+        assert abs(g.time()[1] - o.dt()) <= 1e-13 * o.dt(), f"dt of step {k}"
+        g.step(1)
+        ost, _, oirr = o.step(1)
+        st, irr = g.status()
+        assert np.array_equal(st, ost[0]), f"limiter status after step {k}: {np.nonzero(st != ost[0])[0][:10]}"
+        assert irr - irr_prev == int(oirr[0]), f"irregular change in step {k}"
+        irr_prev = irr
+        assert g.predictor_reruns() == o.predictor_reruns(), f"re-runs after step {k}"
+        if k % 5 == 4:
+            print(f"  step {k + 1}: statuses identical", flush=True)
+    sol, osol = g.solution().reshape(-1, 5), o.solution().reshape(-1, 5)
+    for v in range(5):
+        assert rel_max(sol[:, v], osol[:, v]) <= 1e-11, v
+    print(f"  FV-active {(st >= 2).mean():.3f}, irregular steps {irr_prev}, re-runs {o.predictor_reruns()}", flush=True)
+    assert irr_prev > 0 and (st >= 2).any()
+    g.close()

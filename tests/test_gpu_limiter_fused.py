@@ -17,27 +17,30 @@ from refutil import blast_cells, rel_max
 pytestmark = pytest.mark.gpu
 
 
-def _pair(exa, oracle, order, n, periodic, radius, weight):
+def _pair(exa, oracle, order, n, periodic, radius, weight, helper_layers=1, cure=0):
     nodes = np.array([oracle.table(oracle.T_NODES, order, i) for i in range(order + 1)])
     luh = blast_cells(order, n, nodes=nodes, radius=radius)
     lim_ic = blast_cells(order, n, subcells=True, radius=radius)
     g = exa.LimitingGrid(exa.euler_solver(order), n,
-                         boundary=exa.BC_PERIODIC if periodic else exa.BC_TRANSMISSIVE)
+                         boundary=exa.BC_PERIODIC if periodic else exa.BC_TRANSMISSIVE,
+                         helper_layers=helper_layers, steps_till_cured=cure)
     g.upload(luh, lim_ic)
     g.set_fused_time_stepping(True, weight)
-    o = oracle.LimitingGrid(oracle.make_solver(order), n, periodic=periodic)
+    o = oracle.LimitingGrid(oracle.make_solver(order), n, periodic=periodic, helper_layers=helper_layers,
+                            steps_till_cured=cure)
     o.set_fused(True, weight)
     o.set_solution(luh, lim_ic)
     return g, o
 
 
-@pytest.mark.parametrize("n,periodic,radius,weight,steps",
-                         [(6, True, 0.3, 1.0, 9), (6, True, 0.3, 0.9, 8), (6, False, 0.2, 0.95, 7)])
-def test_fused_limiting_grid_vs_oracle(exa, oracle, n, periodic, radius, weight, steps):
+@pytest.mark.parametrize("n,periodic,radius,weight,steps,helper_layers,cure",
+                         [(6, True, 0.3, 1.0, 9, 1, 0), (6, True, 0.3, 0.9, 8, 1, 0), (6, False, 0.2, 0.95, 7, 1, 0),
+                          (7, True, 0.3, 1.0, 8, 2, 1)])
+def test_fused_limiting_grid_vs_oracle(exa, oracle, n, periodic, radius, weight, steps, helper_layers, cure):
     order = 3
     print(f"case: fused limiting grid vs oracle, N={order} {n}^3 periodic={periodic} r={radius} "
-          f"weight={weight} {steps} steps", flush=True)
-    g, o = _pair(exa, oracle, order, n, periodic, radius, weight)
+          f"weight={weight} {steps} steps, helper layers {helper_layers}, cure {cure}", flush=True)
+    g, o = _pair(exa, oracle, order, n, periodic, radius, weight, helper_layers, cure)
     assert np.array_equal(g.status()[0], o.status())
     irr_prev = 0
     for k in range(steps):
@@ -61,7 +64,7 @@ def test_fused_limiting_grid_vs_oracle(exa, oracle, n, periodic, radius, weight,
         assert o.predictor_reruns() > 0
     S = 2 * order + 1 + 4
     inner = (slice(2, S - 2),) * 3
-    for c in np.nonzero(st >= 2)[0][:8]:
+    for c in np.nonzero(st >= helper_layers + 1)[0][:8]:
         a = g.limiter_solution(int(c)).reshape(S, S, S, 5)[inner]
         b = o.limiter_solution(int(c)).reshape(S, S, S, 5)[inner]
         assert rel_max(a, b) <= 1e-11

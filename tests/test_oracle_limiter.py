@@ -391,3 +391,52 @@ def test_limiter_status_machine_injected_cell(oracle, helper_layers, cure):
     assert (g.status() == 0).all()
     sol = g.solution().reshape(-1, 5)
     assert np.abs(sol[:, 0] - 1.0).max() < 1e-12 and np.abs(sol[:, 1:4]).max() < 1e-12
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_irregular_steps_leave_quiet_cells_as_plain_dg(oracle, periodic):
+    """Rollback and local recomputation touch only the cells with a status
+    above 0 after the spreading (LS:1343-1436): a cell with status 0 before
+    and after a step, whose face neighbours all carried a prediction (status
+    below troubled) before it, ends the step -- regular or irregular -- with
+    exactly the solution the plain ADER-DG driver computes from the same
+    state and dt (bit for bit: the same kernels on the same data).  A rollback
+    or recomputation that also reached such cells, or a merge that used other
+    face data, fails it."""
+    order, n = 3, 8
+    s = oracle.make_solver(order)
+    B = order + 1
+    nodes = np.array([oracle.table(oracle.T_NODES, order, i) for i in range(B)])
+    luh = blast_cells(order, n, nodes=nodes, radius=0.2)
+    lim_ic = blast_cells(order, n, subcells=True, radius=0.2)
+    grid = oracle.LimitingGrid(s, n, periodic=periodic)
+    grid.set_solution(luh, lim_ic)
+    T = 3
+    idx = np.arange(n ** 3).reshape(n, n, n)  # [k][j][i]
+    checked_irregular = 0
+    counts = []
+    for step in range(5):
+        luh0, st0, dt = grid.solution(), grid.status(), grid.dt()
+        st, _, irr = grid.step(1)
+        st1, luh1 = st[0], grid.solution().reshape(n ** 3, -1)
+        pg = oracle.Grid(s, n, transmissive=not periodic)
+        pg.set_solution(luh0)
+        pg.step_dts(np.array([dt]))
+        ref = pg.solution().reshape(n ** 3, -1)
+        S0 = st0.reshape(n, n, n)
+        quiet = np.ones((n, n, n), dtype=bool)
+        for ax in range(3):
+            for sh in (1, -1):
+                nb = np.roll(S0, sh, axis=ax)
+                if not periodic:  # a missing neighbour does not disqualify the cell
+                    edge = [slice(None)] * 3
+                    edge[ax] = 0 if sh == 1 else -1
+                    nb = nb.copy()
+                    nb[tuple(edge)] = 0
+                quiet &= nb < T
+        cells = idx[quiet & (S0 == 0) & (st1.reshape(n, n, n) == 0)]
+        assert len(cells) > 0
+        assert np.array_equal(luh1[cells], ref[cells]), f"step {step}: {len(cells)} cells"
+        counts.append(len(cells))
+        checked_irregular += int(irr[0])
+    assert checked_irregular > 0

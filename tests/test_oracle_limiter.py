@@ -440,3 +440,70 @@ def test_irregular_steps_leave_quiet_cells_as_plain_dg(oracle, periodic):
         counts.append(len(cells))
         checked_irregular += int(irr[0])
     assert checked_irregular > 0
+
+
+def test_fv_cells_step_from_the_previous_level(oracle):
+    """Every cell FV-active before and after a step (regular or irregular,
+    i.e. also after rollback and local recomputation) holds the MUSCL-Hancock
+    update (pinned separately) of its previous patch, with its face ghost
+    layers filled (pinned separately) from the neighbours' previous FV
+    representations -- their previous patch when FV-active, else the
+    projection of their previous DG solution.  A recomputation from the
+    wrong time level, or with ghosts from the updated neighbours, fails it."""
+    order, n = 3, 8
+    s = oracle.make_solver(order)
+    B, L, g, D, A = order + 1, 2 * order + 1, 2, 5, 2
+    S = L + 2 * g
+    nodes = np.array([oracle.table(oracle.T_NODES, order, i) for i in range(B)])
+    luh = blast_cells(order, n, nodes=nodes, radius=0.2)
+    lim_ic = blast_cells(order, n, subcells=True, radius=0.2)
+    grid = oracle.LimitingGrid(s, n, periodic=True)
+    grid.set_solution(luh, lim_ic)
+    inner = (slice(g, g + L),) * 3
+
+    def cell(i, j, k):
+        return ((k % n) * n + (j % n)) * n + (i % n)
+
+    checked = irregular = 0
+    for step in range(4):
+        st0, dt = grid.status(), grid.dt()
+        luh0 = grid.solution().reshape(n ** 3, -1)
+        prev = {}
+
+        def rep(c):
+            if c not in prev:
+                if st0[c] >= A:
+                    prev[c] = grid.limiter_solution(c).reshape(S, S, S, D).copy()
+                else:
+                    prev[c] = oracle.project_on_fv(order, D, g, luh0[c]).reshape(S, S, S, D)
+            return prev[c]
+
+        fv0 = [c for c in range(n ** 3) if st0[c] >= A]
+        for c in fv0:
+            rep(c)
+            k, j, i = c // (n * n), (c // n) % n, c % n
+            for d in range(3):
+                for sd in (-1, 1):
+                    off = [0, 0, 0]
+                    off[d] = sd
+                    rep(cell(i + off[0], j + off[1], k + off[2]))
+        st, _, irr = grid.step(1)
+        irregular += int(irr[0])
+        for c in fv0:
+            if st[0][c] < A:
+                continue
+            k, j, i = c // (n * n), (c // n) % n, c % n
+            own = np.zeros((S, S, S, D))
+            own[inner] = rep(c)[inner]
+            own = own.ravel()
+            for d in range(3):
+                for sd in (-1, 1):
+                    off = [0, 0, 0]
+                    off[d] = sd
+                    nb = rep(cell(i + off[0], j + off[1], k + off[2]))
+                    own = oracle.fv_ghost_fill(L, g, D, own, nb, d, sd)
+            exp, _ = oracle.muscl_update(s, L, g, own, 1.0 / n, dt)
+            got = grid.limiter_solution(c).reshape(S, S, S, D)
+            assert np.array_equal(got[inner], exp.reshape(S, S, S, D)[inner]), (step, c)
+            checked += 1
+    assert checked > 0 and irregular > 0

@@ -395,14 +395,15 @@ def test_limiter_status_machine_injected_cell(oracle, helper_layers, cure):
 
 @pytest.mark.parametrize("periodic", [True, False])
 def test_irregular_steps_leave_quiet_cells_as_plain_dg(oracle, periodic):
-    """Rollback and local recomputation touch only the cells with a status
-    above 0 after the spreading (LS:1343-1436): a cell with status 0 before
-    and after a step, whose face neighbours all carried a prediction (status
-    below troubled) before it, ends the step -- regular or irregular -- with
-    exactly the solution the plain ADER-DG driver computes from the same
-    state and dt (bit for bit: the same kernels on the same data).  A rollback
-    or recomputation that also reached such cells, or a merge that used other
-    face data, fails it."""
+    """Rollback and local recomputation (LS:1343-1436) give the DG cells their
+    own DG update back: a cell that is a DG cell (status below A) before and
+    after a step, whose face neighbours all carried a prediction (status below
+    troubled) before it, ends the step -- regular or irregular -- with exactly
+    the solution the plain ADER-DG driver computes from the same state and dt
+    (bit for bit: the same kernels on the same data).  Status-0 cells are not
+    rolled back at all; helper cells are rolled back and recomputed to the same
+    values.  A rollback that also reached status-0 cells, a helper left at the
+    previous level, or a merge that used other face data fails it."""
     order, n = 3, 8
     s = oracle.make_solver(order)
     B = order + 1
@@ -434,7 +435,7 @@ def test_irregular_steps_leave_quiet_cells_as_plain_dg(oracle, periodic):
                     nb = nb.copy()
                     nb[tuple(edge)] = 0
                 quiet &= nb < T
-        cells = idx[quiet & (S0 == 0) & (st1.reshape(n, n, n) == 0)]
+        cells = idx[quiet & (S0 < 2) & (st1.reshape(n, n, n) < 2)]  # DG before and after (A = 2)
         assert len(cells) > 0
         assert np.array_equal(luh1[cells], ref[cells]), f"step {step}: {len(cells)} cells"
         counts.append(len(cells))

@@ -444,8 +444,9 @@ def test_irregular_steps_leave_quiet_cells_as_plain_dg(oracle, periodic):
 
 
 def test_fv_cells_step_from_the_previous_level(oracle):
-    """Every cell FV-active before and after a step (regular or irregular,
-    i.e. also after rollback and local recomputation) holds the MUSCL-Hancock
+    """Every cell FV-active before a step that keeps a patch after it
+    (regular or irregular, i.e. also after rollback and local recomputation;
+    a cell back to DG then holds the projection of it) holds the MUSCL-Hancock
     update (pinned separately) of its previous patch, with its face ghost
     layers filled (pinned separately) from the neighbours' previous FV
     representations -- their previous patch when FV-active, else the
@@ -465,7 +466,7 @@ def test_fv_cells_step_from_the_previous_level(oracle):
     def cell(i, j, k):
         return ((k % n) * n + (j % n)) * n + (i % n)
 
-    checked = irregular = 0
+    checked = irregular = to_dg = 0
     for step in range(4):
         st0, dt = grid.status(), grid.dt()
         luh0 = grid.solution().reshape(n ** 3, -1)
@@ -490,8 +491,9 @@ def test_fv_cells_step_from_the_previous_level(oracle):
                     rep(cell(i + off[0], j + off[1], k + off[2]))
         st, _, irr = grid.step(1)
         irregular += int(irr[0])
+        luh1 = grid.solution().reshape(n ** 3, -1)
         for c in fv0:
-            if st[0][c] < A:
+            if st[0][c] == 0:  # no patch kept
                 continue
             k, j, i = c // (n * n), (c // n) % n, c % n
             own = np.zeros((S, S, S, D))
@@ -506,5 +508,8 @@ def test_fv_cells_step_from_the_previous_level(oracle):
             exp, _ = oracle.muscl_update(s, L, g, own, 1.0 / n, dt)
             got = grid.limiter_solution(c).reshape(S, S, S, D)
             assert np.array_equal(got[inner], exp.reshape(S, S, S, D)[inner]), (step, c)
+            if st[0][c] < A:  # back to DG: its solution is the projection of that update
+                assert np.array_equal(luh1[c], oracle.project_on_dg(order, D, g, got.ravel())), (step, c)
+                to_dg += 1
             checked += 1
-    assert checked > 0 and irregular > 0
+    assert checked > 0 and to_dg > 0 and irregular > 0

@@ -446,7 +446,8 @@ def test_irregular_steps_leave_quiet_cells_as_plain_dg(oracle, periodic):
 def test_fv_cells_step_from_the_previous_level(oracle):
     """Every cell FV-active before a step that keeps a patch after it
     (regular or irregular, i.e. also after rollback and local recomputation;
-    a cell back to DG then holds the projection of it) holds the MUSCL-Hancock
+    a cell back to DG then holds the projection of it), and every cell that an
+    irregular change made FV-active, holds the MUSCL-Hancock
     update (pinned separately) of its previous patch, with its face ghost
     layers filled (pinned separately) from the neighbours' previous FV
     representations -- their previous patch when FV-active, else the
@@ -466,7 +467,7 @@ def test_fv_cells_step_from_the_previous_level(oracle):
     def cell(i, j, k):
         return ((k % n) * n + (j % n)) * n + (i % n)
 
-    checked = irregular = to_dg = 0
+    checked = irregular = to_dg = newly = 0
     for step in range(4):
         st0, dt = grid.status(), grid.dt()
         luh0 = grid.solution().reshape(n ** 3, -1)
@@ -492,7 +493,11 @@ def test_fv_cells_step_from_the_previous_level(oracle):
         st, _, irr = grid.step(1)
         irregular += int(irr[0])
         luh1 = grid.solution().reshape(n ** 3, -1)
-        for c in fv0:
+        # and, after an irregular change, the cells newly FV-active: rolled
+        # back to their previous DG solution, projected, recomputed
+        new_fv = [c for c in range(n ** 3) if irr[0] and st[0][c] >= A and st0[c] < A]
+        newly += len(new_fv)
+        for c in fv0 + new_fv:
             if st[0][c] == 0:  # no patch kept
                 continue
             k, j, i = c // (n * n), (c // n) % n, c % n
@@ -512,4 +517,4 @@ def test_fv_cells_step_from_the_previous_level(oracle):
                 assert np.array_equal(luh1[c], oracle.project_on_dg(order, D, g, got.ravel())), (step, c)
                 to_dg += 1
             checked += 1
-    assert checked > 0 and to_dg > 0 and irregular > 0
+    assert checked > 0 and to_dg > 0 and newly > 0 and irregular > 0

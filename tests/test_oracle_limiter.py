@@ -518,3 +518,36 @@ def test_fv_cells_step_from_the_previous_level(oracle):
                 to_dg += 1
             checked += 1
     assert checked > 0 and to_dg > 0 and newly > 0 and irregular > 0
+
+
+@pytest.mark.parametrize("periodic,helper_layers", [(True, 1), (False, 1), (True, 2), (False, 2)])
+def test_statuses_after_spreading_in_closed_form(oracle, periodic, helper_layers):
+    """After an irregular change the statuses are spread over T = 2h + 1
+    sweeps (Runner.cpp:889-896, LS:392-405): then every cell's status is
+    max(0, T - d), d its face-neighbour (L1) distance to the nearest troubled
+    cell (status T) -- with periodic wrap-around, or none across a
+    transmissive boundary.  Checked after every irregular step of a blast."""
+    order, n = 3, 8
+    s = oracle.make_solver(order)
+    nodes = np.array([oracle.table(oracle.T_NODES, order, i) for i in range(order + 1)])
+    luh = blast_cells(order, n, nodes=nodes, radius=0.2)
+    lim_ic = blast_cells(order, n, subcells=True, radius=0.2)
+    grid = oracle.LimitingGrid(s, n, periodic=periodic, helper_layers=helper_layers)
+    grid.set_solution(luh, lim_ic)
+    T = 2 * helper_layers + 1
+    K, J, I = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    checked = 0
+    for _ in range(6):
+        st, _, irr = grid.step(1)
+        if not irr[0]:
+            continue
+        S3 = st[0].reshape(n, n, n)
+        d = np.full((n, n, n), 10 ** 9)
+        for k, j, i in np.argwhere(S3 == T):
+            dk, dj, di = np.abs(K - k), np.abs(J - j), np.abs(I - i)
+            if periodic:
+                dk, dj, di = np.minimum(dk, n - dk), np.minimum(dj, n - dj), np.minimum(di, n - di)
+            d = np.minimum(d, dk + dj + di)
+        assert np.array_equal(S3, np.maximum(0, T - d))
+        checked += 1
+    assert checked > 0

@@ -491,39 +491,66 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
 //
 // One wavefront per cell (lane = node; N <= 3, so B^3 <= 64), four cells per
 // workgroup, synchronised per wavefront only.  The local min/max runs the
-// tensor contractions one variable at a time (tensor3_wave: the same sums in
+// tensor contractions one variable at a time (tensor3_rows: the same sums in
 // the same order as tensor3), so a wavefront's LDS holds one variable's
-// temporaries; the subcell values of the FV projection are formed again, per
-// variable, only for the cells that need the projection.
+// temporaries; the Lobatto and subcell values are folded into the extrema in
+// registers, and the subcell values of the FV projection are formed again,
+// per variable, only for the cells that need the projection.
 // ---------------------------------------------------------------------------
-template <int P, int M>
-__device__ inline void tensor3_wave(const double* in, const double* T, double* t1, double* t2, double* out,
-                                    int lane) {
-  for (int i = lane; i < P * P * M; i += kWave) {  // t1[z][y][X]
-    const int X = i % M, r = i / M;
-    const double* row = in + r * P;
-    double acc = 0.0;
+//
+// Each lane owns a row of a phase: in phase 1 lane (z,y) forms t1[z][y][X]
+// for all X, in phase 2 lane (z,X) forms t2[z][Y][X] for all Y, in phase 3
+// lane (Y,X) forms out[Z][Y][X] for all Z and hands each value to
+// emit(Z, Y*M + X, value) in registers.  A lane reads its P inputs once for
+// M outputs, and the table index is uniform in every loop, so the table
+// entries are scalar loads from constant memory: the status kernel is bound
+// by LDS instruction issue, and an output per lane (P input and P table reads
+// each) took three times the LDS instructions.  Every output is the sum
+// 0 + in[0]*T[0][.] + ... + in[P-1]*T[P-1][.] of tensor3, in that order.
+template <int N, bool LOB, int P, int M, class Emit>
+__device__ inline void tensor3_rows(const double* in, double* t1, double* t2, int lane, Emit&& emit) {
+  static_assert(P * P <= kWave && P * M <= kWave && M * M <= kWave, "one row per lane");
+  const double* const T = LOB ? c_lim.uh2lob[N] : c_lim.uh2lim[N];
+  if (lane < P * P) {  // t1[z][y][X], lane = z*P + y
+    double row[P];
 #pragma unroll
-    for (int x = 0; x < P; ++x) acc += row[x] * T[x * M + X];
-    t1[i] = acc;
+    for (int x = 0; x < P; ++x) row[x] = in[lane * P + x];
+#pragma unroll
+    for (int X = 0; X < M; ++X) {
+      double acc = 0.0;
+#pragma unroll
+      for (int x = 0; x < P; ++x) acc += row[x] * T[x * M + X];
+      t1[lane * M + X] = acc;
+    }
   }
   wave_lds_sync();
-  for (int i = lane; i < P * M * M; i += kWave) {  // t2[z][Y][X]
-    const int X = i % M, Y = (i / M) % M, z = i / (M * M);
-    double acc = 0.0;
+  if (lane < P * M) {  // t2[z][Y][X], lane = z*M + X
+    const int z = lane / M, X = lane % M;
+    double col[P];
 #pragma unroll
-    for (int y = 0; y < P; ++y) acc += t1[(z * P + y) * M + X] * T[y * M + Y];
-    t2[i] = acc;
+    for (int y = 0; y < P; ++y) col[y] = t1[(z * P + y) * M + X];
+#pragma unroll
+    for (int Y = 0; Y < M; ++Y) {
+      double acc = 0.0;
+#pragma unroll
+      for (int y = 0; y < P; ++y) acc += col[y] * T[y * M + Y];
+      t2[(z * M + Y) * M + X] = acc;
+    }
   }
   wave_lds_sync();
-  for (int i = lane; i < M * M * M; i += kWave) {  // out[Z][Y][X]
-    const int YX = i % (M * M), Z = i / (M * M);
-    double acc = 0.0;
+  if (lane < M * M) {  // out[Z][Y][X], lane = Y*M + X
+    double col[P];
 #pragma unroll
-    for (int z = 0; z < P; ++z) acc += t2[z * M * M + YX] * T[z * M + Z];
-    out[i] = acc;
+    for (int z = 0; z < P; ++z) col[z] = t2[z * M * M + lane];
+#pragma unroll
+    for (int Z = 0; Z < M; ++Z) {
+      double acc = 0.0;
+#pragma unroll
+      for (int z = 0; z < P; ++z) acc += col[z] * T[z * M + Z];
+      emit(Z, lane, acc);
+    }
   }
-  wave_lds_sync();
+  wave_lds_sync();  // t1, t2 are rewritten by the next call
 }
 
 constexpr int kStatusCells = 4;  // cells (wavefronts) per workgroup of the status kernel
@@ -533,20 +560,17 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
   using Sh = LimShape<N>;
   constexpr int B = Sh::B, B2 = B * B, B3 = Sh::B3, L = Sh::L, L3 = Sh::L3, V = Sh::V, PATCH = Sh::PATCH;
   static_assert(B3 <= kWave, "one node per lane");
-  constexpr int T1 = B * B * L > B3 ? B * B * L : B3, T2 = B * L * L > B3 ? B * L * L : B3;
-  constexpr int PER = V * B3 + T1 + T2 + L3 + B3;  // sQ, t1, t2, subcells, Lobatto values
-  __shared__ double sUh2lim[B * L], sUh2lob[B * B];
+  constexpr int T1 = B * B * L, T2 = B * L * L;  // the subcell transform's; the Lobatto one's are smaller
+  constexpr int PER = V * B3 + T1 + T2 + 12 * V;  // sQ, t1, t2, face extrema
   __shared__ double sWave[kStatusCells * PER];
-  load_tables<N>(sUh2lim, sUh2lob, nullptr);
-  __syncthreads();  // the only workgroup barrier: before any wavefront can leave
+  // no workgroup barrier: each wavefront works in its own part of sWave
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
   const int c = blockIdx.x * kStatusCells + wid;
   if (c >= a.ncells) return;
   double* const sQ = sWave + wid * PER;
   double* const t1 = sQ + V * B3;
   double* const t2 = t1 + T1;
-  double* const sSub = t2 + T2;
-  double* const sLob = sSub + L3;
+  double* const sBnd = t2 + T2;
   // loads whose latency the DG update hides: the merged face extrema (one
   // lane per face-block entry) and the neighbours' previous statuses (one
   // lane per face)
@@ -630,19 +654,17 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
   double mn[V], mx[V];
 #pragma unroll 1
   for (int v = 0; v < V; ++v) {
-    tensor3_wave<B, B>(sQ + v * B3, sUh2lob, t1, t2, sLob, lane);
-    tensor3_wave<B, L>(sQ + v * B3, sUh2lim, t1, t2, sSub, lane);
     double lo = DBL_MAX, hi = -DBL_MAX;
-    for (int i = lane; i < B3; i += kWave) {
-      lo = dmin(lo, sQ[v * B3 + i]);
-      hi = dmax(hi, sQ[v * B3 + i]);
-      lo = dmin(lo, sLob[i]);
-      hi = dmax(hi, sLob[i]);
+    if (active) {  // B3 <= kWave: one node per lane
+      lo = dmin(lo, sQ[v * B3 + lane]);
+      hi = dmax(hi, sQ[v * B3 + lane]);
     }
-    for (int i = lane; i < L3; i += kWave) {
-      lo = dmin(lo, sSub[i]);
-      hi = dmax(hi, sSub[i]);
-    }
+    auto fold = [&](int, int, double u) {
+      lo = dmin(lo, u);
+      hi = dmax(hi, u);
+    };
+    tensor3_rows<N, true, B, B>(sQ + v * B3, t1, t2, lane, fold);   // Lobatto nodes
+    tensor3_rows<N, false, B, L>(sQ + v * B3, t1, t2, lane, fold);  // subcell means
     lo = wave_min(lo);
     hi = -wave_min(-hi);
     // a register array indexed by the rolled v would live in scratch memory:
@@ -661,9 +683,9 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
     }
   }
   // DMP (LIM:114-165) against the merged boundary extrema, staged in LDS
-  // (sSub is free until the helper projection below)
   double* bmn = a.mn + (size_t)c * 6 * V;
   double* bmx = a.mx + (size_t)c * 6 * V;
+  double* const sSub = sBnd;
   if (lane < 6 * V) sSub[lane] = bnd;
   if (lane >= 32 && lane < 32 + 6 * V) sSub[6 * V + lane - 32] = bnd;
   wave_lds_sync();
@@ -687,10 +709,18 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
   if (ok) ok = !(mn[0] <= 0.0) && !(mn[4] <= 0.0);
   if (lane == 0) {  // every lane has its face-block values in registers
     if (s >= a.A) {  // determineLimiterMinAndMax: patch extrema on all faces
+      double pmn[V], pmx[V];  // loaded once, before any store
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        pmn[v] = a.limmm[(size_t)c * 2 * V + v];
+        pmx[v] = a.limmm[(size_t)c * 2 * V + V + v];
+      }
+#pragma unroll
       for (int f = 0; f < 6; ++f)
+#pragma unroll
         for (int v = 0; v < V; ++v) {
-          bmn[f * V + v] = a.limmm[(size_t)c * 2 * V + v];
-          bmx[f * V + v] = a.limmm[(size_t)c * 2 * V + V + v];
+          bmn[f * V + v] = pmn[v];
+          bmx[f * V + v] = pmx[v];
         }
     } else {
       for (int v = 0; v < V; ++v) {
@@ -731,15 +761,12 @@ __global__ __launch_bounds__(kStatusCells * kWave) void lim_update_status_kernel
   if (helper || (fresh && ns < a.A)) {
     double* const dst = patch_ptr(a, c, buf, PATCH);
     constexpr int G = Sh::G, S = Sh::S;
+    const int yy = lane / L, xx = lane % L;  // the lane's subcell row (lanes < L*L)
 #pragma unroll 1
-    for (int v = 0; v < V; ++v) {
-      tensor3_wave<B, L>(sQ + v * B3, sUh2lim, t1, t2, sSub, lane);
-      for (int n = lane; n < L3; n += kWave) {
-        const int zz = n / (L * L), yy = (n / L) % L, xx = n % L;
-        dst[(((zz + G) * S + (yy + G)) * S + (xx + G)) * V + v] = sSub[n];
-      }
-      wave_lds_sync();  // sSub is rewritten by the next variable
-    }
+    for (int v = 0; v < V; ++v)
+      tensor3_rows<N, false, B, L>(sQ + v * B3, t1, t2, lane, [&](int zz, int, double u) {
+        dst[(((zz + G) * S + (yy + G)) * S + (xx + G)) * V + v] = u;
+      });
   }
   if (lane == 0) {
     a.st[c] = ns;

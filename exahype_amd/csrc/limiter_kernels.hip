@@ -38,7 +38,7 @@ __constant__ LimTables c_lim;
 static unsigned blocks256(long n) { return (unsigned)((n + 255) / 256); }
 
 constexpr int kLimNT = 256;     // per-cell kernels
-constexpr int kMusclNT = 384;   // MUSCL-Hancock patch kernel (LDS allows one per CU; six wavefronts)
+constexpr int kMusclNT = 448;   // MUSCL-Hancock patch kernel (LDS allows one per CU; seven wavefronts)
 
 template <int N>
 struct LimShape {

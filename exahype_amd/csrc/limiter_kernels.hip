@@ -507,10 +507,13 @@ __global__ __launch_bounds__(kMusclNT) void lim_fv_update_kernel(LimArgs a, int 
 // by LDS instruction issue, and an output per lane (P input and P table reads
 // each) took three times the LDS instructions.  Every output is the sum
 // 0 + in[0]*T[0][.] + ... + in[P-1]*T[P-1][.] of tensor3, in that order.
-template <int N, bool LOB, int P, int M, class Emit>
+enum { kTabUh2Lim, kTabUh2Lob, kTabLim2Uh };  // the table of tensor3_rows
+
+template <int N, int TAB, int P, int M, class Emit>
 __device__ inline void tensor3_rows(const double* in, double* t1, double* t2, int lane, Emit&& emit) {
   static_assert(P * P <= kWave && P * M <= kWave && M * M <= kWave, "one row per lane");
-  const double* const T = LOB ? c_lim.uh2lob[N] : c_lim.uh2lim[N];
+  const double* const T =
+      TAB == kTabUh2Lob ? c_lim.uh2lob[N] : (TAB == kTabLim2Uh ? c_lim.lim2uh[N] : c_lim.uh2lim[N]);
   if (lane < P * P) {  // t1[z][y][X], lane = z*P + y
     double row[P];
 #pragma unroll
@@ -665,8 +668,8 @@ __global__ __launch_bounds__(kStatusCells * kWave, 5) void lim_update_status_ker
       lo = dmin(lo, u);
       hi = dmax(hi, u);
     };
-    tensor3_rows<N, true, B, B>(sQ + v * B3, t1, t2, lane, fold);   // Lobatto nodes
-    tensor3_rows<N, false, B, L>(sQ + v * B3, t1, t2, lane, fold);  // subcell means
+    tensor3_rows<N, kTabUh2Lob, B, B>(sQ + v * B3, t1, t2, lane, fold);  // Lobatto nodes
+    tensor3_rows<N, kTabUh2Lim, B, L>(sQ + v * B3, t1, t2, lane, fold);  // subcell means
     lo = wave_min(lo);
     hi = -wave_min(-hi);
     // a register array indexed by the rolled v would live in scratch memory:
@@ -766,7 +769,7 @@ __global__ __launch_bounds__(kStatusCells * kWave, 5) void lim_update_status_ker
     const int yy = lane / L, xx = lane % L;  // the lane's subcell row (lanes < L*L)
 #pragma unroll 1
     for (int v = 0; v < V; ++v)
-      tensor3_rows<N, false, B, L>(sQ + v * B3, t1, t2, lane, [&](int zz, int, double u) {
+      tensor3_rows<N, kTabUh2Lim, B, L>(sQ + v * B3, t1, t2, lane, [&](int zz, int, double u) {
         dst[(((zz + G) * S + (yy + G)) * S + (xx + G)) * V + v] = u;
       });
   }
@@ -895,69 +898,123 @@ __global__ __launch_bounds__(kLimNT) void lim_rollback_kernel(LimArgs a) {
 // were not rolled back: their solution is the one the status kernel saw, so
 // its dt candidate and local min/max are reused (lim_recompute_light_kernel);
 // this kernel walks the cells with status > 0 (the rollback's list).
+//
+// One wavefront per cell (lane = node, N <= 3), four cells per workgroup,
+// the contractions row-owned as in the status kernel (tensor3_rows, the same
+// sums in the same order as tensor3); each wavefront's LDS holds the cell's
+// solution [var][node] and one variable's temporaries.
+constexpr int kRecomputeCells = 4;  // cells (wavefronts) per workgroup of the recompute kernel
+
 template <int N>
-__global__ __launch_bounds__(kLimNT) void lim_recompute_kernel(LimArgs a) {
+__global__ __launch_bounds__(kRecomputeCells * kWave) void lim_recompute_kernel(LimArgs a) {
   using Sh = LimShape<N>;
-  constexpr int B3 = Sh::B3, L3 = Sh::L3, V = Sh::V, PATCH = Sh::PATCH, L = Sh::L, B = Sh::B;
-  __shared__ double sQ[V * B3], sLob[V * B3], sSub[V * L3];
-  __shared__ double t1[Sh::TT], t2[Sh::TT];
-  __shared__ double sUh2lim[B * L], sUh2lob[B * B], sLim2uh[B * L];
-  __shared__ double red[2 * V * (kLimNT / kWave)];
+  constexpr int B = Sh::B, B2 = B * B, B3 = Sh::B3, L = Sh::L, L3 = Sh::L3, V = Sh::V, PATCH = Sh::PATCH;
+  constexpr int G = Sh::G, S = Sh::S;
+  static_assert(B3 <= kWave, "one node per lane");
+  constexpr int TT = B * L * L;              // the largest phase temporary of the three transforms
+  constexpr int PER = V * B3 + 2 * TT + L3;  // sQ, t1, t2, one variable's subcell values
+  __shared__ double sWave[kRecomputeCells * PER];
   if (!a.flags[kLimFlagIrregular]) return;
-  const int tid = threadIdx.x;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+  double* const sQ = sWave + wid * PER;
+  double* const t1 = sQ + V * B3;
+  double* const t2 = t1 + TT;
+  double* const sIn = t2 + TT;
   const int count = a.flags[kLimFlagRcCount];
-  load_tables<N>(sUh2lim, sUh2lob, sLim2uh);
   const double invDx[3] = {1.0 / a.dx[0], 1.0 / a.dx[1], 1.0 / a.dx[2]};
-  for (int it = blockIdx.x; it < count; it += gridDim.x) {
+  const bool active = lane < B3;
+  const int node = active ? lane : 0;
+  for (int it = blockIdx.x * kRecomputeCells + wid; it < count; it += gridDim.x * kRecomputeCells) {
     const int c = a.rc_list[it];
     const int s = a.st[c];
-    __syncthreads();  // tables loaded / the previous cell's LDS use is over
-    if (s > 0 && s < a.A) {
-      if (a.pst[c] < a.A) {
-        double* luh = a.luh + (size_t)c * B3 * V;
-        double* prev = a.luh_prev + (size_t)c * B3 * V;
-        for (int i = tid; i < B3 * V; i += kLimNT) {
-          const double t = luh[i];
-          luh[i] = prev[i];
-          prev[i] = t;
-        }
-        // the FV projection follows from the min/max pass below (sSub)
-      } else {
-        const int buf = a.cur[c] ^ 1;
-        double mn[V], mx[V];
-        fv_to_dg<N, kLimNT>(patch_ptr(a, c, buf, PATCH), a.luh + (size_t)c * B3 * V, sSub, t1, t2, sQ, sLim2uh, red,
-                            mn, mx);
-        if (tid == 0) a.cur[c] = (uint8_t)buf;
+    const bool helper = s > 0 && s < a.A;
+    const bool from_dg = helper && a.pst[c] < a.A;  // rolled back to its previous DG solution
+    double* const luh = a.luh + (size_t)c * B3 * V;
+    if (from_dg) {
+      double* const prev = a.luh_prev + (size_t)c * B3 * V;
+      for (int i = lane; i < B3 * V; i += kWave) {
+        const double t = luh[i], p = prev[i];
+        luh[i] = p;
+        prev[i] = t;
+        sQ[(i % V) * B3 + i / V] = p;
       }
+    } else if (helper) {  // the FV patch of the other buffer projected to DG
+      const int buf = a.cur[c] ^ 1;
+      const double* const patch = patch_ptr(a, c, buf, PATCH);
+#pragma unroll 1
+      for (int v = 0; v < V; ++v) {
+        for (int i = lane; i < L3; i += kWave) {
+          const int zz = i / (L * L), yy = (i / L) % L, xx = i % L;
+          sIn[i] = patch[(((zz + G) * S + (yy + G)) * S + (xx + G)) * V + v];
+        }
+        wave_lds_sync();
+        tensor3_rows<N, kTabLim2Uh, L, B>(sIn, t1, t2, lane, [&](int Z, int YX, double u) {
+          luh[(Z * B2 + YX) * V + v] = u;
+          sQ[v * B3 + Z * B2 + YX] = u;
+        });
+      }
+      if (lane == 0) a.cur[c] = (uint8_t)buf;
+    } else {
+      for (int i = lane; i < B3 * V; i += kWave) sQ[(i % V) * B3 + i / V] = luh[i];
     }
-    __syncthreads();
-    const double* luh = a.luh + (size_t)c * B3 * V;
-    for (int i = tid; i < B3 * V; i += kLimNT) sQ[(i % V) * B3 + i / V] = luh[i];
-    __syncthreads();
-    const bool active = tid < B3;
-    const int node = active ? tid : 0;
-    {
+    wave_lds_sync();
+    {  // startNewTimeStep
       double q[V];
 #pragma unroll
       for (int m = 0; m < V; ++m) q[m] = sQ[m * B3 + node];
       double dtn = active ? node_stable_dt<Euler<0>>(q, invDx, a.cfl_pnpm) : DBL_MAX;
       dtn = dtn < DBL_MAX ? dtn : DBL_MAX;
-      dtn = block_min<kLimNT>(dtn, red);
-      if (tid == 0) atomic_min_positive(&a.st_state->dt_slots[c % kDtSlots], dtn);
+      dtn = wave_min(dtn);
+      if (lane == 0) atomic_min_positive(&a.st_state->dt_slots[c % kDtSlots], dtn);
     }
-    __syncthreads();
+    // determineMinAndMax over Gauss-Legendre nodes, Lobatto nodes and subcell
+    // means; a helper rolled back to DG also gets its subcell values as its
+    // patch interior
+    double* const dst = from_dg ? patch_ptr(a, c, a.cur[c], PATCH) : nullptr;
+    const int yy = lane / L, xx = lane % L;  // the lane's subcell column in the last phase
     double mn[V], mx[V];
-    dg_local_minmax<N>(sQ, sLob, sSub, t1, t2, sUh2lim, sUh2lob, red, mn, mx);
-    if (s > 0 && s < a.A && a.pst[c] < a.A) store_interior<N>(patch_ptr(a, c, a.cur[c], PATCH), sSub);
-    if (tid == 0) {
+#pragma unroll 1
+    for (int v = 0; v < V; ++v) {
+      double lo = DBL_MAX, hi = -DBL_MAX;
+      if (active) {
+        lo = dmin(lo, sQ[v * B3 + lane]);
+        hi = dmax(hi, sQ[v * B3 + lane]);
+      }
+      tensor3_rows<N, kTabUh2Lob, B, B>(sQ + v * B3, t1, t2, lane, [&](int, int, double u) {
+        lo = dmin(lo, u);
+        hi = dmax(hi, u);
+      });
+      tensor3_rows<N, kTabUh2Lim, B, L>(sQ + v * B3, t1, t2, lane, [&](int zz, int, double u) {
+        lo = dmin(lo, u);
+        hi = dmax(hi, u);
+        if (from_dg) dst[(((zz + G) * S + (yy + G)) * S + (xx + G)) * V + v] = u;
+      });
+      lo = wave_min(lo);
+      hi = -wave_min(-hi);
+#pragma unroll
+      for (int w = 0; w < V; ++w) {  // no register array indexed by the rolled v
+        mn[w] = (w == v) ? lo : mn[w];
+        mx[w] = (w == v) ? hi : mx[w];
+      }
+    }
+    if (lane == 0) {
       double* bmn = a.mn + (size_t)c * 6 * V;
       double* bmx = a.mx + (size_t)c * 6 * V;
+      double fmn[V], fmx[V];  // loaded once, before any store
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        fmn[v] = (s < a.A) ? mn[v] : a.limmm[(size_t)c * 2 * V + v];
+        fmx[v] = (s < a.A) ? mx[v] : a.limmm[(size_t)c * 2 * V + V + v];
+      }
+#pragma unroll
       for (int f = 0; f < 6; ++f)
+#pragma unroll
         for (int v = 0; v < V; ++v) {
-          bmn[f * V + v] = (s < a.A) ? mn[v] : a.limmm[(size_t)c * 2 * V + v];
-          bmx[f * V + v] = (s < a.A) ? mx[v] : a.limmm[(size_t)c * 2 * V + V + v];
+          bmn[f * V + v] = fmn[v];
+          bmx[f * V + v] = fmx[v];
         }
     }
+    wave_lds_sync();  // sQ is rewritten for the next cell
   }
 }
 
@@ -1379,8 +1436,9 @@ hipError_t launch_lim_recompute_finish(const SolverDesc& sd, const LimArgs& a, h
   return dispatch_lim(sd.order, [&]<int N>() -> hipError_t {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const long g = 8L * cus;
-    hipLaunchKernelGGL(lim_recompute_kernel<N>, dim3((unsigned)(g < a.ncells ? g : a.ncells)), dim3(kLimNT), 0, s, a);
+    const long g = 8L * cus;  // workgroups of kRecomputeCells cells each
+    hipLaunchKernelGGL(lim_recompute_kernel<N>, dim3((unsigned)(g < a.ncells ? g : a.ncells)),
+                       dim3(kRecomputeCells * kWave), 0, s, a);
     return hipGetLastError();
   });
 }

@@ -91,8 +91,11 @@ struct PredictorLayout {
   // rolled loop needs fewer registers (N=5: 110 instead of 166, one more
   // wavefront per SIMD) at the cost of uniform table loads per slice
   // (measured: N=4 -2%, N=5 -4% predictor time; at N=6 the rolled form was
-  // 19% slower, so N=6 stays unrolled).
-  static constexpr bool ROLLED = (B >= 7 && (IS_FP64 || B >= 8)) || V >= 9 || (!IS_FP64 && (B == 5 || B == 6));
+  // 19% slower, so N=6 stays unrolled), and the double form of the
+  // 5-variable PDE at N=3, where the rolled loop leaves room for the
+  // accumulated time averages (the predictor kernel's ACC_FH).
+  static constexpr bool ROLLED = (B >= 7 && (IS_FP64 || B >= 8)) || V >= 9 || (!IS_FP64 && (B == 5 || B == 6)) ||
+                                 (IS_FP64 && B == 4 && V == 5);
   // 8+ waves per workgroup leave 256 registers per lane, no room for slices;
   // a rolled loop cannot index registers.
   static constexpr bool WSQ = NQR > 0 && (NT >= 8 * kWave || ROLLED);
@@ -230,8 +233,12 @@ __global__ __launch_bounds__(node_threads(B), (predictor_min_waves<B, TERMS, T>(
   // 8, where they fit the wavefronts per SIMD that LDS allows (B = 6 and 8
   // with the register cap of predictor_min_waves: 124 and 166 registers; the
   // epilogue's B flux evaluations go, 4% of the predictor at 81^3 N=5; B = 3
-  // and 4 would drop a wavefront per SIMD, B = 7 spills under the cap)
-  constexpr bool ACC_FH = FLOAT_FLUX_ONLY && (B == 5 || B == 6 || B == 8);
+  // and 4 would drop a wavefront per SIMD, B = 7 spills under the cap), and
+  // the double form of the 5-variable PDE at B = 4 with its slice loop rolled
+  // (170 registers, the two wavefronts per SIMD its LDS allows; 3.44 -> 3.36
+  // ms at 81^3 in the limiter workload, where one iteration is the rule)
+  constexpr bool ACC_FH = (FLOAT_FLUX_ONLY && (B == 5 || B == 6 || B == 8)) ||
+                          (IS_FP64 && USE_FLUX && !USE_SRC_NCP && B == 4 && V == 5);
   T fh[3][V];
   // Float form of a PDE without fluxes (source and/or NCP only: the elastic
   // PDE): the time-averaged source term likewise accumulates during the
